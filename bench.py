@@ -1,0 +1,217 @@
+"""Benchmark of the Fire-X fixed step on one MI355X.
+
+Workload (default "uhd-mixed"): 192 x 288 x 192 grid (10,616,832 cells), 96
+primary + 96 residual Jacobi sweeps per step, dt = 1/60, species stored f16
+(velocity, temperature, pressure and all arithmetic f32), three seeded
+Gaussian emitters of radius 6-14 cells near the grid base, Inferno scene
+parameters, zero initial velocity.  Inputs are synthetic and resident on the
+device before the timed region.
+
+Output: one JSON line with cell-updates/s (cells x steps / wall time of the
+timed steps), the roofline of the dominant kernel (HIP-event timing of its
+launches on the solver's stream), per-pass device times, the plain f32
+reference-structured GPU path for comparison, and the FP32 CPU oracle
+timed on the host's cores over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published
+DT = 1.0 / 60.0
+
+# Algorithmic (compulsory) bytes per cell of each pass, f32 reference
+# structure: simulate reads flow 16 + species 16 and writes flow 16 +
+# species 16 + reaction 4 (no liquid input); divergence 16 + 4; one Jacobi
+# sweep reads p and div and writes p; projection reads flow 16 + p 4 and
+# writes flow 16; the fused projection also writes div 4; copyFlow 32.
+BYTES_PER_CELL = {
+    "simulate_f32": 68, "simulate_f16": 52, "divergence": 20, "jacobi_sweep": 12,
+    "project": 36, "project_divergence": 40, "fill_copy": 32,
+}
+
+WORKLOADS = {
+    "uhd-mixed": dict(dims=(192, 288, 192), iters=(96, 96), storage="f16", emitters=3, radius=(6.0, 14.0),
+                      seed=2026, jitter=0.1, velocity_noise=0.0),
+    "hd-plain": dict(dims=(128, 192, 128), iters=(96, 96), storage="f32", emitters=1, radius=(8 * 2 ** 0.5,) * 2,
+                     seed=2026, jitter=0.1, velocity_noise=0.0),
+    "cpu-plain": dict(dims=(64, 96, 64), iters=(24, 24), storage="f32", emitters=1, radius=(4 * 2 ** 0.5,) * 2,
+                      seed=2026, jitter=0.1, velocity_noise=0.0),
+}
+
+
+def make_solver(wl: dict, plan: str, sweeps: int = 0):
+    from fire_amd import FireXHipSolver, seeded_emitters
+
+    s = FireXHipSolver(wl["dims"], tier="uhd", pressure_iterations=wl["iters"][0],
+                       correction_iterations=wl["iters"][1], species_storage=wl["storage"], plan=plan,
+                       heavy_mode="off", jacobi_sweeps_per_launch=sweeps)
+    em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
+    s.set_emitters(em, jitter=wl["jitter"], velocity_noise=wl["velocity_noise"], seed=wl["seed"])
+    return s
+
+
+def time_steps(solver, steps: int, warmup: int) -> float:
+    solver.step(DT, warmup)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.step(DT, steps)
+    solver.sync()
+    return time.perf_counter() - t0
+
+
+def pass_profile(solver, steps: int, cells: int, storage: str) -> dict:
+    solver.set_pass_timing(True)
+    solver.step(DT, steps)
+    t = solver.pass_timing()
+    solver.set_pass_timing(False)
+    out = {}
+    for name in ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy"):
+        ms, n = t[name]["ms"], t[name]["launches"]
+        if n == 0:
+            continue
+        if name == "simulate":
+            bpc = BYTES_PER_CELL["simulate_f16" if storage != "f32" else "simulate_f32"]
+            units = n
+        elif name == "jacobi":
+            bpc = BYTES_PER_CELL["jacobi_sweep"]
+            units = t["jacobi_sweeps"]
+        else:
+            bpc = BYTES_PER_CELL[name]
+            units = n
+        gbs = bpc * cells * units / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[name] = {"ms_per_step": ms / t["steps"], "launches_per_step": n / t["steps"],
+                     "avg_launch_us": ms / n * 1e3, "algorithmic_GBps": round(gbs, 1)}
+    out["jacobi_sweeps_per_step"] = t["jacobi_sweeps"] / max(1, t["steps"])
+    out["_raw"] = t
+    return out
+
+
+def cpu_baseline(wl: dict, sample_steps: int) -> dict:
+    from fire_amd import seeded_emitters
+    from oracle import oracle as orc
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = orc.Oracle(wl["dims"], precision=32, pressure_iterations=wl["iters"][0],
+                   correction_iterations=wl["iters"][1])
+    o.set_bool("cinematic", True)
+    o.set_heavy_mode(orc.HEAVY_OFF)
+    if wl["storage"] != "f32":
+        o.set_species_storage({"f16": orc.SPECIES_F16, "bf16": orc.SPECIES_BF16}[wl["storage"]])
+        o.reset()
+    em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
+    o.set_emitters(em, wl["jitter"], wl["velocity_noise"], wl["seed"])
+    o.step(DT, 1)  # warm-up (page-in)
+    t0 = time.perf_counter()
+    o.step(DT, sample_steps)
+    el = time.perf_counter() - t0
+    cells = wl["dims"][0] * wl["dims"][1] * wl["dims"][2]
+    o.close()
+    return {"value": cells * sample_steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{sample_steps} steps of the same workload after 1 warm-up step, FP32 CPU oracle "
+                      f"(OpenMP over z, {cores} threads), {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="uhd-mixed", choices=sorted(WORKLOADS))
+    ap.add_argument("--plan", default="fused", choices=["fused", "reference"])
+    ap.add_argument("--sweeps", type=int, default=0, help="Jacobi sweeps per launch (0 = tuned default)")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=60, help="steps timed on the plain f32 path (0 = skip)")
+    ap.add_argument("--cpu-sample-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the Fire-X solver here is a single-GPU build "
+              "(a plume would shard by z slabs across GPUs; that path is not built).", file=sys.stderr)
+        return 2
+
+    from fire_amd import device_count
+
+    if device_count() < 1:
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    wl = WORKLOADS[args.workload]
+    cells = wl["dims"][0] * wl["dims"][1] * wl["dims"][2]
+    print(f"bench: workload {args.workload} {wl['dims']} plan {args.plan} steps {args.steps} warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+
+    solver = make_solver(wl, args.plan, args.sweeps)
+    elapsed = time_steps(solver, args.steps, args.warmup)
+    ms_per_step = elapsed / args.steps * 1e3
+    value = cells * args.steps / elapsed
+    print(f"bench: timed {args.steps} steps in {elapsed:.3f} s ({ms_per_step:.3f} ms/step)", file=sys.stderr, flush=True)
+    passes = pass_profile(solver, args.profile_steps, cells, wl["storage"])
+    t_fuel = solver.reduce("sum_fuel")
+    solver.close()
+
+    jac = passes.get("jacobi", {})
+    raw = passes.pop("_raw")
+    sweeps_per_launch = raw["jacobi_sweeps"] / max(1, raw["jacobi"]["launches"])
+    avg_launch_s = jac.get("avg_launch_us", 0.0) * 1e-6
+    bytes_per_launch = BYTES_PER_CELL["jacobi_sweep"] * cells * sweeps_per_launch
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "kernel": f"k_jacobi_tb (Jacobi, {sweeps_per_launch:g} sweeps per launch)",
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "avg_launch_us": round(jac.get("avg_launch_us", 0.0), 2)}
+
+    plain = None
+    if args.plain_steps > 0:
+        print("bench: plain f32 reference-structured path", file=sys.stderr, flush=True)
+        pw = dict(wl, storage="f32")
+        ps = make_solver(pw, "reference")
+        pel = time_steps(ps, args.plain_steps, 5)
+        ps.close()
+        plain = {"cell_updates_per_s": cells * args.plain_steps / pel, "ms_per_step": pel / args.plain_steps * 1e3,
+                 "steps": args.plain_steps}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (FP32 oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(wl, args.cpu_sample_steps)
+
+    line = {
+        "metric": "cell-updates/s at 192x288x192 (96+96 Jacobi, f16 species)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": list(wl["dims"]), "cells": cells,
+                   "jacobi": list(wl["iters"]), "species_storage": wl["storage"], "dt": DT,
+                   "emitters": wl["emitters"], "seed": wl["seed"], "plan": args.plan,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "passes": passes,
+        "plain_f32_gpu": plain,
+        "multiple_over_plain_f32": (value / plain["cell_updates_per_s"]) if plain else None,
+        "check": {"sum_fuel_after_run": t_fuel},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,397 @@
+// firex_host.cpp — host-side logic of the Fire-X solver library: tier tables,
+// memory model and preflight, offline timeline, fixed-step clock, parameter
+// clamps, uniform packing and the per-step constants handed to the kernels.
+#include "firex_host.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+namespace fx {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+// ------------------------------------------------------------------ params
+static double finite_or(double v, double fallback) { return std::isfinite(v) ? v : fallback; }
+static double clampd(double v, double lo, double hi) { return std::max(lo, std::min(hi, v)); }
+static double detail_target(double t) { return t >= 768 ? 1024 : t >= 384 ? 512 : t >= 192 ? 256 : 0; }
+
+bool params_set(Params& p, const char* name, double v) {
+  const std::string n(name ? name : "");
+  if (n == "burnerSize") p.burnerSize = clampd(finite_or(v, p.burnerSize), 0.35, 3.2);
+  else if (n == "burnerDepth") p.burnerDepth = clampd(finite_or(v, p.burnerDepth), 0.35, 3.2);
+  else if (n == "sourceThickness") p.sourceThickness = clampd(finite_or(v, p.sourceThickness), 0.35, 2.5);
+  else if (n == "firePower") p.firePower = clampd(finite_or(v, p.firePower), 0.1, 4);
+  else if (n == "sourceLift") p.sourceLift = clampd(finite_or(v, p.sourceLift), 0, 4);
+  else if (n == "buoyancyScale") p.buoyancyScale = clampd(finite_or(v, p.buoyancyScale), 0, 4);
+  else if (n == "flamePersistence") p.flamePersistence = clampd(finite_or(v, p.flamePersistence), 0.4, 3);
+  else if (n == "airEntrainment") p.airEntrainment = clampd(finite_or(v, p.airEntrainment), 0, 3);
+  else if (n == "fuelRate") p.fuelRate = finite_or(v, p.fuelRate);
+  else if (n == "oxygenRate") p.oxygenRate = finite_or(v, p.oxygenRate);
+  else if (n == "heatEfficiency") p.heatEfficiency = finite_or(v, p.heatEfficiency);
+  else if (n == "vorticity") p.vorticity = finite_or(v, p.vorticity);
+  else if (n == "waterFlow") p.waterFlow = finite_or(v, p.waterFlow);
+  else if (n == "sprayAngle") p.sprayAngle = finite_or(v, p.sprayAngle);
+  else if (n == "cameraYaw") p.cameraYaw = finite_or(v, p.cameraYaw);
+  else if (n == "cameraPitch") p.cameraPitch = finite_or(v, p.cameraPitch);
+  else if (n == "cameraDistance") p.cameraDistance = finite_or(v, p.cameraDistance);
+  else if (n == "viewMode") p.viewMode = clampd(std::round(finite_or(v, p.viewMode)), 0, 7);
+  else if (n == "aimHeight") p.aimHeight = clampd(finite_or(v, p.aimHeight), 0.08, 0.82);
+  else if (n == "nozzleType") p.nozzleType = finite_or(v, p.nozzleType) >= 0.5 ? 1 : 0;
+  else if (n == "exposure") p.exposure = clampd(finite_or(v, p.exposure), 0.25, 3);
+  else if (n == "frontDefinition") p.frontDefinition = clampd(finite_or(v, p.frontDefinition), 0.5, 2.5);
+  else if (n == "opticalDetailTarget") p.opticalDetailTarget = detail_target(finite_or(v, p.opticalDetailTarget));
+  else if (n == "domainWidth") p.domainWidth = clampd(finite_or(v, p.domainWidth), 0.5, 4);
+  else if (n == "domainHeight") p.domainHeight = clampd(finite_or(v, p.domainHeight), 0.5, 6);
+  else if (n == "domainDepth") p.domainDepth = clampd(finite_or(v, p.domainDepth), 0.5, 4);
+  else if (n == "viewZoom") p.viewZoom = clampd(finite_or(v, p.viewZoom), 0.5, 5);
+  else return false;
+  return true;
+}
+
+bool params_set_bool(Params& p, const char* name, bool value) {
+  const std::string n(name ? name : "");
+  if (n == "waterEnabled") {
+    p.waterEnabled = value;
+    return true;
+  }
+  return false;
+}
+
+bool params_get(const Params& p, const char* name, double* value) {
+  const std::string n(name ? name : "");
+  const struct {
+    const char* key;
+    double v;
+  } table[] = {
+      {"burnerSize", p.burnerSize}, {"burnerDepth", p.burnerDepth}, {"sourceThickness", p.sourceThickness},
+      {"firePower", p.firePower}, {"sourceLift", p.sourceLift}, {"buoyancyScale", p.buoyancyScale},
+      {"flamePersistence", p.flamePersistence}, {"airEntrainment", p.airEntrainment}, {"fuelRate", p.fuelRate},
+      {"oxygenRate", p.oxygenRate}, {"heatEfficiency", p.heatEfficiency}, {"vorticity", p.vorticity},
+      {"waterFlow", p.waterFlow}, {"sprayAngle", p.sprayAngle}, {"waterEnabled", p.waterEnabled ? 1.0 : 0.0},
+      {"cameraYaw", p.cameraYaw}, {"cameraPitch", p.cameraPitch}, {"cameraDistance", p.cameraDistance},
+      {"viewMode", p.viewMode}, {"aimHeight", p.aimHeight}, {"nozzleType", p.nozzleType},
+      {"exposure", p.exposure}, {"frontDefinition", p.frontDefinition},
+      {"opticalDetailTarget", p.opticalDetailTarget}, {"domainWidth", p.domainWidth},
+      {"domainHeight", p.domainHeight}, {"domainDepth", p.domainDepth}, {"viewZoom", p.viewZoom},
+  };
+  for (const auto& e : table) {
+    if (n == e.key) {
+      *value = e.v;
+      return true;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------- uniforms
+void write_uniforms(const Params& p, const int32_t dims[3], double dt, double simulation_time, double frame_index,
+                    double time_step, bool cinematic, float u[FXU_COUNT]) {
+  memset(u, 0, sizeof(float) * FXU_COUNT);
+  double cells = (double)dims[0] * (double)dims[1] * (double)dims[2];
+  u[FXU_GRID_X] = (float)dims[0];
+  u[FXU_GRID_Y] = (float)dims[1];
+  u[FXU_GRID_Z] = (float)dims[2];
+  u[FXU_CELLS] = (float)cells;
+  u[FXU_DT] = (float)dt;
+  u[FXU_SIM_TIME] = (float)simulation_time;
+  u[FXU_FRAME] = (float)frame_index;
+  u[FXU_TIME_STEP] = (float)time_step;
+  u[FXU_FUEL_RATE] = (float)p.fuelRate;
+  u[FXU_OXYGEN_RATE] = (float)p.oxygenRate;
+  u[FXU_HEAT_EFFICIENCY] = (float)p.heatEfficiency;
+  u[FXU_VORTICITY] = (float)p.vorticity;
+  u[FXU_WATER_FLOW] = (float)p.waterFlow;
+  u[FXU_SPRAY_ANGLE] = (float)(p.sprayAngle * M_PI / 180.0);
+  u[FXU_WATER_ENABLED] = p.waterEnabled ? 1.0f : 0.0f;
+  u[FXU_AMBIENT] = 300.0f;
+  u[FXU_CAMERA_YAW] = (float)p.cameraYaw;
+  u[FXU_CAMERA_PITCH] = (float)p.cameraPitch;
+  u[FXU_CAMERA_DISTANCE] = (float)p.cameraDistance;
+  u[FXU_VIEW_MODE] = (float)p.viewMode;
+  u[FXU_AIM_HEIGHT] = (float)p.aimHeight;
+  u[FXU_NOZZLE] = (float)p.nozzleType;
+  u[FXU_EXPOSURE] = (float)p.exposure;
+  u[FXU_BURNER_SIZE] = (float)p.burnerSize;
+  u[FXU_BURNER_DEPTH] = (float)p.burnerDepth;
+  u[FXU_SOURCE_THICKNESS] = (float)p.sourceThickness;
+  u[FXU_FIRE_POWER] = (float)p.firePower;
+  u[FXU_DOMAIN_X] = (float)p.domainWidth;
+  u[FXU_DOMAIN_Y] = (float)p.domainHeight;
+  u[FXU_DOMAIN_Z] = (float)p.domainDepth;
+  u[FXU_SOURCE_LIFT] = (float)p.sourceLift;
+  u[FXU_BUOYANCY] = (float)p.buoyancyScale;
+  u[FXU_PERSISTENCE] = (float)p.flamePersistence;
+  u[FXU_VIEW_ZOOM] = (float)p.viewZoom;
+  u[FXU_ENTRAINMENT] = (float)p.airEntrainment;
+  u[FXU_FRONT_DEFINITION] = (float)p.frontDefinition;
+  u[FXU_CINEMATIC] = cinematic ? 1.0f : 0.0f;
+  u[FXU_OPTICAL_DETAIL] = (float)p.opticalDetailTarget;
+}
+
+FxStep make_step(const float u[FXU_COUNT], const int32_t dims[3], const EmitterConfig& em, uint64_t frame,
+                 bool has_liquid) {
+  FxStep st;
+  memset(&st, 0, sizeof st);
+  st.nx = dims[0];
+  st.ny = dims[1];
+  st.nz = dims[2];
+  st.has_liquid = has_liquid ? 1 : 0;
+  st.gx = u[FXU_GRID_X];
+  st.gy = u[FXU_GRID_Y];
+  st.gz = u[FXU_GRID_Z];
+  const float dt = u[FXU_DT], time = u[FXU_SIM_TIME];
+  st.dt = dt;
+  st.time = time;
+  // domainExtent / cellMetric, shaders.ts:35-41
+  st.ex = fxm_max(u[FXU_DOMAIN_X], 0.001f);
+  st.ey = fxm_max(u[FXU_DOMAIN_Y], 0.001f);
+  st.ez = fxm_max(u[FXU_DOMAIN_Z], 0.001f);
+  st.mx = st.gx / st.ex;
+  st.my = st.gy / st.ey;
+  st.mz = st.gz / st.ez;
+  st.ambient = u[FXU_AMBIENT];
+  st.chem_x = u[FXU_FUEL_RATE];
+  st.chem_y = u[FXU_OXYGEN_RATE];
+  st.chem_z = u[FXU_HEAT_EFFICIENCY];
+  st.chem_w = u[FXU_VORTICITY];
+  // simulate, shaders.ts:178-187
+  st.mix_velocity = fxm_clamp(dt * 0.32f, 0.0f, 0.12f);
+  st.mix_temperature = fxm_clamp(dt * 0.18f, 0.0f, 0.08f);
+  st.mix_species = fxm_clamp(dt * 0.12f, 0.0f, 0.06f);
+  // shaders.ts:190-201
+  st.fire_power = fxm_clamp(u[FXU_FIRE_POWER], 0.1f, 4.0f);
+  st.power_root = fxm_sqrt(st.fire_power);
+  st.thermal_power = fxm_min(st.power_root, 1.0f + 0.22f * (st.power_root - 1.0f));
+  st.swirl_tz = time * 2.3f;
+  st.swirl_tx = time * 1.9f;
+  st.lift_clamp = fxm_clamp(u[FXU_SOURCE_LIFT], 0.0f, 4.0f);
+  // shaders.ts:220-221, 253, 256, 270, 284, 288
+  st.entrainment = fxm_clamp(u[FXU_ENTRAINMENT], 0.0f, 3.0f);
+  st.ambient_oxygen = fxm_max(st.chem_y, 0.001f);
+  st.heat_release = 2150.0f + 1450.0f * st.chem_z;
+  st.max_dt = fxm_max(dt, 0.0001f);
+  st.vortex_k = 0.00135f * st.chem_w * dt;
+  st.turbulence_base = 0.075f + 0.16f * st.chem_w;
+  st.tt31 = time * 3.1f;
+  st.tt53 = time * 5.3f;
+  st.tt41 = time * 4.1f;
+  st.tt27 = time * 2.7f;
+  st.tt47 = time * 4.7f;
+  st.buoyancy_clamp = fxm_clamp(u[FXU_BUOYANCY], 0.0f, 4.0f);
+  // spray momentum, shaders.ts:293-296
+  {
+    float nox = 0.035f * st.ex, noy = 0.50f * st.ey, noz = 0.5f * st.ez;
+    float stx = 0.52f * st.ex, sty = u[FXU_AIM_HEIGHT] * st.ey, stz = 0.5f * st.ez;
+    float smx = stx - nox, smy = sty - noy, smz = stz - noz;
+    float slen = fxm_sqrt(smx * smx + smy * smy + smz * smz);
+    st.spray_x = smx / slen;
+    st.spray_y = smy / slen;
+    st.spray_z = smz / slen;
+  }
+  // persistence and radiation, shaders.ts:299-304
+  st.persistence = fxm_clamp(u[FXU_PERSISTENCE], 0.4f, 3.0f);
+  st.ambient4 = fxm_pow(st.ambient, 4.0f);
+  st.radiation_denominator = 1005.0f * st.persistence;
+  st.decay_products = fxm_exp(-dt * 0.075f / st.persistence);
+  st.decay_vapor = fxm_exp(-dt * 0.14f / st.persistence);
+  // emitter, shaders.ts:101-133
+  {
+    st.em_cx = st.ex * 0.5f + fxm_sin(time * 1.73f) * 0.013f;
+    st.em_cy = 0.075f;
+    st.em_cz = st.ez * 0.5f + fxm_cos(time * 1.37f) * 0.011f;
+    const float baseRadius = (float)std::sqrt(0.02024);
+    float radiusX = fxm_min(baseRadius * fxm_clamp(u[FXU_BURNER_SIZE], 0.35f, 3.2f), st.ex * 0.42f);
+    float radiusZ = fxm_min(baseRadius * fxm_clamp(u[FXU_BURNER_DEPTH], 0.35f, 3.2f), st.ez * 0.42f);
+    float thick = fxm_clamp(u[FXU_SOURCE_THICKNESS], 0.35f, 2.5f);
+    st.em_rx2 = radiusX * radiusX;
+    st.em_rz2 = radiusZ * radiusZ;
+    st.em_vdenom = 0.0022f * thick * thick;
+    st.em_rx_safe = fxm_max(radiusX, 0.001f);
+    st.em_rz_safe = fxm_max(radiusZ, 0.001f);
+    st.em_cinematic = fxm_clamp(u[FXU_CINEMATIC], 0.0f, 1.0f);
+    st.et52 = time * 5.2f;
+    st.et44 = time * 4.4f;
+    st.et17 = time * 1.7f;
+    st.et13 = time * 1.3f;
+    st.et37 = time * 3.7f;
+  }
+  st.seeded_count = em.count;
+  memcpy(st.seeded, em.data, sizeof st.seeded);
+  st.jitter = em.jitter;
+  st.velocity_noise = em.velocity_noise;
+  st.seed = em.seed;
+  st.frame = frame;
+  // Jacobi stencil, shaders.ts:386-389, 402
+  st.wx = st.mx * st.mx;
+  st.wy = st.my * st.my;
+  st.wz = st.mz * st.mz;
+  st.jacobi_inv = 1.0f / (2.0f * (st.wx + st.wy + st.wz));
+  return st;
+}
+
+// ------------------------------------------------------------------- clock
+int32_t FixedStepClock::advance(double delta_seconds) {
+  if (!std::isfinite(delta_seconds)) return 0;
+  double bounded = std::max(0.0, std::min(delta_seconds, step_seconds * maximum_steps));
+  accumulator += bounded;
+  int32_t count = 0;
+  while (accumulator >= step_seconds && count < maximum_steps) {
+    accumulator -= step_seconds;
+    count += 1;
+  }
+  if (count == maximum_steps && accumulator >= step_seconds) accumulator = std::fmod(accumulator, step_seconds);
+  return count;
+}
+
+// ------------------------------------------------------------------ memory
+bool heavy_grid(const int32_t dims[3]) {
+  return (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2] >= kHeavyCellThreshold;
+}
+
+uint64_t device_bytes(const int32_t dims[3], int32_t species_storage, bool f16_pressure, bool compact_f32,
+                      bool with_liquid) {
+  uint64_t cells = (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2];
+  uint64_t species = species_storage == 0 ? 16 : 8;
+  uint64_t pressure = f16_pressure ? 2 : 4;
+  uint64_t bytes = cells * (2 * 16 + 2 * species + 2 * pressure + 4 /* div */ + 4 /* reaction */);
+  if (!compact_f32) bytes += cells * 2 * pressure;  // residual pair
+  if (with_liquid) bytes += cells * 4;
+  return bytes + (uint64_t)kReduceScratchBytes;
+}
+
+}  // namespace fx
+
+// =========================================================== C ABI (host only)
+using namespace fx;
+
+namespace {
+
+struct Tier {
+  const char* name;
+  int32_t dims[3];
+  int32_t pressure, correction;
+  double time_step;
+  int32_t cinematic;
+};
+
+// FIREX_QUALITY (FireXEngine.ts:187-238) and FIREX_OFFLINE_QUALITY (:240-277).
+// The uniform's cinematic flag is set for the cinematic live tier and for
+// every offline tier (usesCinematicPresentation, FireXEngine.ts:2819-2821).
+const Tier kTiers[] = {
+    {"low", {32, 48, 32}, 14, 14, 1.0 / 30.0, 0},
+    {"balanced", {40, 64, 40}, 24, 24, 1.0 / 30.0, 0},
+    {"high", {64, 96, 64}, 36, 36, 1.0 / 30.0, 0},
+    {"maximum", {80, 120, 80}, 48, 48, 1.0 / 30.0, 0},
+    {"cinematic", {80, 120, 80}, 48, 48, 1.0 / 30.0, 1},
+    {"hd", {128, 192, 128}, 64, 64, 1.0 / 60.0, 1},
+    {"qhd", {160, 240, 160}, 80, 80, 1.0 / 60.0, 1},
+    {"uhd", {192, 288, 192}, 96, 96, 1.0 / 60.0, 1},
+};
+
+const int64_t kGridBytesPerCell = 116;       // gridConfiguration.ts:106
+const int64_t kHeavyGridBytesPerCell = 108;  // gridConfiguration.ts:107
+const int64_t kParticleBytes = 80;           // gridConfiguration.ts:108
+const int64_t kLargestStoragePerCell = 16;   // gridConfiguration.ts:109
+const int64_t kDetailTextureBytes = 64 * 64 * 64 * 4;  // gridConfiguration.ts:110
+
+}  // namespace
+
+extern "C" {
+
+const char* firex_last_error(void) { return g_last_error.c_str(); }
+
+int firex_abi_version(void) { return FIREX_ABI_VERSION; }
+
+void firex_config_init(firex_config* c) {
+  if (!c) return;
+  memset(c, 0, sizeof *c);
+  c->dims[0] = 40;
+  c->dims[1] = 64;
+  c->dims[2] = 40;
+  c->pressure_iterations = 24;
+  c->correction_iterations = 24;
+  c->time_step = 1.0 / 30.0;
+  c->maximum_substeps = 1;
+  c->species_storage = FIREX_STORAGE_F32;
+  c->heavy_mode = FIREX_HEAVY_AUTO;
+  c->plan = FIREX_PLAN_REFERENCE;
+}
+
+int firex_config_from_tier(firex_config* c, const char* tier) {
+  if (!c || !tier) return set_error(FIREX_ERR_INVALID, "config and tier must be non-null");
+  for (const Tier& t : kTiers) {
+    if (strcmp(t.name, tier) == 0) {
+      memcpy(c->dims, t.dims, sizeof c->dims);
+      c->pressure_iterations = t.pressure;
+      c->correction_iterations = t.correction;
+      c->time_step = t.time_step;
+      c->maximum_substeps = 1;
+      c->cinematic = t.cinematic;
+      return FIREX_OK;
+    }
+  }
+  return set_error(FIREX_ERR_INVALID, std::string("unknown Fire-X tier: ") + tier);
+}
+
+int firex_estimate_memory(const int32_t dims[3], int64_t particle_count, firex_memory_estimate* out) {
+  if (!dims || !out) return set_error(FIREX_ERR_INVALID, "dims and out must be non-null");
+  int64_t cells = (int64_t)dims[0] * (int64_t)dims[1] * (int64_t)dims[2];
+  int64_t per_cell = (uint64_t)cells >= kHeavyCellThreshold ? kHeavyGridBytesPerCell : kGridBytesPerCell;
+  int64_t particles = particle_count > 0 ? particle_count : 0;
+  out->cell_count = cells;
+  out->grid_bytes = cells * per_cell;
+  out->particle_bytes = particles * kParticleBytes;
+  out->fixed_bytes = kDetailTextureBytes;
+  out->total_bytes = out->grid_bytes + out->particle_bytes + out->fixed_bytes;
+  out->largest_storage_buffer_bytes = cells * kLargestStoragePerCell;
+  return FIREX_OK;
+}
+
+int firex_preflight(const int32_t dims[3], const firex_device_limits* limits, int64_t particle_count,
+                    int64_t memory_budget_bytes, uint32_t* issues) {
+  if (!dims || !limits || !issues) return set_error(FIREX_ERR_INVALID, "dims, limits and issues must be non-null");
+  uint32_t bits = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (dims[i] <= 0) bits |= FIREX_ISSUE_INVALID_DIMENSION;
+    else if (dims[i] % 4 != 0) bits |= FIREX_ISSUE_UNALIGNED_DIMENSION;
+    if (dims[i] > limits->max_texture_dimension_3d) bits |= FIREX_ISSUE_TEXTURE_DIMENSION_LIMIT;
+  }
+  firex_memory_estimate e;
+  firex_estimate_memory(dims, particle_count, &e);
+  if (e.largest_storage_buffer_bytes > limits->max_storage_buffer_binding_size) bits |= FIREX_ISSUE_STORAGE_BINDING_LIMIT;
+  if (e.largest_storage_buffer_bytes > limits->max_buffer_size) bits |= FIREX_ISSUE_BUFFER_SIZE_LIMIT;
+  if (memory_budget_bytes >= 0 && e.total_bytes > memory_budget_bytes) bits |= FIREX_ISSUE_MEMORY_BUDGET;
+  *issues = bits;
+  if (bits) return set_error(FIREX_ERR_UNSUPPORTED, "the grid fails preflight");
+  return FIREX_OK;
+}
+
+int firex_split_offline_delta(double delta, double maximum_step, double* steps, int32_t cap, int32_t* count) {
+  if (!count) return set_error(FIREX_ERR_INVALID, "count must be non-null");
+  *count = 0;
+  if (!std::isfinite(delta) || delta < 0)
+    return set_error(FIREX_ERR_INVALID, "Offline frame delta must be finite and non-negative.");
+  if (!std::isfinite(maximum_step) || maximum_step <= 0)
+    return set_error(FIREX_ERR_INVALID, "Offline maximum step must be positive and finite.");
+  if (delta == 0) return FIREX_OK;
+  double n = std::max(1.0, std::ceil(delta / maximum_step - 1e-12));
+  if (n > 2147483647.0) return set_error(FIREX_ERR_INVALID, "Offline frame delta needs too many steps.");
+  int32_t k = (int32_t)n;
+  double step = delta / (double)k;
+  for (int32_t i = 0; i < k && i < cap; ++i) steps[i] = step;
+  *count = k;
+  return FIREX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// firex_jacobi.h — temporally blocked Jacobi sweeps and the fused
+// projection + divergence pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fx_common.h"
+
+namespace fx {
+
+// Launch geometry of the temporally blocked sweep kernel.
+struct JacobiPlan {
+  int32_t sweeps_per_launch = 4;  // K: sweeps advanced per launch
+  int32_t rows = 4;               // R: y rows per thread
+  int32_t waves = 8;              // W: waves per block (stacked in y)
+  int32_t chunk_z = 16;           // output planes per block
+};
+
+JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch);
+
+// One launch of the temporally blocked kernel: `sweeps` sweeps from
+// buffers[src] (or from the zero field, without reading it) into
+// buffers[1 - src].
+struct JacobiLaunch {
+  int32_t sweeps;
+  int32_t src;
+  bool zero_start;
+};
+
+// Splits `sweeps` sweeps starting at buffers[start] into launches; returns
+// the number of launches written to out (at most cap) and stores the index
+// of the buffer holding the result in *result.
+int jacobi_schedule(const JacobiPlan& plan, int start, bool zero_start, int sweeps, JacobiLaunch* out, int cap,
+                    int* result);
+
+// f16 selects f16 pressure storage: every sweep's result is rounded to f16,
+// as pressureShaders.ts:116 stores it.
+void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, void* const buffers[2],
+                      const float* div, const FxStep& st, hipStream_t s);
+
+// projectVelocity (shaders.ts:470-501) fused with computeDivergence of its
+// result (shaders.ts:344-364): writes fout and div.
+void launch_project_divergence(int f16, const float4* fin, const void* p, float4* fout, float* div, const FxStep& st,
+                               hipStream_t s);
+
+// Host-side check of a plan against the kernel's assumptions.
+bool jacobi_plan_valid(const JacobiPlan& plan);
+
+}  // namespace fx

@@ -1,0 +1,334 @@
+// firex_jacobi.hip — temporally blocked Jacobi pressure sweeps for gfx950 and
+// the projection pass fused with the divergence of its result.
+//
+// Jacobi sweep (solvePressureGlobal, src/sim/firex/shaders.ts:375-403):
+//   p'(c) = (wx (p[x-1] + p[x+1]) + wy (p[y-1] + p[y+1]) + wz (p[z-1] + p[z+1]) - div(c))
+//           * (1 / (2 (wx + wy + wz)))
+// with clamped (Neumann) neighbours on the side walls and floor, zero
+// (Dirichlet) above the open top, and p' = 0 on the top row y = ny - 1.
+//
+// One launch advances K sweeps.  A block owns a 64 x H column tile (x on the
+// 64 lanes of a wave, H = W * R rows in y: R rows per thread, W waves stacked
+// in y) plus one z chunk, and streams the chunk's planes bottom to top.
+// Sweep level s works on plane (t - s) while level s - 1 works on plane
+// (t - s + 1), so each level needs only the three most recent planes of the
+// level below it.  Those stay in registers (per thread: planes z - 1, z, z + 1
+// for its own columns); x neighbours come from the adjacent lanes, y
+// neighbours from the thread's own rows or, at wave edges, from a small
+// double-buffered LDS exchange (one barrier per streamed plane).  Each level
+// loses one column/row of validity at the tile edges, so a tile outputs
+// (64 - 2K) x (H - 2K) columns; halo columns are recomputed by the
+// neighbouring tile.  HBM traffic per launch is one read of p and div and one
+// write of p per output cell, instead of K of each.
+//
+// Every cell update uses the same f32 operations in the same order as one
+// sweep of the reference kernel, so K fused sweeps are bit-identical to K
+// separate sweeps.
+#include <hip/hip_runtime.h>
+
+#include "firex_jacobi.h"
+#include "firex_kernels.h"
+
+namespace fx {
+
+namespace {
+
+template <int P> struct PIO;
+template <> struct PIO<0> {
+  using Word = float;
+  __device__ static inline float load(const Word* p, uint32_t i) { return p[i]; }
+  __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = v; }
+  __device__ static inline float round(float v) { return v; }
+};
+template <> struct PIO<1> {
+  using Word = _Float16;
+  __device__ static inline float load(const Word* p, uint32_t i) { return (float)p[i]; }
+  __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = (_Float16)v; }
+  __device__ static inline float round(float v) { return (float)(_Float16)v; }
+};
+
+__device__ inline uint32_t xcd_remap(uint32_t hw, uint32_t total) {
+  uint32_t q = total >> 3, r = total & 7u;
+  uint32_t xcd = hw & 7u, slot = hw >> 3;
+  uint32_t base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return base + slot;
+}
+
+template <int P, int K, int R, int W>
+__global__ __launch_bounds__(64 * W) void k_jacobi_tb(const typename PIO<P>::Word* __restrict__ pin,
+                                                      typename PIO<P>::Word* __restrict__ pout,
+                                                      const float* __restrict__ div, FxStep st, int chunk_z,
+                                                      int tiles_x, int tiles_y, int zero_start) {
+  constexpr int H = W * R;
+  constexpr int TXI = 64 - 2 * K;
+  constexpr int TYI = H - 2 * K;
+  // [parity][level][wave][row 0 | row R-1][lane]
+  __shared__ float edge[2][K][W][2][64];
+
+  const int nx = st.nx, ny = st.ny, nz = st.nz;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t chunks = (uint32_t)(nz + chunk_z - 1) / (uint32_t)chunk_z;
+  const uint32_t total = (uint32_t)tiles_x * (uint32_t)tiles_y * chunks;
+  const uint32_t b = xcd_remap(blockIdx.x, total);
+  const int tile_x = (int)(b % (uint32_t)tiles_x);
+  const uint32_t rest = b / (uint32_t)tiles_x;
+  const int tile_y = (int)(rest % (uint32_t)tiles_y);
+  const int chunk = (int)(rest / (uint32_t)tiles_y);
+
+  const int gx = tile_x * TXI - K + lane;
+  const int gy0 = tile_y * TYI - K + wave * R;
+  const int z0 = chunk * chunk_z;
+  const int z1 = min(z0 + chunk_z, nz);
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const int gxc = min(max(gx, 0), nx - 1);
+  uint32_t rowoff[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) rowoff[r] = (uint32_t)gxc + (uint32_t)nx * (uint32_t)min(max(gy0 + r, 0), ny - 1);
+
+  const bool x_lo = gx == 0, x_hi = gx == nx - 1;
+  const bool out_lane = lane >= K && lane < 64 - K && gx < nx;
+  const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.jacobi_inv;
+
+  float A[K][R], B[K][R], C[K][R], D[K + 1][R];
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+#pragma unroll
+    for (int r = 0; r < R; ++r) A[s][r] = B[s][r] = C[s][r] = 0.0f;
+#pragma unroll
+  for (int s = 0; s <= K; ++s)
+#pragma unroll
+    for (int r = 0; r < R; ++r) D[s][r] = 0.0f;
+
+  const int lo0 = max(z0 - K, 0);
+  const int hi0 = min(z1 - 1 + K, nz - 1);
+  const int t_end = z1 - 1 + K;
+  for (int t = lo0; t <= t_end; ++t) {
+    const int par = t & 1;
+    const bool act0 = t <= hi0;
+    if (act0) {
+      const uint32_t base = (uint32_t)t * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) C[0][r] = zero_start ? 0.0f : PIO<P>::load(pin, base + rowoff[r]);
+    }
+#pragma unroll
+    for (int s = K; s >= 2; --s)
+#pragma unroll
+      for (int r = 0; r < R; ++r) D[s][r] = D[s - 1][r];
+    if (t >= 1 && t - 1 < nz) {
+      const uint32_t base = (uint32_t)(t - 1) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) D[1][r] = div[base + rowoff[r]];
+    }
+
+    bool active[K + 1];
+    active[0] = act0;
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+      const int z = t - s;
+      const int lo_s = max(z0 - (K - s), 0);
+      const int hi_s = min(z1 - 1 + (K - s), nz - 1);
+      active[s] = z >= lo_s && z <= hi_s;
+      if (!active[s]) continue;
+      const bool z_lo = z == 0, z_hi = z == nz - 1;
+      float res[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int gy = gy0 + r;
+        const float ctr = B[s - 1][r];
+        float left = __shfl_up(ctr, 1, 64);
+        float right = __shfl_down(ctr, 1, 64);
+        if (x_lo) left = ctr;
+        if (x_hi) right = ctr;
+        float down, up;
+        if (r > 0) down = B[s - 1][r - 1];
+        else down = wave > 0 ? edge[par][s - 1][wave - 1][1][lane] : ctr;
+        if (r < R - 1) up = B[s - 1][r + 1];
+        else up = wave < W - 1 ? edge[par][s - 1][wave + 1][0][lane] : ctr;
+        if (gy <= 0) down = ctr;
+        if (gy + 1 >= ny) up = 0.0f;
+        const float zl = z_lo ? ctr : A[s - 1][r];
+        const float zh = z_hi ? ctr : C[s - 1][r];
+        const float nb = wx * (left + right) + wy * (down + up) + wz * (zl + zh);
+        float v = (nb - D[s][r]) * inv;
+        if (gy == ny - 1) v = 0.0f;
+        res[r] = PIO<P>::round(v);
+      }
+      if (s < K) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) C[s][r] = res[r];
+      } else if (out_lane) {
+        const uint32_t base = (uint32_t)z * plane;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int row = wave * R + r;
+          const int gy = gy0 + r;
+          if (row >= K && row < H - K && gy < ny) PIO<P>::store(pout, base + (uint32_t)gx + (uint32_t)nx * (uint32_t)gy, res[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      if (active[s]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          A[s][r] = B[s][r];
+          B[s][r] = C[s][r];
+        }
+      }
+      edge[par ^ 1][s][wave][0][lane] = B[s][0];
+      edge[par ^ 1][s][wave][1][lane] = B[s][R - 1];
+    }
+    __syncthreads();
+  }
+}
+
+template <int P, int K>
+void launch_tb(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st, int zero_start,
+               hipStream_t s) {
+  constexpr int R = 4, W = 8;
+  const int TXI = 64 - 2 * K, TYI = W * R - 2 * K;
+  const int tiles_x = (st.nx + TXI - 1) / TXI;
+  const int tiles_y = (st.ny + TYI - 1) / TYI;
+  const int chunks = (st.nz + plan.chunk_z - 1) / plan.chunk_z;
+  const uint32_t blocks = (uint32_t)tiles_x * (uint32_t)tiles_y * (uint32_t)chunks;
+  using Word = typename PIO<P>::Word;
+  k_jacobi_tb<P, K, R, W><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, st, plan.chunk_z, tiles_x,
+                                                    tiles_y, zero_start);
+}
+
+template <int P>
+void launch_tb_k(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
+                 int zero_start, hipStream_t s) {
+  switch (k) {
+    case 1: launch_tb<P, 1>(plan, pin, pout, div, st, zero_start, s); break;
+    case 2: launch_tb<P, 2>(plan, pin, pout, div, st, zero_start, s); break;
+    case 3: launch_tb<P, 3>(plan, pin, pout, div, st, zero_start, s); break;
+    case 4: launch_tb<P, 4>(plan, pin, pout, div, st, zero_start, s); break;
+    case 5: launch_tb<P, 5>(plan, pin, pout, div, st, zero_start, s); break;
+    case 6: launch_tb<P, 6>(plan, pin, pout, div, st, zero_start, s); break;
+    case 7: launch_tb<P, 7>(plan, pin, pout, div, st, zero_start, s); break;
+    default: launch_tb<P, 8>(plan, pin, pout, div, st, zero_start, s); break;
+  }
+}
+
+// ----------------------------------------------------- projection + divergence
+template <int P>
+__device__ inline float lp(const typename PIO<P>::Word* p, const FxStep& st, int x, int y, int z) {
+  if (y >= st.ny) return 0.0f;
+  x = min(max(x, 0), st.nx - 1);
+  y = max(y, 0);
+  z = min(max(z, 0), st.nz - 1);
+  return PIO<P>::load(p, (uint32_t)x + (uint32_t)st.nx * ((uint32_t)y + (uint32_t)st.ny * (uint32_t)z));
+}
+
+__device__ inline uint32_t ci(const FxStep& st, int x, int y, int z) {
+  x = min(max(x, 0), st.nx - 1);
+  y = min(max(y, 0), st.ny - 1);
+  z = min(max(z, 0), st.nz - 1);
+  return (uint32_t)x + (uint32_t)st.nx * ((uint32_t)y + (uint32_t)st.ny * (uint32_t)z);
+}
+
+// Component `axis` of projectVelocity's output at cell (x, y, z) (in range).
+template <int P, int AXIS>
+__device__ inline float projected(const float4* fin, const typename PIO<P>::Word* p, const FxStep& st, int x, int y,
+                                  int z) {
+  const bool top = y == st.ny - 1;
+  const int sy = top ? y - 1 : y;  // the top cell copies the cell below
+  const float c = lp<P>(p, st, x, sy, z);
+  const float4 f = fin[ci(st, x, sy, z)];
+  float v;
+  if (AXIS == 0) {
+    v = f.x - (lp<P>(p, st, x + 1, sy, z) - c) * st.mx;
+    if (x == 0 || x == st.nx - 1) v = 0.0f;
+  } else if (AXIS == 1) {
+    v = f.y - (lp<P>(p, st, x, sy + 1, z) - c) * st.my;
+    if (y == 0) v = 0.0f;
+    if (top) v = fxm_max(0.0f, v);
+  } else {
+    v = f.z - (lp<P>(p, st, x, sy, z + 1) - c) * st.mz;
+    if (z == 0 || z == st.nz - 1) v = 0.0f;
+  }
+  return v;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_project_divergence(const float4* __restrict__ fin,
+                                                            const typename PIO<P>::Word* __restrict__ p,
+                                                            float4* __restrict__ fout, float* __restrict__ div,
+                                                            FxStep st) {
+  const uint32_t bx = (uint32_t)(st.nx + 63) / 64, by = (uint32_t)(st.ny + 3) / 4;
+  const uint32_t total = bx * by * (uint32_t)st.nz;
+  const uint32_t b = xcd_remap(blockIdx.x, total);
+  const int x = (int)((b % bx) * 64 + threadIdx.x);
+  const uint32_t rest = b / bx;
+  const int y = (int)((rest % by) * 4 + threadIdx.y);
+  const int z = (int)(rest / by);
+  if (x >= st.nx || y >= st.ny) return;
+  const uint32_t id = ci(st, x, y, z);
+  const float ux = projected<P, 0>(fin, p, st, x, y, z);
+  const float uy = projected<P, 1>(fin, p, st, x, y, z);
+  const float uz = projected<P, 2>(fin, p, st, x, y, z);
+  fout[id] = make_float4(ux, uy, uz, fin[id].w);
+  const float left = x > 0 ? projected<P, 0>(fin, p, st, x - 1, y, z) : ux;
+  const float bottom = y > 0 ? projected<P, 1>(fin, p, st, x, y - 1, z) : uy;
+  const float back = z > 0 ? projected<P, 2>(fin, p, st, x, y, z - 1) : uz;
+  div[id] = (ux - left) * st.mx + (uy - bottom) * st.my + (uz - back) * st.mz;
+}
+
+}  // namespace
+
+bool jacobi_plan_valid(const JacobiPlan& plan) {
+  return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= 8 && plan.rows == 4 && plan.waves == 8 &&
+         plan.chunk_z >= 1;
+}
+
+JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch) {
+  JacobiPlan p;
+  p.sweeps_per_launch = sweeps_per_launch > 0 ? (sweeps_per_launch > 8 ? 8 : sweeps_per_launch) : 4;
+  // Enough z chunks to give every CU a couple of blocks.
+  const int K = p.sweeps_per_launch;
+  const int tiles = ((dims[0] + 63 - 2 * K) / (64 - 2 * K)) * ((dims[1] + 31 - 2 * K) / (32 - 2 * K));
+  int chunk = dims[2];
+  while (chunk > 8 && (long)tiles * ((dims[2] + chunk - 1) / chunk) < 512) chunk = (chunk + 1) / 2;
+  p.chunk_z = chunk;
+  return p;
+}
+
+int jacobi_schedule(const JacobiPlan& plan, int start, bool zero_start, int sweeps, JacobiLaunch* out, int cap,
+                    int* result) {
+  int cur = start, n = 0;
+  bool zero = zero_start;
+  int left = sweeps;
+  while (left > 0) {
+    const int k = left < plan.sweeps_per_launch ? left : plan.sweeps_per_launch;
+    if (n < cap) out[n] = JacobiLaunch{k, cur, zero};
+    ++n;
+    cur = 1 - cur;
+    zero = false;
+    left -= k;
+  }
+  *result = cur;
+  return n;
+}
+
+void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, void* const buffers[2],
+                      const float* div, const FxStep& st, hipStream_t s) {
+  const void* pin = buffers[l.src];
+  void* pout = buffers[1 - l.src];
+  if (f16)
+    launch_tb_k<1>(l.sweeps, plan, pin, pout, div, st, l.zero_start ? 1 : 0, s);
+  else
+    launch_tb_k<0>(l.sweeps, plan, pin, pout, div, st, l.zero_start ? 1 : 0, s);
+}
+
+void launch_project_divergence(int f16, const float4* fin, const void* p, float4* fout, float* div, const FxStep& st,
+                               hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((st.nx + 63) / 64) * (uint32_t)((st.ny + 3) / 4) * (uint32_t)st.nz;
+  if (f16)
+    k_project_divergence<1><<<blocks, dim3(64, 4), 0, s>>>(fin, (const _Float16*)p, fout, div, st);
+  else
+    k_project_divergence<0><<<blocks, dim3(64, 4), 0, s>>>(fin, (const float*)p, fout, div, st);
+}
+
+}  // namespace fx

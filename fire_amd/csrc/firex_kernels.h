@@ -1,0 +1,38 @@
+// firex_kernels.h — host-side launchers of the gfx950 Fire-X kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fx_common.h"
+
+enum { FX_STORAGE_F32 = 0, FX_STORAGE_F16 = 1, FX_STORAGE_BF16 = 2 };
+
+namespace fx {
+
+struct SimulateArgs {
+  const float4* flow_in;
+  const void* species_in;
+  float4* flow_out;
+  void* species_out;
+  float* reaction;
+  const uint32_t* liquid;  // may be null when st.has_liquid == 0
+  FxStep st;
+};
+
+void launch_initialize(int storage, float4* flow, void* species, uint32_t cells, float ambient, float oxygen,
+                       hipStream_t s);
+void launch_fill_u32(void* p, uint64_t words, uint32_t value, hipStream_t s);
+void launch_copy_u128(void* dst, const void* src, uint64_t n16, hipStream_t s);
+void launch_simulate(int storage, const SimulateArgs& a, hipStream_t s);
+void launch_divergence(const float4* flow, float* div, const FxStep& st, hipStream_t s);
+void launch_jacobi(int f16, const void* pin, void* pout, const float* div, const FxStep& st, hipStream_t s);
+void launch_project(int f16, const float4* fin, const void* p, float4* fout, const FxStep& st, hipStream_t s);
+void launch_reduce_field(int storage, int kind, int component, const void* data, uint32_t cells, double* partial,
+                         double* out, hipStream_t s);
+void launch_reduce_residual(int f16, const void* p, const float* div, const FxStep& st, double* partial, double* out,
+                            hipStream_t s);
+void launch_reduce_flow_divergence(const float4* flow, const FxStep& st, double* partial, double* out, hipStream_t s);
+
+constexpr uint32_t kReducePartials = 1024;
+
+}  // namespace fx

@@ -1,0 +1,703 @@
+// firex_solver.hip — the C ABI solver object: device buffers of one plume,
+// the step graph of FireXEngine.encodeSimulationStep
+// (src/sim/firex/FireXEngine.ts:2115-2276) and field transfer.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/firex.h"
+#include "firex_host.h"
+#include "firex_jacobi.h"
+#include "firex_kernels.h"
+
+using namespace fx;
+
+struct firex_solver {
+  firex_config cfg;
+  int32_t dims[3];
+  uint32_t cells = 0;
+  bool heavy = false;      // >= 64 Mi cell path
+  bool f16p = false;       // f16 pressure storage
+  bool compact = false;    // compact f32 heavy fallback (shared pressure, cold start)
+  Params params;
+  EmitterConfig emitters;
+  FixedStepClock clock;
+  double simulation_time = 0;
+  uint64_t frame_index = 0;
+  int species_index = 0, pressure_index = 0;
+  int flow_a = 0;          // buffer holding the reference's flow[0]
+  bool has_liquid = false;
+  bool lost = false;
+  float4* flow[2] = {nullptr, nullptr};
+  void* species[2] = {nullptr, nullptr};
+  void* pressure[2] = {nullptr, nullptr};
+  void* corr[2] = {nullptr, nullptr};
+  float* divergence = nullptr;
+  float* reaction = nullptr;
+  uint32_t* liquid = nullptr;
+  double* scratch = nullptr;
+  std::vector<void*> allocations;
+  hipStream_t stream = nullptr;
+  uint64_t bytes = 0;
+  JacobiPlan jacobi;
+  int corr_index = 0;      // buffer holding the last residual solution
+  // HIP-event timing of launches (firex_set_pass_timing)
+  struct TimedLaunch {
+    int pass;
+    hipEvent_t start, stop;
+  };
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<TimedLaunch> pending;
+  firex_pass_timing totals;
+};
+
+namespace {
+
+int hip_fail(firex_solver* s, hipError_t e, const char* what) {
+  if (s && (e == hipErrorIllegalAddress || e == hipErrorLaunchFailure || e == hipErrorAssert ||
+            e == hipErrorECCNotCorrectable || e == hipErrorLaunchTimeOut))
+    s->lost = true;
+  return set_error(s && s->lost ? FIREX_ERR_DEVICE_LOST : FIREX_ERR_DEVICE,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define FX_HIP(s, call, what)                         \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return hip_fail(s, e_, what); \
+  } while (0)
+
+size_t species_word(int storage) { return storage == FIREX_STORAGE_F32 ? 16 : 8; }
+
+int check_usable(firex_solver* s) {
+  if (!s) return set_error(FIREX_ERR_INVALID, "solver is null");
+  if (s->lost) return set_error(FIREX_ERR_DEVICE_LOST, "The Fire-X HIP device was lost.");
+  return FIREX_OK;
+}
+
+// Launch errors are reported at the next call that checks them.
+int check_launch(firex_solver* s, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(s, e, what);
+  return FIREX_OK;
+}
+
+void free_all(firex_solver* s) {
+  for (void* p : s->allocations) hipFree(p);
+  s->allocations.clear();
+}
+
+int alloc(firex_solver* s, void** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(p, bytes < 256 ? 256 : bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(FIREX_ERR_OUT_OF_MEMORY,
+                     std::string("The selected Fire-X grid could not be allocated (") + what + ", " +
+                         std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+  }
+  s->allocations.push_back(*p);
+  return FIREX_OK;
+}
+
+// initializeFields, FireXEngine.ts:1933-1974: clear every buffer, then
+// flow[0] = (0, 0, 0, 300), species[0] = (0, oxygenRate, 0, 0).
+int initialize_fields(firex_solver* s) {
+  const uint64_t c = s->cells;
+  size_t pw = s->f16p ? 2 : 4;
+  for (int k = 0; k < 2; ++k) {
+    launch_fill_u32(s->flow[k], c * 4, 0, s->stream);
+    launch_fill_u32(s->species[k], c * species_word(s->cfg.species_storage) / 4, 0, s->stream);
+    launch_fill_u32(s->pressure[k], (c * pw + 3) / 4, 0, s->stream);
+    if (!s->compact) launch_fill_u32(s->corr[k], (c * pw + 3) / 4, 0, s->stream);
+  }
+  launch_fill_u32(s->divergence, c, 0, s->stream);
+  launch_fill_u32(s->reaction, c, 0, s->stream);
+  if (s->liquid) launch_fill_u32(s->liquid, c, 0, s->stream);
+  s->flow_a = 0;
+  s->corr_index = 0;
+  launch_initialize(s->cfg.species_storage, s->flow[0], s->species[0], s->cells, 300.0f, (float)s->params.oxygenRate,
+                    s->stream);
+  return check_launch(s, "field initialization");
+}
+
+hipEvent_t take_event(firex_solver* s) {
+  if (!s->event_pool.empty()) {
+    hipEvent_t e = s->event_pool.back();
+    s->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Folds the finished event pairs into the totals (waits for the stream).
+int collect_timing(firex_solver* s) {
+  if (s->pending.empty()) return FIREX_OK;
+  FX_HIP(s, hipStreamSynchronize(s->stream), "timing synchronize");
+  for (const auto& t : s->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) s->totals.milliseconds[t.pass] += ms;
+    s->event_pool.push_back(t.start);
+    s->event_pool.push_back(t.stop);
+  }
+  s->pending.clear();
+  return FIREX_OK;
+}
+
+template <typename F>
+void timed(firex_solver* s, int pass, F&& launch) {
+  if (!s->timing) {
+    launch();
+    return;
+  }
+  hipEvent_t a = take_event(s), b = take_event(s);
+  if (!a || !b) {
+    launch();
+    return;
+  }
+  hipEventRecord(a, s->stream);
+  launch();
+  hipEventRecord(b, s->stream);
+  s->pending.push_back({pass, a, b});
+  s->totals.launches[pass] += 1;
+}
+
+float4* flow0(firex_solver* s) { return s->flow[s->flow_a]; }
+float4* flow1(firex_solver* s) { return s->flow[1 - s->flow_a]; }
+
+// Jacobi sweeps on a buffer pair; returns the index of the result buffer.
+int sweeps(firex_solver* s, int f16, void* const bufs[2], int start, bool zero_start, int count, const FxStep& st,
+           bool tb) {
+  if (!tb) {
+    int cur = start;
+    for (int i = 0; i < count; ++i) {
+      timed(s, FIREX_PASS_JACOBI, [&] { launch_jacobi(f16, bufs[cur], bufs[1 - cur], s->divergence, st, s->stream); });
+      cur = 1 - cur;
+    }
+    s->totals.jacobi_sweeps += s->timing ? count : 0;
+    return cur;
+  }
+  JacobiLaunch launches[1024];
+  int result = start;
+  int n = jacobi_schedule(s->jacobi, start, zero_start, count, launches, 1024, &result);
+  for (int i = 0; i < n && i < 1024; ++i)
+    timed(s, FIREX_PASS_JACOBI, [&] { launch_jacobi_tb(s->jacobi, f16, launches[i], bufs, s->divergence, st, s->stream); });
+  s->totals.jacobi_sweeps += s->timing ? count : 0;
+  return result;
+}
+
+void fill(firex_solver* s, void* p, uint64_t words) {
+  timed(s, FIREX_PASS_FILL_COPY, [&] { launch_fill_u32(p, words, 0, s->stream); });
+}
+
+// One solver step with the given uniform block (encodeSimulationStep,
+// FireXEngine.ts:2158-2275).
+int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
+  const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
+  hipStream_t q = s->stream;
+  const bool fused = s->cfg.plan == FIREX_PLAN_FUSED;
+  const int N = s->cfg.pressure_iterations, M = s->cfg.correction_iterations;
+  const uint64_t pwords = ((uint64_t)s->cells * (s->f16p ? 2 : 4) + 3) / 4;
+  const int f16 = s->f16p ? 1 : 0;
+
+  SimulateArgs a{flow0(s), s->species[s->species_index], flow1(s), s->species[1 - s->species_index], s->reaction,
+                 s->liquid, st};
+  timed(s, FIREX_PASS_SIMULATE, [&] { launch_simulate(s->cfg.species_storage, a, q); });
+  s->species_index = 1 - s->species_index;
+  timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow1(s), s->divergence, st, q); });
+
+  if (s->heavy && s->compact) {
+    // compact f32 fallback, FireXEngine.ts:2214-2246: shared fields, cleared
+    // before each solve (no warm start).
+    fill(s, s->pressure[0], pwords);
+    fill(s, s->pressure[1], pwords);
+    s->pressure_index = sweeps(s, 0, s->pressure, 0, fused, N, st, fused);
+    timed(s, FIREX_PASS_PROJECT, [&] { launch_project(0, flow1(s), s->pressure[s->pressure_index], flow0(s), st, q); });
+    timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow0(s), s->divergence, st, q); });
+    if (!fused || M == 0) {
+      fill(s, s->pressure[0], pwords);
+      fill(s, s->pressure[1], pwords);
+    }
+    s->corr_index = sweeps(s, 0, s->pressure, 0, fused, M, st, fused);
+    timed(s, FIREX_PASS_PROJECT, [&] { launch_project(0, flow0(s), s->pressure[s->corr_index], flow1(s), st, q); });
+    timed(s, FIREX_PASS_FILL_COPY, [&] { launch_copy_u128(flow0(s), flow1(s), s->cells, q); });
+  } else if (fused) {
+    // The same passes, merged: temporally blocked sweeps, projection fused
+    // with the residual divergence, the residual solve started from its
+    // known zero field, and copyFlow replaced by swapping the flow buffers.
+    s->pressure_index = sweeps(s, f16, s->pressure, s->pressure_index, false, N, st, true);
+    timed(s, FIREX_PASS_PROJECT_DIVERGENCE, [&] {
+      launch_project_divergence(f16, flow1(s), s->pressure[s->pressure_index], flow0(s), s->divergence, st, q);
+    });
+    if (M == 0) {
+      fill(s, s->corr[0], pwords);
+      fill(s, s->corr[1], pwords);
+    }
+    s->corr_index = sweeps(s, f16, s->corr, 0, true, M, st, true);
+    timed(s, FIREX_PASS_PROJECT, [&] { launch_project(f16, flow0(s), s->corr[s->corr_index], flow1(s), st, q); });
+    s->flow_a = 1 - s->flow_a;
+  } else {
+    s->pressure_index = sweeps(s, f16, s->pressure, s->pressure_index, false, N, st, false);
+    timed(s, FIREX_PASS_PROJECT, [&] { launch_project(f16, flow1(s), s->pressure[s->pressure_index], flow0(s), st, q); });
+    timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow0(s), s->divergence, st, q); });
+    fill(s, s->corr[0], pwords);
+    fill(s, s->corr[1], pwords);
+    s->corr_index = sweeps(s, f16, s->corr, 0, false, M, st, false);
+    timed(s, FIREX_PASS_PROJECT, [&] { launch_project(f16, flow0(s), s->corr[s->corr_index], flow1(s), st, q); });
+    timed(s, FIREX_PASS_FILL_COPY, [&] { launch_copy_u128(flow0(s), flow1(s), s->cells, q); });
+  }
+  if (s->timing) {
+    s->totals.steps += 1;
+    if (s->pending.size() > 8192) {
+      int rc = collect_timing(s);
+      if (rc) return rc;
+    }
+  }
+  return check_launch(s, "solver step");
+}
+
+int step_with_dt(firex_solver* s, double dt) {
+  float u[FXU_COUNT];
+  write_uniforms(s->params, s->dims, dt, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
+                 s->cfg.cinematic != 0, u);
+  int rc = encode_step(s, u);
+  if (rc) return rc;
+  s->simulation_time += dt;
+  s->frame_index += 1;
+  return FIREX_OK;
+}
+
+int correction_buffer(firex_solver* s, void** p) {
+  *p = s->compact ? s->pressure[s->corr_index] : s->corr[s->corr_index];
+  return FIREX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int firex_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int firex_create(const firex_config* cfg, firex_solver** out) {
+  clear_error();
+  if (!cfg || !out) return set_error(FIREX_ERR_INVALID, "config and out must be non-null");
+  *out = nullptr;
+  for (int i = 0; i < 3; ++i) {
+    if (cfg->dims[i] < 2 || cfg->dims[i] > 1024)
+      return set_error(FIREX_ERR_UNSUPPORTED, "each grid axis must be in [2, 1024]");
+  }
+  if (cfg->pressure_iterations < 0 || cfg->correction_iterations < 0)
+    return set_error(FIREX_ERR_INVALID, "iteration counts must be non-negative");
+  if (!(cfg->time_step > 0) || !isfinite(cfg->time_step))
+    return set_error(FIREX_ERR_INVALID, "Fixed step must be finite and positive.");
+  if (cfg->maximum_substeps <= 0) return set_error(FIREX_ERR_INVALID, "Maximum step count must be a positive integer.");
+  if (cfg->species_storage < 0 || cfg->species_storage > 2) return set_error(FIREX_ERR_INVALID, "unknown species storage");
+  if (cfg->heavy_mode < 0 || cfg->heavy_mode > 3) return set_error(FIREX_ERR_INVALID, "unknown heavy mode");
+  if (cfg->plan < 0 || cfg->plan > 1) return set_error(FIREX_ERR_INVALID, "unknown plan");
+
+  bool heavy = cfg->heavy_mode == FIREX_HEAVY_AUTO ? heavy_grid(cfg->dims)
+                                                     : (cfg->heavy_mode == FIREX_HEAVY_F16 || cfg->heavy_mode == FIREX_HEAVY_F32);
+  bool f16p = heavy && cfg->heavy_mode != FIREX_HEAVY_F32;
+  bool compact = heavy && !f16p;
+  uint64_t need = device_bytes(cfg->dims, cfg->species_storage, f16p, compact, false);
+  if (cfg->memory_budget_bytes && need > cfg->memory_budget_bytes)
+    return set_error(FIREX_ERR_BUDGET, "Transactional rebuild peak requires " + std::to_string(need) +
+                                           " bytes, above the selected " + std::to_string(cfg->memory_budget_bytes) +
+                                           " byte guard.");
+
+  int ndev = firex_device_count();
+  if (ndev <= 0) return set_error(FIREX_ERR_DEVICE, "No HIP device is available for the Fire-X solver.");
+  if (cfg->device < 0 || cfg->device >= ndev) return set_error(FIREX_ERR_INVALID, "device ordinal out of range");
+
+  firex_solver* s = new firex_solver();
+  s->cfg = *cfg;
+  memcpy(s->dims, cfg->dims, sizeof s->dims);
+  s->cells = (uint32_t)((uint64_t)cfg->dims[0] * cfg->dims[1] * cfg->dims[2]);
+  s->heavy = heavy;
+  s->f16p = f16p;
+  s->compact = compact;
+  s->clock.step_seconds = cfg->time_step;
+  s->clock.maximum_steps = cfg->maximum_substeps;
+  s->bytes = need;
+
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_fail(nullptr, e, "hipSetDevice");
+  }
+  e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_fail(nullptr, e, "hipStreamCreate");
+  }
+  const uint64_t c = s->cells;
+  const size_t pbytes = c * (f16p ? 2 : 4);
+  int rc = FIREX_OK;
+  for (int k = 0; k < 2 && !rc; ++k) {
+    rc = alloc(s, (void**)&s->flow[k], c * 16, "flow");
+    if (!rc) rc = alloc(s, &s->species[k], c * species_word(cfg->species_storage), "species");
+    if (!rc) rc = alloc(s, &s->pressure[k], pbytes, "pressure");
+    if (!rc && !compact) rc = alloc(s, &s->corr[k], pbytes, "residual pressure");
+  }
+  if (!rc) rc = alloc(s, (void**)&s->divergence, c * 4, "divergence");
+  if (!rc) rc = alloc(s, (void**)&s->reaction, c * 4, "reaction");
+  if (!rc) rc = alloc(s, (void**)&s->scratch, kReduceScratchBytes, "reduction scratch");
+  if (!rc) {
+    s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch);
+    rc = initialize_fields(s);
+  }
+  if (!rc) {
+    e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = hip_fail(s, e, "field initialization");
+  }
+  if (rc) {
+    std::string msg = firex_last_error();
+    free_all(s);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return set_error(rc, msg);
+  }
+  *out = s;
+  return FIREX_OK;
+}
+
+void firex_destroy(firex_solver* s) {
+  if (!s) return;
+  hipSetDevice(s->cfg.device);
+  if (!s->lost) hipStreamSynchronize(s->stream);
+  for (const auto& t : s->pending) {
+    hipEventDestroy(t.start);
+    hipEventDestroy(t.stop);
+  }
+  for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
+  free_all(s);
+  hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int firex_reset(firex_solver* s) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  hipSetDevice(s->cfg.device);
+  s->clock.reset();
+  s->simulation_time = 0;
+  s->frame_index = 0;
+  s->species_index = 0;
+  s->pressure_index = 0;
+  return initialize_fields(s);
+}
+
+int firex_set_parameter(firex_solver* s, const char* name, double value) {
+  if (!s || !name) return set_error(FIREX_ERR_INVALID, "solver and name must be non-null");
+  params_set(s->params, name, value);  // unknown keys are ignored, as in the reference
+  return FIREX_OK;
+}
+
+int firex_set_parameter_bool(firex_solver* s, const char* name, int32_t value) {
+  if (!s || !name) return set_error(FIREX_ERR_INVALID, "solver and name must be non-null");
+  params_set_bool(s->params, name, value != 0);
+  return FIREX_OK;
+}
+
+int firex_get_parameter(firex_solver* s, const char* name, double* value) {
+  if (!s || !name || !value) return set_error(FIREX_ERR_INVALID, "solver, name and value must be non-null");
+  if (!params_get(s->params, name, value)) return set_error(FIREX_ERR_INVALID, std::string("unknown parameter: ") + name);
+  return FIREX_OK;
+}
+
+int firex_set_emitters(firex_solver* s, const firex_emitter* list, int32_t count, float jitter, float velocity_noise,
+                       uint64_t seed) {
+  if (!s) return set_error(FIREX_ERR_INVALID, "solver is null");
+  if (count < 0 || count > FX_MAX_EMITTERS) return set_error(FIREX_ERR_INVALID, "between 0 and 8 emitters");
+  if (count > 0 && !list) return set_error(FIREX_ERR_INVALID, "emitter list is null");
+  EmitterConfig e;
+  e.count = count;
+  for (int i = 0; i < count; ++i) {
+    if (!(list[i].radius > 0) || !isfinite(list[i].radius) || !isfinite(list[i].amplitude))
+      return set_error(FIREX_ERR_INVALID, "emitter radius must be positive and values finite");
+    e.data[i][0] = list[i].center[0];
+    e.data[i][1] = list[i].center[1];
+    e.data[i][2] = list[i].center[2];
+    e.data[i][3] = list[i].radius;
+    e.data[i][4] = list[i].amplitude;
+  }
+  e.jitter = jitter;
+  e.velocity_noise = velocity_noise;
+  e.seed = seed;
+  s->emitters = e;
+  return FIREX_OK;
+}
+
+int firex_step(firex_solver* s, double dt, int32_t count) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!isfinite(dt) || dt < 0) return set_error(FIREX_ERR_INVALID, "step dt must be finite and non-negative");
+  if (count < 0) return set_error(FIREX_ERR_INVALID, "count must be non-negative");
+  hipSetDevice(s->cfg.device);
+  for (int32_t i = 0; i < count; ++i) {
+    rc = step_with_dt(s, dt);
+    if (rc) return rc;
+  }
+  return FIREX_OK;
+}
+
+int firex_render_offline_frame(firex_solver* s, double delta, const volatile int32_t* abort_flag) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!isfinite(delta) || delta < 0)
+    return set_error(FIREX_ERR_INVALID, "Offline frame delta must be finite and non-negative.");
+  hipSetDevice(s->cfg.device);
+  s->clock.reset();
+  int32_t count = 0;
+  rc = firex_split_offline_delta(delta, s->cfg.time_step, nullptr, 0, &count);
+  if (rc) return rc;
+  const double step = count > 0 ? delta / (double)count : 0.0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (abort_flag && *abort_flag) return set_error(FIREX_ERR_ABORTED, "The Fire-X offline render was aborted.");
+    rc = step_with_dt(s, step);
+    if (rc) return rc;
+  }
+  return firex_sync(s);
+}
+
+int firex_frame(firex_solver* s, double delta, int32_t* steps_taken) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  hipSetDevice(s->cfg.device);
+  int32_t n = s->clock.advance(delta);
+  if (steps_taken) *steps_taken = n;
+  if (n == 0) return FIREX_OK;
+  // One uniform write covers every step of the frame (FireXEngine.ts:1089-1097).
+  float u[FXU_COUNT];
+  write_uniforms(s->params, s->dims, s->cfg.time_step, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
+                 s->cfg.cinematic != 0, u);
+  for (int32_t i = 0; i < n; ++i) {
+    rc = encode_step(s, u);
+    if (rc) return rc;
+    s->simulation_time += s->cfg.time_step;
+    s->frame_index += 1;
+  }
+  return FIREX_OK;
+}
+
+int firex_sync(firex_solver* s) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize");
+  return FIREX_OK;
+}
+
+int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, int32_t dtype) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!dst) return set_error(FIREX_ERR_INVALID, "dst is null");
+  hipSetDevice(s->cfg.device);
+  const uint64_t c = s->cells;
+  const void* src = nullptr;
+  int comps = 1;
+  int kind = 0;  // 0 f32, 1 f16, 2 bf16, 3 u32
+  switch (field) {
+    case FIREX_FIELD_FLOW: src = flow0(s); comps = 4; break;
+    case FIREX_FIELD_FLOW_PREDICTED:
+      // After a step the reference's flow[1] equals flow[0] (copyFlow); the
+      // fused plan swaps instead of copying, so both read the same buffer.
+      src = s->cfg.plan == FIREX_PLAN_FUSED && !(s->heavy && s->compact) && s->frame_index > 0 ? flow0(s) : flow1(s);
+      comps = 4;
+      break;
+    case FIREX_FIELD_SPECIES:
+      src = s->species[s->species_index];
+      comps = 4;
+      kind = s->cfg.species_storage;
+      break;
+    case FIREX_FIELD_PRESSURE: src = s->pressure[s->pressure_index]; kind = s->f16p ? 1 : 0; break;
+    case FIREX_FIELD_CORRECTION: {
+      void* p;
+      correction_buffer(s, &p);
+      src = p;
+      kind = s->f16p ? 1 : 0;
+      break;
+    }
+    case FIREX_FIELD_DIVERGENCE: src = s->divergence; break;
+    case FIREX_FIELD_REACTION: src = s->reaction; break;
+    case FIREX_FIELD_LIQUID: src = s->liquid; kind = 3; break;
+    default: return set_error(FIREX_ERR_INVALID, "unknown field");
+  }
+  const uint64_t n = c * comps;
+  size_t elem = dtype == FIREX_DTYPE_F64 ? 8 : 4;
+  if (dtype != FIREX_DTYPE_F32 && dtype != FIREX_DTYPE_F64 && dtype != FIREX_DTYPE_U32)
+    return set_error(FIREX_ERR_INVALID, "unknown dtype");
+  if ((kind == 3) != (dtype == FIREX_DTYPE_U32) && !(kind == 3 && dtype == FIREX_DTYPE_F64))
+    return set_error(FIREX_ERR_INVALID, "dtype does not match the field");
+  if (bytes != n * elem) return set_error(FIREX_ERR_INVALID, "byte count does not match the field size");
+  FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before read");
+  if (kind == 3 && !src) {
+    memset(dst, 0, bytes);  // no liquid field was ever written: it is zero
+    return FIREX_OK;
+  }
+  const size_t sbytes = n * (kind == 1 || kind == 2 ? 2 : 4);
+  std::vector<uint8_t> staging;
+  void* host = dst;
+  if (!(kind == 0 && dtype == FIREX_DTYPE_F32) && !(kind == 3 && dtype == FIREX_DTYPE_U32)) {
+    staging.resize(sbytes);
+    host = staging.data();
+  }
+  FX_HIP(s, hipMemcpy(host, src, sbytes, hipMemcpyDeviceToHost), "field readback");
+  if (host == dst) return FIREX_OK;
+  for (uint64_t i = 0; i < n; ++i) {
+    double v;
+    if (kind == 0) v = ((const float*)host)[i];
+    else if (kind == 1) v = (double)(float)((const _Float16*)host)[i];
+    else if (kind == 2) v = fx_bf16_bits_to_float(((const uint16_t*)host)[i]);
+    else v = (double)((const uint32_t*)host)[i];
+    if (dtype == FIREX_DTYPE_F64) ((double*)dst)[i] = v;
+    else ((float*)dst)[i] = (float)v;
+  }
+  return FIREX_OK;
+}
+
+int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t bytes, int32_t dtype) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!srcp) return set_error(FIREX_ERR_INVALID, "src is null");
+  hipSetDevice(s->cfg.device);
+  const uint64_t c = s->cells;
+  void* dst = nullptr;
+  int comps = 1, kind = 0;
+  switch (field) {
+    case FIREX_FIELD_FLOW: dst = flow0(s); comps = 4; break;
+    case FIREX_FIELD_FLOW_PREDICTED: dst = flow1(s); comps = 4; break;
+    case FIREX_FIELD_SPECIES: dst = s->species[s->species_index]; comps = 4; kind = s->cfg.species_storage; break;
+    case FIREX_FIELD_PRESSURE: dst = s->pressure[s->pressure_index]; kind = s->f16p ? 1 : 0; break;
+    case FIREX_FIELD_DIVERGENCE: dst = s->divergence; break;
+    case FIREX_FIELD_REACTION: dst = s->reaction; break;
+    case FIREX_FIELD_LIQUID:
+      kind = 3;
+      if (!s->liquid) {
+        rc = alloc(s, (void**)&s->liquid, c * 4, "liquid");
+        if (rc) return rc;
+        s->bytes += c * 4;
+      }
+      dst = s->liquid;
+      s->has_liquid = true;
+      break;
+    default: return set_error(FIREX_ERR_INVALID, "field is not writable");
+  }
+  if (dtype != FIREX_DTYPE_F32 && dtype != FIREX_DTYPE_F64 && dtype != FIREX_DTYPE_U32)
+    return set_error(FIREX_ERR_INVALID, "unknown dtype");
+  if (dtype == FIREX_DTYPE_U32 && kind != 3) return set_error(FIREX_ERR_INVALID, "dtype does not match the field");
+  const uint64_t n = c * comps;
+  size_t elem = dtype == FIREX_DTYPE_F64 ? 8 : 4;
+  if (bytes != n * elem) return set_error(FIREX_ERR_INVALID, "byte count does not match the field size");
+  const size_t sbytes = n * (kind == 1 || kind == 2 ? 2 : 4);
+  std::vector<uint8_t> staging(sbytes);
+  for (uint64_t i = 0; i < n; ++i) {
+    double v = dtype == FIREX_DTYPE_F64 ? ((const double*)srcp)[i]
+             : dtype == FIREX_DTYPE_U32 ? (double)((const uint32_t*)srcp)[i]
+                                         : (double)((const float*)srcp)[i];
+    if (kind == 0) ((float*)staging.data())[i] = (float)v;
+    else if (kind == 1) ((_Float16*)staging.data())[i] = (_Float16)(float)v;
+    else if (kind == 2) ((uint16_t*)staging.data())[i] = fx_float_to_bf16_bits((float)v);
+    else ((uint32_t*)staging.data())[i] = dtype == FIREX_DTYPE_U32 ? ((const uint32_t*)srcp)[i] : (uint32_t)v;
+  }
+  FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before write");
+  FX_HIP(s, hipMemcpy(dst, staging.data(), sbytes, hipMemcpyHostToDevice), "field upload");
+  return FIREX_OK;
+}
+
+int firex_reduce(firex_solver* s, int32_t which, double* out) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
+  hipSetDevice(s->cfg.device);
+  float u[FXU_COUNT];
+  write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
+                 s->cfg.cinematic != 0, u);
+  const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
+  double* partial = s->scratch;
+  double* result = s->scratch + kReducePartials;
+  switch (which) {
+    case FIREX_REDUCE_SUM_FUEL: case FIREX_REDUCE_SUM_OXYGEN: case FIREX_REDUCE_SUM_PRODUCTS: case FIREX_REDUCE_SUM_VAPOR:
+      launch_reduce_field(s->cfg.species_storage, 0, which - FIREX_REDUCE_SUM_FUEL, s->species[s->species_index], s->cells,
+                          partial, result, s->stream);
+      break;
+    case FIREX_REDUCE_SUM_TEMPERATURE:
+      launch_reduce_field(s->cfg.species_storage, 1, 3, flow0(s), s->cells, partial, result, s->stream);
+      break;
+    case FIREX_REDUCE_SUM_REACTION:
+      launch_reduce_field(s->cfg.species_storage, 2, 0, s->reaction, s->cells, partial, result, s->stream);
+      break;
+    case FIREX_REDUCE_RESIDUAL_L2: {
+      void* p;
+      correction_buffer(s, &p);
+      launch_reduce_residual(s->f16p ? 1 : 0, p, s->divergence, st, partial, result, s->stream);
+      break;
+    }
+    case FIREX_REDUCE_FLOW_DIVERGENCE_L2:
+      launch_reduce_flow_divergence(flow0(s), st, partial, result, s->stream);
+      break;
+    default: return set_error(FIREX_ERR_INVALID, "unknown reduction");
+  }
+  rc = check_launch(s, "reduction");
+  if (rc) return rc;
+  double v = 0;
+  FX_HIP(s, hipMemcpyAsync(&v, result, sizeof v, hipMemcpyDeviceToHost, s->stream), "reduction readback");
+  FX_HIP(s, hipStreamSynchronize(s->stream), "reduction");
+  if (which == FIREX_REDUCE_RESIDUAL_L2 || which == FIREX_REDUCE_FLOW_DIVERGENCE_L2) v = sqrt(v);
+  *out = v;
+  return FIREX_OK;
+}
+
+int firex_set_pass_timing(firex_solver* s, int32_t enabled) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  hipSetDevice(s->cfg.device);
+  rc = collect_timing(s);
+  if (rc) return rc;
+  memset(&s->totals, 0, sizeof s->totals);
+  s->timing = enabled != 0;
+  return FIREX_OK;
+}
+
+int firex_get_pass_timing(firex_solver* s, firex_pass_timing* out) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
+  hipSetDevice(s->cfg.device);
+  rc = collect_timing(s);
+  if (rc) return rc;
+  *out = s->totals;
+  return FIREX_OK;
+}
+
+int firex_get_info(firex_solver* s, firex_info* info) {
+  if (!s || !info) return set_error(FIREX_ERR_INVALID, "solver and info must be non-null");
+  memset(info, 0, sizeof *info);
+  memcpy(info->dims, s->dims, sizeof info->dims);
+  info->cell_count = s->cells;
+  info->simulation_time = s->simulation_time;
+  info->frame_index = (int64_t)s->frame_index;
+  info->species_index = s->species_index;
+  info->pressure_index = s->pressure_index;
+  info->heavy = s->heavy;
+  info->f16_pressure = s->f16p;
+  info->species_storage = s->cfg.species_storage;
+  info->plan = s->cfg.plan;
+  info->device_bytes = s->bytes;
+  return FIREX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// fx_common.h — data shared by the host side and the gfx950 kernels of the
+// Fire-X solver: the per-step constant block and the storage formats.
+#pragma once
+
+#include <stdint.h>
+
+#include "fx_math.h"
+
+// The 44-float uniform block of the reference (FireXEngine.ts:2047-2099;
+// WGSL struct Parameters, src/sim/firex/shaders.ts:2-14).  Slot indices:
+enum FxUniformSlot {
+  FXU_GRID_X = 0, FXU_GRID_Y = 1, FXU_GRID_Z = 2, FXU_CELLS = 3,
+  FXU_DT = 4, FXU_SIM_TIME = 5, FXU_FRAME = 6, FXU_TIME_STEP = 7,
+  FXU_FUEL_RATE = 8, FXU_OXYGEN_RATE = 9, FXU_HEAT_EFFICIENCY = 10, FXU_VORTICITY = 11,
+  FXU_WATER_FLOW = 12, FXU_SPRAY_ANGLE = 13, FXU_WATER_ENABLED = 14, FXU_AMBIENT = 15,
+  FXU_CAMERA_YAW = 16, FXU_CAMERA_PITCH = 17, FXU_CAMERA_DISTANCE = 18, FXU_ASPECT = 19,
+  FXU_CANVAS_W = 20, FXU_CANVAS_H = 21, FXU_RAY_STEPS = 22, FXU_PARTICLES = 23,
+  FXU_VIEW_MODE = 24, FXU_AIM_HEIGHT = 25, FXU_NOZZLE = 26, FXU_EXPOSURE = 27,
+  FXU_BURNER_SIZE = 28, FXU_BURNER_DEPTH = 29, FXU_SOURCE_THICKNESS = 30, FXU_FIRE_POWER = 31,
+  FXU_DOMAIN_X = 32, FXU_DOMAIN_Y = 33, FXU_DOMAIN_Z = 34,
+  FXU_SOURCE_LIFT = 36, FXU_BUOYANCY = 37, FXU_PERSISTENCE = 38, FXU_VIEW_ZOOM = 39,
+  FXU_ENTRAINMENT = 40, FXU_FRONT_DEFINITION = 41, FXU_CINEMATIC = 42, FXU_OPTICAL_DETAIL = 43,
+  FXU_COUNT = 44
+};
+
+#define FX_MAX_EMITTERS 8
+
+// Per-step constants handed to the kernels by value.  Every member is either
+// a uniform slot or a subexpression of the reference kernels that depends on
+// uniforms only; the host computes the latter with the same f32 operations,
+// in the same order, as the per-cell code of shaders.ts would.
+struct FxStep {
+  int32_t nx, ny, nz;
+  int32_t has_liquid;
+  float gx, gy, gz;            // params.grid.xyz
+  float dt, time;              // params.time.x, params.time.y
+  float mx, my, mz;            // cellMetric() = grid / domainExtent()
+  float ex, ey, ez;            // domainExtent() = max(domain, 0.001)
+  float ambient;               // params.water.w (300 K)
+  float chem_x, chem_y, chem_z, chem_w;
+  float mix_velocity;          // clamp(dt * 0.32, 0, 0.12)
+  float mix_temperature;       // clamp(dt * 0.18, 0, 0.08)
+  float mix_species;           // clamp(dt * 0.12, 0, 0.06)
+  float fire_power, power_root, thermal_power;
+  float swirl_tz, swirl_tx;    // time * 2.3, time * 1.9
+  float lift_clamp;            // clamp(fire.x, 0, 4)
+  float entrainment, ambient_oxygen;
+  float heat_release;          // 2150 + 1450 * chemistry.z
+  float max_dt;                // max(dt, 0.0001)
+  float vortex_k;              // 0.00135 * chemistry.w * dt
+  float turbulence_base;       // 0.075 + 0.16 * chemistry.w
+  float tt31, tt53, tt41, tt27, tt47;  // time * {3.1, 5.3, 4.1, 2.7, 4.7}
+  float buoyancy_clamp;        // clamp(fire.y, 0, 4)
+  float spray_x, spray_y, spray_z;     // normalize(sprayTarget - nozzleOrigin)
+  float persistence;           // clamp(fire.z, 0.4, 3)
+  float ambient4;              // pow(water.w, 4)
+  float radiation_denominator; // 1005 * persistence
+  float decay_products, decay_vapor;   // exp(-dt * 0.075 / q), exp(-dt * 0.14 / q)
+  // reference emitter (shaders.ts:101-133)
+  float em_cx, em_cy, em_cz;
+  float em_rx2, em_rz2, em_vdenom, em_rx_safe, em_rz_safe, em_cinematic;
+  float et52, et44, et17, et13, et37;  // time * {5.2, 4.4, 1.7, 1.3, 3.7}
+  // seeded emitters (addition)
+  int32_t seeded_count;
+  float seeded[FX_MAX_EMITTERS][5];    // cx cy cz radius amplitude (cells)
+  float jitter, velocity_noise;
+  uint64_t seed, frame;
+  // pressure stencil
+  float wx, wy, wz, jacobi_inv;        // metric^2, 1 / (2 (wx + wy + wz))
+};
+
+// Counter-based seeded noise: splitmix64 finaliser on (seed, frame, cell, stream).
+#if defined(__HIP__)
+__host__ __device__
+#endif
+static inline uint64_t fx_mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+#if defined(__HIP__)
+__host__ __device__
+#endif
+static inline float fx_seeded_uniform(uint64_t seed, uint64_t frame, uint64_t cell, uint32_t stream) {
+  uint64_t k = fx_mix64(seed ^ fx_mix64(frame));
+  k = fx_mix64(k ^ (cell * 4ull + stream));
+  return (float)(uint32_t)(k >> 40) * 5.9604644775390625e-08f;
+}
+
+// bf16 round-to-nearest-even (NaN becomes the canonical quiet NaN).
+#if defined(__HIP__)
+__host__ __device__
+#endif
+static inline uint16_t fx_float_to_bf16_bits(float v) {
+  uint32_t u = fxm_float_to_bits(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+#if defined(__HIP__)
+__host__ __device__
+#endif
+static inline float fx_bf16_bits_to_float(uint16_t b) { return fxm_bits_to_float((uint32_t)b << 16); }

@@ -1,0 +1,431 @@
+"""Python mirror of the Fire-X solver API over the C ABI of libfirex_hip.so.
+
+`FireXHipSolver` re-exposes the compute subset of the reference engine
+(class FireXEngine, src/sim/firex/FireXEngine.ts:370): construct with a grid
+and tier, `set_parameter` (FireXEngine.ts:937-1032), `step` /
+`render_offline_frame` (FireXEngine.ts:643-697) / `frame`
+(FireXEngine.ts:629-641, 1064-1101), `reset` (FireXEngine.ts:843-852) and
+field readback (new surface: the reference has no field readback).
+
+The module also mirrors the reference's host-side helpers that the solver
+relies on: the grid memory model and preflight
+(src/sim/firex/gridConfiguration.ts:272-350), the tier tables
+(FireXEngine.ts:187-277) and the offline timeline split
+(src/sim/offlineTimeline.ts:5-20).
+
+Every call goes to the native library; there is no Python fallback.  A
+missing library raises FireXLibraryError.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(HERE, "libfirex_hip.so")
+
+# status codes (include/firex.h)
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_BUDGET, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_ABORTED, ERR_DEVICE_LOST = range(8)
+
+FIELDS = {
+    "flow": 0, "species": 1, "pressure": 2, "correction": 3,
+    "divergence": 4, "reaction": 5, "liquid": 6, "flow_predicted": 7,
+}
+VECTOR_FIELDS = {"flow", "species", "flow_predicted"}
+DTYPE_F32, DTYPE_F64, DTYPE_U32 = 0, 1, 2
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+HEAVY = {"auto": 0, "off": 1, "f16": 2, "f32": 3}
+PLANS = {"reference": 0, "fused": 1}
+REDUCTIONS = {
+    "sum_fuel": 0, "sum_oxygen": 1, "sum_products": 2, "sum_vapor": 3,
+    "sum_temperature": 4, "sum_reaction": 5, "residual_l2": 6, "flow_divergence_l2": 7,
+}
+ISSUES = {
+    "invalid-dimension": 1 << 0, "unaligned-dimension": 1 << 1, "texture-dimension-limit": 1 << 2,
+    "storage-binding-limit": 1 << 3, "buffer-size-limit": 1 << 4, "memory-budget": 1 << 5,
+}
+
+
+class FireXError(RuntimeError):
+    """A FIREX_ERR_* status from the native library."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class FireXAbortError(FireXError):
+    """The offline render was aborted (the reference's AbortError)."""
+
+
+class FireXLibraryError(ImportError):
+    """libfirex_hip.so is not built or cannot be loaded."""
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int32 * 3),
+        ("pressure_iterations", ctypes.c_int32),
+        ("correction_iterations", ctypes.c_int32),
+        ("time_step", ctypes.c_double),
+        ("maximum_substeps", ctypes.c_int32),
+        ("species_storage", ctypes.c_int32),
+        ("heavy_mode", ctypes.c_int32),
+        ("plan", ctypes.c_int32),
+        ("cinematic", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("memory_budget_bytes", ctypes.c_uint64),
+        ("jacobi_sweeps_per_launch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class Emitter(ctypes.Structure):
+    _fields_ = [("center", ctypes.c_float * 3), ("radius", ctypes.c_float), ("amplitude", ctypes.c_float)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int32 * 3),
+        ("cell_count", ctypes.c_int64),
+        ("simulation_time", ctypes.c_double),
+        ("frame_index", ctypes.c_int64),
+        ("species_index", ctypes.c_int32),
+        ("pressure_index", ctypes.c_int32),
+        ("heavy", ctypes.c_int32),
+        ("f16_pressure", ctypes.c_int32),
+        ("species_storage", ctypes.c_int32),
+        ("plan", ctypes.c_int32),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+class MemoryEstimate(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "cell_count", "grid_bytes", "particle_bytes", "fixed_bytes", "total_bytes", "largest_storage_buffer_bytes")]
+
+
+class DeviceLimits(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "max_texture_dimension_3d", "max_storage_buffer_binding_size", "max_buffer_size")]
+
+
+PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy")
+
+
+class PassTiming(ctypes.Structure):
+    _fields_ = [
+        ("milliseconds", ctypes.c_double * 8),
+        ("launches", ctypes.c_int64 * 8),
+        ("jacobi_sweeps", ctypes.c_int64),
+        ("steps", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+# Exported symbols and their ctypes signatures (mirrors include/firex.h).
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "firex_last_error": (ctypes.c_char_p, []),
+    "firex_abi_version": (ctypes.c_int, []),
+    "firex_device_count": (ctypes.c_int, []),
+    "firex_config_init": (None, [ctypes.POINTER(Config)]),
+    "firex_config_from_tier": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_char_p]),
+    "firex_estimate_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, ctypes.POINTER(MemoryEstimate)]),
+    "firex_preflight": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(DeviceLimits), ctypes.c_int64,
+                                       ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32)]),
+    "firex_split_offline_delta": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "firex_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
+    "firex_destroy": (None, [_P]),
+    "firex_reset": (ctypes.c_int, [_P]),
+    "firex_set_parameter": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_double]),
+    "firex_set_parameter_bool": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
+    "firex_get_parameter": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
+    "firex_set_emitters": (ctypes.c_int, [_P, ctypes.POINTER(Emitter), ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_uint64]),
+    "firex_step": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int32]),
+    "firex_render_offline_frame": (ctypes.c_int, [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_int32)]),
+    "firex_frame": (ctypes.c_int, [_P, ctypes.c_double, ctypes.POINTER(ctypes.c_int32)]),
+    "firex_sync": (ctypes.c_int, [_P]),
+    "firex_read_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
+    "firex_write_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
+    "firex_reduce": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "firex_get_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
+    "firex_set_pass_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "firex_get_pass_timing": (ctypes.c_int, [_P, ctypes.POINTER(PassTiming)]),
+}
+
+
+def load_library() -> ctypes.CDLL:
+    """Load the in-tree libfirex_hip.so (raises FireXLibraryError if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY_PATH):
+        raise FireXLibraryError(
+            f"{LIBRARY_PATH} is not built; run `python -m fire_amd.build` (the solver has no CPU fallback)")
+    try:
+        lib = ctypes.CDLL(LIBRARY_PATH)
+    except OSError as exc:  # pragma: no cover - depends on the machine
+        raise FireXLibraryError(f"cannot load {LIBRARY_PATH}: {exc}") from exc
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = load_library().firex_last_error().decode(errors="replace")
+        if rc == ERR_ABORTED:
+            raise FireXAbortError(rc, msg)
+        raise FireXError(rc, msg)
+
+
+def device_count() -> int:
+    return int(load_library().firex_device_count())
+
+
+def abi_version() -> int:
+    return int(load_library().firex_abi_version())
+
+
+# ------------------------------------------------------------- host helpers
+def config_from_tier(tier: str) -> Config:
+    cfg = Config()
+    lib = load_library()
+    lib.firex_config_init(ctypes.byref(cfg))
+    _check(lib.firex_config_from_tier(ctypes.byref(cfg), tier.encode()))
+    return cfg
+
+
+@dataclass(frozen=True)
+class GridMemoryEstimate:
+    cell_count: int
+    grid_bytes: int
+    particle_bytes: int
+    fixed_bytes: int
+    total_bytes: int
+    largest_storage_buffer_bytes: int
+
+
+def estimate_memory(dims: Sequence[int], particle_count: int = 0) -> GridMemoryEstimate:
+    """estimateFireXGridMemory, gridConfiguration.ts:272-291."""
+    d = (ctypes.c_int32 * 3)(*[int(v) for v in dims])
+    out = MemoryEstimate()
+    _check(load_library().firex_estimate_memory(d, int(particle_count), ctypes.byref(out)))
+    return GridMemoryEstimate(**{n: int(getattr(out, n)) for n, _ in MemoryEstimate._fields_})
+
+
+def preflight(dims: Sequence[int], limits: dict, particle_count: int = 0,
+              memory_budget_bytes: Optional[int] = 512 * 1024 * 1024) -> list:
+    """preflightFireXGrid, gridConfiguration.ts:297-350: returns issue codes
+    (an empty list means ok).  memory_budget_bytes=None disables the guard."""
+    d = (ctypes.c_int32 * 3)(*[int(v) for v in dims])
+    lim = DeviceLimits(int(limits["maxTextureDimension3D"]), int(limits["maxStorageBufferBindingSize"]),
+                       int(limits["maxBufferSize"]))
+    bits = ctypes.c_uint32(0)
+    budget = -1 if memory_budget_bytes is None else int(memory_budget_bytes)
+    rc = load_library().firex_preflight(d, ctypes.byref(lim), int(particle_count), budget, ctypes.byref(bits))
+    if rc not in (OK, ERR_UNSUPPORTED):
+        _check(rc)
+    return [code for code, bit in ISSUES.items() if bits.value & bit]
+
+
+def split_offline_delta(delta_seconds: float, maximum_step_seconds: float) -> list:
+    """splitOfflineSimulationDelta, src/sim/offlineTimeline.ts:5-20."""
+    lib = load_library()
+    count = ctypes.c_int32(0)
+    rc = lib.firex_split_offline_delta(float(delta_seconds), float(maximum_step_seconds), None, 0, ctypes.byref(count))
+    if rc != OK:
+        raise ValueError(lib.firex_last_error().decode())
+    steps = (ctypes.c_double * max(1, count.value))()
+    _check(lib.firex_split_offline_delta(float(delta_seconds), float(maximum_step_seconds), steps, count.value,
+                                         ctypes.byref(count)))
+    return [steps[i] for i in range(count.value)]
+
+
+# ------------------------------------------------------------------- solver
+class FireXHipSolver:
+    """One resident plume on one MI355X, behind the C ABI."""
+
+    def __init__(self, dims: Optional[Sequence[int]] = None, tier: str = "balanced", *,
+                 pressure_iterations: Optional[int] = None, correction_iterations: Optional[int] = None,
+                 time_step: Optional[float] = None, maximum_substeps: Optional[int] = None,
+                 species_storage: str = "f32", heavy_mode: str = "auto", plan: str = "reference",
+                 cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
+                 jacobi_sweeps_per_launch: int = 0):
+        self._lib = load_library()
+        cfg = config_from_tier(tier)
+        if dims is not None:
+            for i in range(3):
+                cfg.dims[i] = int(dims[i])
+        if pressure_iterations is not None:
+            cfg.pressure_iterations = int(pressure_iterations)
+        if correction_iterations is not None:
+            cfg.correction_iterations = int(correction_iterations)
+        if time_step is not None:
+            cfg.time_step = float(time_step)
+        if maximum_substeps is not None:
+            cfg.maximum_substeps = int(maximum_substeps)
+        if cinematic is not None:
+            cfg.cinematic = int(bool(cinematic))
+        cfg.species_storage = STORAGE[species_storage]
+        cfg.heavy_mode = HEAVY[heavy_mode]
+        cfg.plan = PLANS[plan]
+        cfg.device = int(device)
+        cfg.memory_budget_bytes = int(memory_budget_bytes)
+        cfg.jacobi_sweeps_per_launch = int(jacobi_sweeps_per_launch)
+        self.config = cfg
+        self.dims = tuple(int(v) for v in cfg.dims)
+        self.cells = self.dims[0] * self.dims[1] * self.dims[2]
+        handle = ctypes.c_void_p()
+        _check(self._lib.firex_create(ctypes.byref(cfg), ctypes.byref(handle)))
+        self._h = handle
+
+    # lifetime ---------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.firex_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        _check(self._lib.firex_reset(self._h))
+
+    # parameters -------------------------------------------------------------
+    def set_parameter(self, name: str, value) -> None:
+        if isinstance(value, bool):
+            _check(self._lib.firex_set_parameter_bool(self._h, name.encode(), int(value)))
+        else:
+            _check(self._lib.firex_set_parameter(self._h, name.encode(), float(value)))
+
+    def get_parameter(self, name: str) -> float:
+        out = ctypes.c_double()
+        _check(self._lib.firex_get_parameter(self._h, name.encode(), ctypes.byref(out)))
+        return out.value
+
+    def set_emitters(self, emitters: Iterable[Sequence[float]], jitter: float = 0.0, velocity_noise: float = 0.0,
+                     seed: int = 0) -> None:
+        items = list(emitters)
+        arr = (Emitter * max(1, len(items)))()
+        for i, e in enumerate(items):
+            arr[i].center[0], arr[i].center[1], arr[i].center[2] = float(e[0]), float(e[1]), float(e[2])
+            arr[i].radius, arr[i].amplitude = float(e[3]), float(e[4])
+        _check(self._lib.firex_set_emitters(self._h, arr, len(items), float(jitter), float(velocity_noise),
+                                            int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    # stepping ---------------------------------------------------------------
+    def step(self, dt: float, count: int = 1) -> None:
+        _check(self._lib.firex_step(self._h, float(dt), int(count)))
+
+    def render_offline_frame(self, delta_seconds: float, abort_flag: Optional[ctypes.c_int32] = None) -> None:
+        ptr = ctypes.byref(abort_flag) if abort_flag is not None else None
+        _check(self._lib.firex_render_offline_frame(self._h, float(delta_seconds), ptr))
+
+    def frame(self, delta_seconds: float) -> int:
+        steps = ctypes.c_int32(0)
+        _check(self._lib.firex_frame(self._h, float(delta_seconds), ctypes.byref(steps)))
+        return steps.value
+
+    def sync(self) -> None:
+        _check(self._lib.firex_sync(self._h))
+
+    # fields -----------------------------------------------------------------
+    def _shape(self, name: str):
+        nx, ny, nz = self.dims
+        return (nz, ny, nx, 4) if name in VECTOR_FIELDS else (nz, ny, nx)
+
+    def read_field(self, name: str, dtype=np.float32) -> np.ndarray:
+        dt = np.dtype(dtype)
+        code = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.float64): DTYPE_F64,
+                np.dtype(np.uint32): DTYPE_U32}[dt]
+        out = np.empty(self._shape(name), dtype=dt)
+        _check(self._lib.firex_read_field(self._h, FIELDS[name], out.ctypes.data, out.nbytes, code))
+        return out
+
+    def write_field(self, name: str, values) -> None:
+        arr = np.asarray(values)
+        if arr.dtype == np.uint32:
+            code = DTYPE_U32
+        elif arr.dtype == np.float64:
+            code = DTYPE_F64
+        else:
+            arr = arr.astype(np.float32)
+            code = DTYPE_F32
+        arr = np.ascontiguousarray(arr.reshape(self._shape(name)))
+        _check(self._lib.firex_write_field(self._h, FIELDS[name], arr.ctypes.data, arr.nbytes, code))
+
+    def reduce(self, name: str) -> float:
+        out = ctypes.c_double()
+        _check(self._lib.firex_reduce(self._h, REDUCTIONS[name], ctypes.byref(out)))
+        return out.value
+
+    # instrumentation --------------------------------------------------------
+    def set_pass_timing(self, enabled: bool) -> None:
+        """HIP-event timing of every launch on the solver's stream (clears totals)."""
+        _check(self._lib.firex_set_pass_timing(self._h, int(bool(enabled))))
+
+    def pass_timing(self) -> dict:
+        t = PassTiming()
+        _check(self._lib.firex_get_pass_timing(self._h, ctypes.byref(t)))
+        out = {name: {"ms": t.milliseconds[i], "launches": int(t.launches[i])} for i, name in enumerate(PASSES)}
+        out["jacobi_sweeps"] = int(t.jacobi_sweeps)
+        out["steps"] = int(t.steps)
+        return out
+
+    @property
+    def info(self) -> dict:
+        inf = Info()
+        _check(self._lib.firex_get_info(self._h, ctypes.byref(inf)))
+        return {
+            "dims": tuple(inf.dims), "cell_count": inf.cell_count, "simulation_time": inf.simulation_time,
+            "frame_index": inf.frame_index, "species_index": inf.species_index,
+            "pressure_index": inf.pressure_index, "heavy": bool(inf.heavy), "f16_pressure": bool(inf.f16_pressure),
+            "species_storage": inf.species_storage, "plan": inf.plan, "device_bytes": int(inf.device_bytes),
+        }
+
+
+def seeded_emitters(dims: Sequence[int], count: int, radius_range=(6.0, 14.0), seed: int = 0,
+                    base_height: Optional[float] = None, amplitude: float = 1.0) -> list:
+    """Seeded emitter set near the grid base: centres uniform over the middle
+    half of x and z, radii uniform in radius_range (cells); splitmix64 stream."""
+    state = [seed & 0xFFFFFFFFFFFFFFFF]
+
+    def nxt() -> float:
+        state[0] = (state[0] + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = state[0]
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z ^= z >> 31
+        return (z >> 11) / float(1 << 53)
+
+    nx, ny, nz = dims
+    out = []
+    for _ in range(count):
+        r = radius_range[0] + (radius_range[1] - radius_range[0]) * nxt()
+        cx = nx * (0.25 + 0.5 * nxt())
+        cz = nz * (0.25 + 0.5 * nxt())
+        cy = base_height if base_height is not None else 1.5 * r / math.sqrt(2.0)
+        out.append((cx, cy, cz, r, amplitude))
+    return out

@@ -1,0 +1,248 @@
+/*
+ * firex.h — C ABI of the MI355X (gfx950) Fire-X solver library, libfirex_hip.so.
+ *
+ * The library is a drop-in for the compute half of the reference engine
+ * (class FireXEngine, src/sim/firex/FireXEngine.ts:370): it owns every device
+ * buffer of one plume, advances it with the Fire-X fixed step and hands fields
+ * back to the caller.  Rendering, droplets and video export are not part of it.
+ *
+ * Conventions
+ *   - Every function returning int returns FIREX_OK (0) or a FIREX_ERR_* code;
+ *     firex_last_error() then holds a human-readable message (thread-local).
+ *   - Fields are laid out x fastest: index = x + nx * (y + ny * z)
+ *     (cellIndex, src/sim/firex/shaders.ts:47-51).  Vector fields hold 4
+ *     components per cell.
+ *   - Plain pointers and sizes only; the caller owns every host buffer.
+ *   - One solver handle must not be used from two threads at once
+ *     (the reference is single-threaded, FireXEngine.ts:1882-1889).
+ */
+#ifndef FIREX_H
+#define FIREX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIREX_ABI_VERSION 1
+
+typedef struct firex_solver firex_solver;
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  FIREX_OK = 0,
+  FIREX_ERR_INVALID = 1,        /* bad argument (thrown Error in the reference) */
+  FIREX_ERR_UNSUPPORTED = 2,    /* grid fails preflight (gridConfiguration.ts:297-350) */
+  FIREX_ERR_BUDGET = 3,         /* transactional peak above the memory guard (FireXEngine.ts:2823-2853) */
+  FIREX_ERR_OUT_OF_MEMORY = 4,  /* device allocation failed; no partial state (FireXEngine.ts:1891-1915) */
+  FIREX_ERR_DEVICE = 5,         /* no usable HIP device / HIP call failed */
+  FIREX_ERR_ABORTED = 6,        /* caller's abort flag was set (FireXEngine.ts:2571-2574) */
+  FIREX_ERR_DEVICE_LOST = 7     /* sticky: a device fault was seen earlier (FireXEngine.ts:538-542) */
+};
+
+/* ---- fields ------------------------------------------------------------ */
+enum {
+  FIREX_FIELD_FLOW = 0,            /* vec4 f32 (u, v, w, T): flow[0] after a step        */
+  FIREX_FIELD_SPECIES = 1,         /* vec4 (fuel, O2, products, vapor): species[current]  */
+  FIREX_FIELD_PRESSURE = 2,        /* f32 primary pressure (warm-started)                 */
+  FIREX_FIELD_CORRECTION = 3,      /* f32 last residual-pressure solution                 */
+  FIREX_FIELD_DIVERGENCE = 4,      /* f32 divergence buffer (residual RHS after a step)   */
+  FIREX_FIELD_REACTION = 5,        /* f32 heat-release diagnostic                         */
+  FIREX_FIELD_LIQUID = 6,          /* u32 fixed-point liquid (x 4096), input of simulate  */
+  FIREX_FIELD_FLOW_PREDICTED = 7   /* vec4 f32 flow[1]                                    */
+};
+
+enum { FIREX_DTYPE_F32 = 0, FIREX_DTYPE_F64 = 1, FIREX_DTYPE_U32 = 2 };
+
+/* Storage of the advected species field. Velocity, temperature, pressure and
+ * all arithmetic stay f32. */
+enum { FIREX_STORAGE_F32 = 0, FIREX_STORAGE_F16 = 1, FIREX_STORAGE_BF16 = 2 };
+
+/* Pressure path on heavy grids (>= 64 Mi cells, gridConfiguration.ts:105). */
+enum {
+  FIREX_HEAVY_AUTO = 0,  /* reference selection: f16 pressure storage at >= 64 Mi cells */
+  FIREX_HEAVY_OFF = 1,   /* always the warm-started f32 path                           */
+  FIREX_HEAVY_F16 = 2,   /* force f16 pressure storage (pressureShaders.ts)            */
+  FIREX_HEAVY_F32 = 3    /* force the compact f32 path: shared buffers, cold start     */
+};
+
+/* Kernel plan.  Both plans give bit-identical fields. */
+enum {
+  FIREX_PLAN_REFERENCE = 0,  /* one launch per reference pass, 2N + 6 per step         */
+  FIREX_PLAN_FUSED = 1       /* temporally blocked Jacobi, fused projection/divergence */
+};
+
+typedef struct firex_config {
+  int32_t dims[3];               /* nx, ny, nz                                       */
+  int32_t pressure_iterations;   /* primary Jacobi sweeps N                          */
+  int32_t correction_iterations; /* residual Jacobi sweeps                           */
+  double time_step;              /* configured fixed step (QualityConfiguration)     */
+  int32_t maximum_substeps;      /* FixedStepClock steps per live frame              */
+  int32_t species_storage;       /* FIREX_STORAGE_*                                  */
+  int32_t heavy_mode;            /* FIREX_HEAVY_*                                    */
+  int32_t plan;                  /* FIREX_PLAN_*                                     */
+  int32_t cinematic;             /* uniform mixing.z: 1 for offline tiers / cinematic */
+  int32_t device;                /* HIP device ordinal                               */
+  uint64_t memory_budget_bytes;  /* guard on current + replacement bytes; 0 = none   */
+  int32_t jacobi_sweeps_per_launch; /* FUSED plan temporal depth; 0 = tuned default  */
+  int32_t reserved[7];
+} firex_config;
+
+/* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */
+typedef struct firex_emitter {
+  float center[3];   /* cell units; cell i spans [i, i+1)                  */
+  float radius;      /* cells; source = amplitude * exp(-|p - c|^2 / r^2)  */
+  float amplitude;
+} firex_emitter;
+
+typedef struct firex_info {
+  int32_t dims[3];
+  int64_t cell_count;
+  double simulation_time;       /* seconds of simulated time so far          */
+  int64_t frame_index;          /* steps taken since the last reset           */
+  int32_t species_index;        /* which species buffer is current            */
+  int32_t pressure_index;       /* which pressure buffer is current           */
+  int32_t heavy;                /* 1 if the >= 64 Mi cell path is active      */
+  int32_t f16_pressure;         /* 1 if pressure is stored as f16             */
+  int32_t species_storage;
+  int32_t plan;
+  uint64_t device_bytes;        /* bytes this solver holds on the device      */
+} firex_info;
+
+typedef struct firex_memory_estimate {
+  int64_t cell_count;
+  int64_t grid_bytes;
+  int64_t particle_bytes;
+  int64_t fixed_bytes;
+  int64_t total_bytes;
+  int64_t largest_storage_buffer_bytes;
+} firex_memory_estimate;
+
+typedef struct firex_device_limits {
+  int64_t max_texture_dimension_3d;
+  int64_t max_storage_buffer_binding_size;
+  int64_t max_buffer_size;
+} firex_device_limits;
+
+/* Preflight issue bits (codes of preflightFireXGrid, gridConfiguration.ts:297-350). */
+enum {
+  FIREX_ISSUE_INVALID_DIMENSION = 1u << 0,
+  FIREX_ISSUE_UNALIGNED_DIMENSION = 1u << 1,
+  FIREX_ISSUE_TEXTURE_DIMENSION_LIMIT = 1u << 2,
+  FIREX_ISSUE_STORAGE_BINDING_LIMIT = 1u << 3,
+  FIREX_ISSUE_BUFFER_SIZE_LIMIT = 1u << 4,
+  FIREX_ISSUE_MEMORY_BUDGET = 1u << 5
+};
+
+/* FP64 reductions (additions; the reference computes none). */
+enum {
+  FIREX_REDUCE_SUM_FUEL = 0,
+  FIREX_REDUCE_SUM_OXYGEN = 1,
+  FIREX_REDUCE_SUM_PRODUCTS = 2,
+  FIREX_REDUCE_SUM_VAPOR = 3,
+  FIREX_REDUCE_SUM_TEMPERATURE = 4,
+  FIREX_REDUCE_SUM_REACTION = 5,
+  FIREX_REDUCE_RESIDUAL_L2 = 6,       /* ||div - L(corr)||_2 over pressure unknowns  */
+  FIREX_REDUCE_FLOW_DIVERGENCE_L2 = 7 /* ||D(flow)||_2 of the current flow field      */
+};
+
+/* ---- library ------------------------------------------------------------ */
+const char* firex_last_error(void);
+int firex_abi_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int firex_device_count(void);
+
+/* ---- configuration (host only) ---------------------------------------- */
+/* Defaults: balanced live tier (FireXEngine.ts:199-208), f32 storage,
+ * reference plan, heavy auto, cinematic 0, device 0, no guard. */
+void firex_config_init(firex_config* config);
+/* Fill dims, iterations, time_step, maximum_substeps and cinematic from a tier
+ * name: live "low" "balanced" "high" "maximum" "cinematic"
+ * (FIREX_QUALITY, FireXEngine.ts:187-238) or offline "hd" "qhd" "uhd"
+ * (FIREX_OFFLINE_QUALITY, FireXEngine.ts:240-277). */
+int firex_config_from_tier(firex_config* config, const char* tier);
+
+/* estimateFireXGridMemory, gridConfiguration.ts:272-291. */
+int firex_estimate_memory(const int32_t dims[3], int64_t particle_count, firex_memory_estimate* out);
+/* preflightFireXGrid, gridConfiguration.ts:297-350.  memory_budget_bytes < 0
+ * disables the aggregate guard (the reference's null); *issues receives the
+ * FIREX_ISSUE_* bits; returns FIREX_OK when no bit is set. */
+int firex_preflight(const int32_t dims[3], const firex_device_limits* limits, int64_t particle_count,
+                    int64_t memory_budget_bytes, uint32_t* issues);
+/* splitOfflineSimulationDelta, src/sim/offlineTimeline.ts:5-20.  Writes at most
+ * cap steps; *count receives the full step count. */
+int firex_split_offline_delta(double delta_seconds, double maximum_step_seconds, double* steps, int32_t cap,
+                              int32_t* count);
+
+/* ---- solver lifetime --------------------------------------------------- */
+/* Allocate and initialise a plume (createResourceSet + initializeFields,
+ * FireXEngine.ts:1536-1974).  Fails without partial state. */
+int firex_create(const firex_config* config, firex_solver** out);
+void firex_destroy(firex_solver* solver);
+/* reset(), FireXEngine.ts:843-852. */
+int firex_reset(firex_solver* solver);
+
+/* setParameter(name, number), FireXEngine.ts:937-1032: same keys and clamps;
+ * non-finite values keep the previous value; unknown keys are ignored. */
+int firex_set_parameter(firex_solver* solver, const char* name, double value);
+/* setParameter(name, boolean) for "waterEnabled". */
+int firex_set_parameter_bool(firex_solver* solver, const char* name, int32_t value);
+int firex_get_parameter(firex_solver* solver, const char* name, double* value);
+/* Seeded emitters (count 0 restores the reference emitter; at most 8). */
+int firex_set_emitters(firex_solver* solver, const firex_emitter* emitters, int32_t count, float jitter,
+                       float velocity_noise, uint64_t seed);
+
+/* ---- stepping ----------------------------------------------------------- */
+/* count solver steps of dt seconds each (writeUniforms(dt) +
+ * encodeSimulationStep, FireXEngine.ts:2043-2276); enqueued, not waited for. */
+int firex_step(firex_solver* solver, double dt, int32_t count);
+/* renderOfflineFrame(delta, present = false), FireXEngine.ts:643-697: splits
+ * delta into exact steps no longer than time_step, runs them and waits.
+ * abort_flag may be NULL; when *abort_flag != 0 before a step, returns
+ * FIREX_ERR_ABORTED. */
+int firex_render_offline_frame(firex_solver* solver, double delta_seconds, const volatile int32_t* abort_flag);
+/* frame(t, dt) simulation half, FireXEngine.ts:1064-1101 with FixedStepClock
+ * (src/sim/FixedStepClock.ts:16-33); *steps_taken may be NULL. */
+int firex_frame(firex_solver* solver, double delta_seconds, int32_t* steps_taken);
+/* Wait for all enqueued work. */
+int firex_sync(firex_solver* solver);
+
+/* ---- fields -------------------------------------------------------------- */
+/* Copy a field to host memory (waits for enqueued steps).  bytes must equal
+ * cells * components * sizeof(dtype). */
+int firex_read_field(firex_solver* solver, int32_t field, void* dst, size_t bytes, int32_t dtype);
+/* Overwrite a field from host memory (seeding and tests). */
+int firex_write_field(firex_solver* solver, int32_t field, const void* src, size_t bytes, int32_t dtype);
+/* FP64 reduction on the device (FIREX_REDUCE_*). */
+int firex_reduce(firex_solver* solver, int32_t which, double* out);
+int firex_get_info(firex_solver* solver, firex_info* info);
+
+/* ---- instrumentation (additions: the reference has no GPU timestamps) -- */
+enum {
+  FIREX_PASS_SIMULATE = 0,            /* advection + sources                      */
+  FIREX_PASS_DIVERGENCE = 1,
+  FIREX_PASS_JACOBI = 2,              /* one entry per Jacobi launch              */
+  FIREX_PASS_PROJECT = 3,
+  FIREX_PASS_PROJECT_DIVERGENCE = 4,  /* fused projection + residual divergence   */
+  FIREX_PASS_FILL_COPY = 5,           /* clears and copyFlow                      */
+  FIREX_PASS_COUNT = 8
+};
+typedef struct firex_pass_timing {
+  double milliseconds[FIREX_PASS_COUNT];  /* summed device time per pass kind     */
+  int64_t launches[FIREX_PASS_COUNT];
+  int64_t jacobi_sweeps;                  /* sweeps covered by the Jacobi launches */
+  int64_t steps;
+} firex_pass_timing;
+/* Enable (1) or disable (0) HIP-event timing of every launch; enabling clears
+ * the totals.  Timing adds an event pair per launch on the solver's stream. */
+int firex_set_pass_timing(firex_solver* solver, int32_t enabled);
+/* Wait for the device and return the totals since timing was enabled. */
+int firex_get_pass_timing(firex_solver* solver, firex_pass_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIREX_H */

@@ -1,0 +1,930 @@
+// firex_oracle.cpp — CPU restatement of the Fire-X fixed-step solver.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP solver is
+// checked against.  Only the test suite, the smoke check and the CPU-baseline
+// leg of the benchmark load it; the product never does.
+//
+// It restates, kernel by kernel and in the same operation order, the WGSL of
+// the reference solver and the host step graph that drives it:
+//   * helpers gridDimensions/domainExtent/cellMetric/clampCell/cellIndex:
+//     src/sim/firex/shaders.ts:31-51
+//   * trilinearFlow/trilinearSpecies:            shaders.ts:61-95
+//   * emitter():                                 shaders.ts:101-133
+//   * initializeFields:                          shaders.ts:135-149
+//   * simulate (advection + sources):            shaders.ts:151-342
+//   * computeDivergence:                         shaders.ts:344-364
+//   * loadPressure/solvePressureGlobal/Tiled:    shaders.ts:366-459
+//   * pressureGradient/projectVelocity:          shaders.ts:461-501
+//   * copyFlow:                                  shaders.ts:503-511
+//   * f16 pressure storage (solvePressureF16Tiled/projectVelocityF16):
+//                                                src/sim/firex/pressureShaders.ts:49-160
+//   * uniform block packing (writeUniforms):     src/sim/firex/FireXEngine.ts:2043-2113
+//   * field reset (initializeFields host side):  FireXEngine.ts:1933-1974
+//   * pass order / ping-pong / warm start:       FireXEngine.ts:2115-2276
+//   * parameter defaults and clamps:             FireXEngine.ts:422-451, 937-1032
+//
+// Two precisions are instantiated.  State<double> is the FP64 judge: every
+// operation in double, libm transcendentals, true division.  State<float>
+// is the FP32 "reference step": f32 arithmetic in WGSL operation order with
+// the f32 built-in layer of fx_math.h standing in for the WGSL built-ins
+// (whose f32 results the WGSL spec leaves implementation-defined).
+//
+// Additions that the reference does not have (documented as such):
+//   * seeded emitters (Gaussian blobs with seeded jitter and seeded velocity
+//     noise) that replace emitter() when enabled;
+//   * species storage quantised to f16 or bf16 (mixed-precision storage);
+//   * FP64 residual and conservation sums.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "../fire_amd/csrc/fx_math.h"
+
+namespace {
+
+// ---------------------------------------------------------------- math policy
+template <typename T> struct Math;
+
+template <> struct Math<float> {
+  static float exp(float x) { return fxm_exp(x); }
+  static float sin(float x) { return fxm_sin(x); }
+  static float cos(float x) { return fxm_cos(x); }
+  static float pow(float x, float y) { return fxm_pow(x, y); }
+  static float sqrt(float x) { return fxm_sqrt(x); }
+  static float floor(float x) { return fxm_floor(x); }
+};
+
+template <> struct Math<double> {
+  static double exp(double x) { return std::exp(x); }
+  static double sin(double x) { return std::sin(x); }
+  static double cos(double x) { return std::cos(x); }
+  static double pow(double x, double y) { return std::pow(x, y); }
+  static double sqrt(double x) { return std::sqrt(x); }
+  static double floor(double x) { return std::floor(x); }
+};
+
+// WGSL min/max/clamp/mix/smoothstep, written as in fx_math.h.
+template <typename T> inline T wmin(T a, T b) { return (b < a) ? b : a; }
+template <typename T> inline T wmax(T a, T b) { return (a < b) ? b : a; }
+template <typename T> inline T wclamp(T x, T lo, T hi) { return wmin(wmax(x, lo), hi); }
+template <typename T> inline T wmix(T a, T b, T t) { return a * (T(1) - t) + b * t; }
+template <typename T> inline T wsmooth(T lo, T hi, T x) {
+  T t = wclamp((x - lo) / (hi - lo), T(0), T(1));
+  return t * t * (T(3) - T(2) * t);
+}
+
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> struct V4 { T x, y, z, w; };
+
+template <typename T> inline V4<T> mix4(const V4<T>& a, const V4<T>& b, T t) {
+  return {wmix(a.x, b.x, t), wmix(a.y, b.y, t), wmix(a.z, b.z, t), wmix(a.w, b.w, t)};
+}
+
+// ------------------------------------------------------------ f16 / bf16 rounding
+// Round-to-nearest-even conversions, used to model reduced-precision storage.
+inline float round_to_f16(float v) {
+  uint32_t u = fxm_float_to_bits(v);
+  uint32_t sign = u & 0x80000000u;
+  uint32_t au = u & 0x7fffffffu;
+  if (au >= 0x7f800000u) return v;  // inf / nan
+  float a = fxm_bits_to_float(au);
+  if (a >= 65520.0f) return fxm_bits_to_float(sign | 0x7f800000u);
+  float r;
+  if (a < 6.103515625e-05f) {
+    // f16 subnormal range: quantum 2^-24.
+    float q = a * 16777216.0f;             // exact scaling
+    float n = std::nearbyint(q);           // ties to even
+    r = n / 16777216.0f;
+  } else {
+    int e;
+    std::frexp(a, &e);                      // a = m * 2^e, m in [0.5, 1)
+    float quantum = std::ldexp(1.0f, e - 11);  // 11 significant bits
+    r = std::nearbyint(a / quantum) * quantum;
+  }
+  return fxm_bits_to_float(sign | fxm_float_to_bits(r));
+}
+
+inline float round_to_bf16(float v) {
+  uint32_t u = fxm_float_to_bits(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return fxm_bits_to_float(0x7fc00000u);  // canonical nan
+  uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return fxm_bits_to_float(u & 0xffff0000u);
+}
+
+// -------------------------------------------------------------- seeded noise
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+// Uniform in [0, 1) with 24 bits, from (seed, frame, cell, stream).
+inline float seeded_uniform(uint64_t seed, uint64_t frame, uint64_t cell, uint32_t stream) {
+  uint64_t k = mix64(seed ^ mix64(frame));
+  k = mix64(k ^ (cell * 4ull + stream));
+  return (float)(uint32_t)(k >> 40) * 5.9604644775390625e-08f;  // 2^-24
+}
+
+// -------------------------------------------------------------------- params
+// Engine parameter set, defaults from FireXEngine.ts:422-451 (Inferno).
+struct Params {
+  double burnerSize = 3.1, burnerDepth = 2.7, sourceThickness = 1.35, firePower = 2.5;
+  double sourceLift = 1.6, buoyancyScale = 2.15, flamePersistence = 1.45, airEntrainment = 1.65;
+  double fuelRate = 0.30, oxygenRate = 1, heatEfficiency = 0.86, vorticity = 3.6;
+  double waterFlow = 0, sprayAngle = 10;
+  bool waterEnabled = false;
+  double cameraYaw = -0.34, cameraPitch = 0.03, cameraDistance = 4.9;
+  double viewMode = 0, aimHeight = 0.42, nozzleType = 0, exposure = 1.25, frontDefinition = 1.65;
+  double domainWidth = 1.15, domainHeight = 1.35, domainDepth = 1.15, viewZoom = 2.1;
+  double opticalDetailTarget = 1024;
+};
+
+inline double finite_or(double v, double fallback) { return std::isfinite(v) ? v : fallback; }
+inline double clampd(double v, double lo, double hi) { return std::max(lo, std::min(hi, v)); }
+
+// FireXEngine.setParameter, FireXEngine.ts:937-1032 (numeric keys).
+bool set_parameter(Params& p, const char* name, double v) {
+  std::string n(name);
+  if (n == "burnerSize") p.burnerSize = clampd(finite_or(v, p.burnerSize), 0.35, 3.2);
+  else if (n == "burnerDepth") p.burnerDepth = clampd(finite_or(v, p.burnerDepth), 0.35, 3.2);
+  else if (n == "sourceThickness") p.sourceThickness = clampd(finite_or(v, p.sourceThickness), 0.35, 2.5);
+  else if (n == "firePower") p.firePower = clampd(finite_or(v, p.firePower), 0.1, 4);
+  else if (n == "sourceLift") p.sourceLift = clampd(finite_or(v, p.sourceLift), 0, 4);
+  else if (n == "buoyancyScale") p.buoyancyScale = clampd(finite_or(v, p.buoyancyScale), 0, 4);
+  else if (n == "flamePersistence") p.flamePersistence = clampd(finite_or(v, p.flamePersistence), 0.4, 3);
+  else if (n == "airEntrainment") p.airEntrainment = clampd(finite_or(v, p.airEntrainment), 0, 3);
+  else if (n == "fuelRate") p.fuelRate = finite_or(v, p.fuelRate);
+  else if (n == "oxygenRate") p.oxygenRate = finite_or(v, p.oxygenRate);
+  else if (n == "heatEfficiency") p.heatEfficiency = finite_or(v, p.heatEfficiency);
+  else if (n == "vorticity") p.vorticity = finite_or(v, p.vorticity);
+  else if (n == "waterFlow") p.waterFlow = finite_or(v, p.waterFlow);
+  else if (n == "sprayAngle") p.sprayAngle = finite_or(v, p.sprayAngle);
+  else if (n == "aimHeight") p.aimHeight = clampd(finite_or(v, p.aimHeight), 0.08, 0.82);
+  else if (n == "domainWidth") p.domainWidth = clampd(finite_or(v, p.domainWidth), 0.5, 4);
+  else if (n == "domainHeight") p.domainHeight = clampd(finite_or(v, p.domainHeight), 0.5, 6);
+  else if (n == "domainDepth") p.domainDepth = clampd(finite_or(v, p.domainDepth), 0.5, 4);
+  else return false;
+  return true;
+}
+
+// --------------------------------------------------------------- emitters
+struct SeededEmitter {
+  float cx, cy, cz;   // centre in cell units (cell i spans [i, i+1))
+  float radius;       // cells; profile exp(-|p - c|^2 / radius^2)
+  float amplitude;
+};
+
+struct EmitterSet {
+  int count = 0;            // 0 = reference analytic emitter
+  SeededEmitter e[8];
+  float jitter = 0;         // source *= 1 + jitter * (2u - 1)
+  float velocityNoise = 0;  // injected velocity noise amplitude
+  uint64_t seed = 0;
+};
+
+// ------------------------------------------------------------------- state
+enum { SPECIES_F32 = 0, SPECIES_F16 = 1, SPECIES_BF16 = 2 };
+enum { HEAVY_AUTO = 0, HEAVY_OFF = 1, HEAVY_F16 = 2, HEAVY_F32 = 3 };
+constexpr uint64_t kHeavyThreshold = 64ull * 1024 * 1024;  // gridConfiguration.ts:105
+
+template <typename T> struct State {
+  int nx = 0, ny = 0, nz = 0;
+  size_t n = 0;
+  std::vector<V4<T>> flow[2], species[2];
+  std::vector<T> pressure[2], corr[2], divergence, reaction;
+  std::vector<uint32_t> liquid;
+  int speciesIndex = 0, pressureIndex = 0;
+  double simulationTime = 0;
+  double frameIndex = 0;
+  int pressureIterations = 24, correctionIterations = 24;
+  double timeStep = 1.0 / 60.0;
+  bool cinematic = true;
+  Params params;
+  EmitterSet emitters;
+  int speciesStorage = SPECIES_F32;
+  int heavyMode = HEAVY_AUTO;
+  std::string log;
+  float u[44];  // uniform block as the device sees it
+
+  // ---- uniform-derived values (T precision)
+  T gx, gy, gz, dt, time, mx, my, mz, ex, ey, ez;
+
+  size_t idx(int x, int y, int z) const {
+    x = std::min(std::max(x, 0), nx - 1);
+    y = std::min(std::max(y, 0), ny - 1);
+    z = std::min(std::max(z, 0), nz - 1);
+    return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z);
+  }
+
+  bool heavy() const {
+    if (heavyMode == HEAVY_AUTO) return (uint64_t)n >= kHeavyThreshold;
+    return heavyMode == HEAVY_F16 || heavyMode == HEAVY_F32;
+  }
+  bool f16Pressure() const {
+    if (heavyMode == HEAVY_AUTO) return (uint64_t)n >= kHeavyThreshold;  // shader-f16 assumed
+    return heavyMode == HEAVY_F16;
+  }
+
+  // writeUniforms, FireXEngine.ts:2043-2113 (solver-relevant slots).
+  void write_uniforms(double deltaTime) {
+    std::memset(u, 0, sizeof u);
+    u[0] = (float)nx; u[1] = (float)ny; u[2] = (float)nz; u[3] = (float)n;
+    u[4] = (float)deltaTime; u[5] = (float)simulationTime; u[6] = (float)frameIndex; u[7] = (float)timeStep;
+    u[8] = (float)params.fuelRate; u[9] = (float)params.oxygenRate;
+    u[10] = (float)params.heatEfficiency; u[11] = (float)params.vorticity;
+    u[12] = (float)params.waterFlow; u[13] = (float)(params.sprayAngle * M_PI / 180.0);
+    u[14] = params.waterEnabled ? 1.0f : 0.0f; u[15] = 300.0f;
+    u[16] = (float)params.cameraYaw; u[17] = (float)params.cameraPitch; u[18] = (float)params.cameraDistance;
+    u[24] = (float)params.viewMode; u[25] = (float)params.aimHeight; u[26] = (float)params.nozzleType;
+    u[27] = (float)params.exposure;
+    u[28] = (float)params.burnerSize; u[29] = (float)params.burnerDepth;
+    u[30] = (float)params.sourceThickness; u[31] = (float)params.firePower;
+    u[32] = (float)params.domainWidth; u[33] = (float)params.domainHeight; u[34] = (float)params.domainDepth;
+    u[36] = (float)params.sourceLift; u[37] = (float)params.buoyancyScale;
+    u[38] = (float)params.flamePersistence; u[39] = (float)params.viewZoom;
+    u[40] = (float)params.airEntrainment; u[41] = (float)params.frontDefinition;
+    u[42] = cinematic ? 1.0f : 0.0f; u[43] = (float)params.opticalDetailTarget;
+    derive();
+  }
+
+  void derive() {
+    gx = T(u[0]); gy = T(u[1]); gz = T(u[2]);
+    dt = T(u[4]); time = T(u[5]);
+    ex = wmax(T(u[32]), T(0.001)); ey = wmax(T(u[33]), T(0.001)); ez = wmax(T(u[34]), T(0.001));
+    mx = gx / ex; my = gy / ey; mz = gz / ez;
+  }
+
+  // ------------------------------------------------------------- helpers
+  V4<T> trilinear(const std::vector<V4<T>>& f, T px, T py, T pz) const {
+    T sx = wclamp(px - T(0.5), T(0), gx - T(1.001));
+    T sy = wclamp(py - T(0.5), T(0), gy - T(1.001));
+    T sz = wclamp(pz - T(0.5), T(0), gz - T(1.001));
+    T flx = Math<T>::floor(sx), fly = Math<T>::floor(sy), flz = Math<T>::floor(sz);
+    int bx = (int)flx, by = (int)fly, bz = (int)flz;
+    T fx = sx - flx, fy = sy - fly, fz = sz - flz;
+    const V4<T>& c000 = f[idx(bx, by, bz)];
+    const V4<T>& c100 = f[idx(bx + 1, by, bz)];
+    const V4<T>& c010 = f[idx(bx, by + 1, bz)];
+    const V4<T>& c110 = f[idx(bx + 1, by + 1, bz)];
+    const V4<T>& c001 = f[idx(bx, by, bz + 1)];
+    const V4<T>& c101 = f[idx(bx + 1, by, bz + 1)];
+    const V4<T>& c011 = f[idx(bx, by + 1, bz + 1)];
+    const V4<T>& c111 = f[idx(bx + 1, by + 1, bz + 1)];
+    V4<T> z0 = mix4(mix4(c000, c100, fx), mix4(c010, c110, fx), fy);
+    V4<T> z1 = mix4(mix4(c001, c101, fx), mix4(c011, c111, fx), fy);
+    return mix4(z0, z1, fz);
+  }
+
+  // emitter(), shaders.ts:101-133.
+  T emitter_reference(T npx, T npy, T npz) const {
+    T t = time;
+    T px = npx * ex, py = npy * ey, pz = npz * ez;
+    T cx = ex * T(0.5) + Math<T>::sin(t * T(1.73)) * T(0.013);
+    T cy = T(0.075);
+    T cz = ez * T(0.5) + Math<T>::cos(t * T(1.37)) * T(0.011);
+    T ox = px - cx, oy = py - cy, oz = pz - cz;
+    T baseRadius = T(std::sqrt(0.02024));
+    T radiusX = wmin(baseRadius * wclamp(T(u[28]), T(0.35), T(3.2)), ex * T(0.42));
+    T radiusZ = wmin(baseRadius * wclamp(T(u[29]), T(0.35), T(3.2)), ez * T(0.42));
+    T thick = wclamp(T(u[30]), T(0.35), T(2.5));
+    T radial = ox * ox / (radiusX * radiusX) + oz * oz / (radiusZ * radiusZ);
+    T vertical = oy * oy / (T(0.0022) * thick * thick);
+    T cellular = T(0.76) + T(0.24) * Math<T>::sin(px * T(49) + t * T(5.2)) * Math<T>::sin(pz * T(43) - t * T(4.4));
+    T lobeArg = Math<T>::sin(ox / wmax(radiusX, T(0.001)) * T(6.2) + t * T(1.7))
+              * Math<T>::sin(oz / wmax(radiusZ, T(0.001)) * T(5.4) - t * T(1.3));
+    T lobes = wmix(T(0.48), T(1.18), wsmooth(T(-0.42), T(0.48), lobeArg));
+    T texture = wmix(cellular, cellular * lobes, wclamp(T(u[42]), T(0), T(1)));
+    T pulse = T(0.84) + T(0.16) * Math<T>::sin(t * T(3.7) + px * T(27) + pz * T(21));
+    return Math<T>::exp(-(radial + vertical)) * texture * pulse;
+  }
+
+  // Seeded Gaussian emitters (addition; replaces emitter() when enabled).
+  T emitter_seeded(T pcx, T pcy, T pcz, size_t cell) const {
+    T s = T(0);
+    for (int i = 0; i < emitters.count; ++i) {
+      const SeededEmitter& e = emitters.e[i];
+      T dx = pcx - T(e.cx), dy = pcy - T(e.cy), dz = pcz - T(e.cz);
+      T d2 = dx * dx + dy * dy + dz * dz;
+      T r = T(e.radius);
+      s = s + T(e.amplitude) * Math<T>::exp(-(d2 / (r * r)));
+    }
+    if (s > T(0) && emitters.jitter > 0.0f) {
+      T uj = T(seeded_uniform(emitters.seed, (uint64_t)frameIndex, cell, 0));
+      s = s * (T(1) + T(emitters.jitter) * (T(2) * uj - T(1)));
+    }
+    return s;
+  }
+
+  // ---------------------------------------------------------- kernels
+  // initializeFields host + kernel, FireXEngine.ts:1933-1974, shaders.ts:135-149.
+  void initialize_fields() {
+    for (int k = 0; k < 2; ++k) {
+      flow[k].assign(n, V4<T>{0, 0, 0, 0});
+      species[k].assign(n, V4<T>{0, 0, 0, 0});
+      pressure[k].assign(n, T(0));
+      corr[k].assign(n, T(0));
+    }
+    divergence.assign(n, T(0));
+    reaction.assign(n, T(0));
+    liquid.assign(n, 0u);
+    // initialization uniforms: grid, chemistry.y = oxygenRate, water.w = 300;
+    // flowOut = flow[0], speciesOut = species[0] (group at FireXEngine.ts:1667-1675).
+    T o2 = wmax(T((float)params.oxygenRate), T(0));
+    const V4<T> s0 = store_species(V4<T>{0, o2, 0, 0});
+    for (size_t i = 0; i < n; ++i) {
+      flow[0][i] = V4<T>{0, 0, 0, T(300)};
+      species[0][i] = s0;
+    }
+  }
+
+  void simulate(int sIn) {
+    const std::vector<V4<T>>& flowIn = flow[0];
+    const std::vector<V4<T>>& specIn = species[sIn];
+    std::vector<V4<T>>& flowOut = flow[1];
+    std::vector<V4<T>>& specOut = species[1 - sIn];
+    const T ambient = T(u[15]);
+    const T chemX = T(u[8]), chemY = T(u[9]), chemZ = T(u[10]), chemW = T(u[11]);
+    const T fireX = T(u[36]), fireY = T(u[37]), fireZ = T(u[38]);
+    const T mixingX = T(u[40]);
+    const T controlsY = T(u[25]);
+    const bool seeded = emitters.count > 0;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z) {
+      for (int y = 0; y < ny; ++y) {
+        for (int x = 0; x < nx; ++x) {
+          const size_t id = idx(x, y, z);
+          T pcx = T(x) + T(0.5), pcy = T(y) + T(0.5), pcz = T(z) + T(0.5);
+          T npx = pcx / gx, npy = pcy / gy, npz = pcz / gz;
+          const V4<T> orig = flowIn[id];
+          T bx = pcx - orig.x * dt * mx;
+          T by = pcy - orig.y * dt * my;
+          T bz = pcz - orig.z * dt * mz;
+          V4<T> fl = trilinear(flowIn, bx, by, bz);
+          V4<T> sp = trilinear(specIn, bx, by, bz);
+          sp = V4<T>{wmax(sp.x, T(0)), wmax(sp.y, T(0)), wmax(sp.z, T(0)), wmax(sp.w, T(0))};
+
+          const V4<T> L = flowIn[idx(x - 1, y, z)], R = flowIn[idx(x + 1, y, z)];
+          const V4<T> B = flowIn[idx(x, y - 1, z)], Tp = flowIn[idx(x, y + 1, z)];
+          const V4<T> K = flowIn[idx(x, y, z - 1)], F = flowIn[idx(x, y, z + 1)];
+          T vax = (L.x + R.x + B.x + Tp.x + K.x + F.x) / T(6);
+          T vay = (L.y + R.y + B.y + Tp.y + K.y + F.y) / T(6);
+          T vaz = (L.z + R.z + B.z + Tp.z + K.z + F.z) / T(6);
+          T mv = wclamp(dt * T(0.32), T(0), T(0.12));
+          fl.x = wmix(fl.x, vax, mv); fl.y = wmix(fl.y, vay, mv); fl.z = wmix(fl.z, vaz, mv);
+          T ta = (L.w + R.w + B.w + Tp.w + K.w + F.w) / T(6);
+          fl.w = wmix(fl.w, ta, wclamp(dt * T(0.18), T(0), T(0.08)));
+
+          const V4<T> sL = specIn[idx(x - 1, y, z)], sR = specIn[idx(x + 1, y, z)];
+          const V4<T> sB = specIn[idx(x, y - 1, z)], sT = specIn[idx(x, y + 1, z)];
+          const V4<T> sK = specIn[idx(x, y, z - 1)], sF = specIn[idx(x, y, z + 1)];
+          V4<T> sa{(sL.x + sR.x + sB.x + sT.x + sK.x + sF.x) / T(6),
+                   (sL.y + sR.y + sB.y + sT.y + sK.y + sF.y) / T(6),
+                   (sL.z + sR.z + sB.z + sT.z + sK.z + sF.z) / T(6),
+                   (sL.w + sR.w + sB.w + sT.w + sK.w + sF.w) / T(6)};
+          sp = mix4(sp, sa, wclamp(dt * T(0.12), T(0), T(0.06)));
+
+          T source = seeded ? emitter_seeded(pcx, pcy, pcz, id) : emitter_reference(npx, npy, npz);
+          T firePower = wclamp(T(u[31]), T(0.1), T(4));
+          T powerRoot = Math<T>::sqrt(firePower);
+          T thermalPower = wmin(powerRoot, T(1) + T(0.22) * (powerRoot - T(1)));
+          sp.x = wmax(sp.x, source * chemX * firePower);
+          fl.w = wmax(fl.w, ambient + source * T(1180) * thermalPower);
+          T swx = Math<T>::sin(time * T(2.3) + npz * T(22));
+          T swz = Math<T>::cos(time * T(1.9) + npx * T(25));
+          T swk = source * dt * T(0.11) * chemW * powerRoot;
+          fl.x = fl.x + swx * swk;
+          fl.y = fl.y + T(0) * swk;
+          fl.z = fl.z + swz * swk;
+          T lift = source * dt * T(0.52) * wclamp(fireX, T(0), T(4)) * powerRoot;
+          fl.y += lift;
+          if (seeded && source > T(0) && emitters.velocityNoise > 0.0f) {
+            T vn = T(emitters.velocityNoise);
+            T nxv = (T(2) * T(seeded_uniform(emitters.seed, (uint64_t)frameIndex, id, 1)) - T(1)) * vn;
+            T nyv = (T(2) * T(seeded_uniform(emitters.seed, (uint64_t)frameIndex, id, 2)) - T(1)) * vn;
+            T nzv = (T(2) * T(seeded_uniform(emitters.seed, (uint64_t)frameIndex, id, 3)) - T(1)) * vn;
+            fl.x = fl.x + nxv * lift;
+            fl.y = fl.y + nyv * lift;
+            fl.z = fl.z + nzv * lift;
+          }
+
+          bool ambientBoundary = x == 0 || x == nx - 1 || z == 0 || z == nz - 1 || y == ny - 1;
+          if (ambientBoundary) {
+            sp.y = wmax(sp.y, chemY);
+            fl.w = wmix(fl.w, ambient, T(0.035));
+          }
+
+          T liq = T((float)liquid[id]) * T(1.0 / 4096.0);
+          T water = wclamp(liq * T(3.4), T(0), T(5));
+          T evaporation = water * wsmooth(T(335), T(1040), fl.w) * dt * T(13.5);
+          fl.w = wmax(ambient, fl.w - evaporation * T(2080) - water * dt * T(260));
+          sp.w = wclamp(sp.w + evaporation * T(2.15), T(0), T(3));
+
+          T entrainment = wclamp(mixingX, T(0), T(3));
+          T ambientOxygen = wmax(chemY, T(0.001));
+          T oxygenDeficit = wclamp((ambientOxygen - sp.y) / ambientOxygen, T(0), T(1));
+          T fuelCore = wsmooth(T(0.30), T(0.78), sp.x);
+          T interfaceWeight = oxygenDeficit * wmix(T(1), T(0.18), fuelCore);
+          T speed = Math<T>::sqrt(fl.x * fl.x + fl.y * fl.y + fl.z * fl.z);
+          T entrainmentMix = wclamp(
+              dt * entrainment * (T(0.12) + T(0.28) * speed + T(0.22) * wsmooth(T(400), T(1500), fl.w)) * interfaceWeight,
+              T(0), T(0.08));
+          sp.y = wmix(sp.y, ambientOxygen, entrainmentMix);
+
+          T fuelBefore = wmax(sp.x, T(0));
+          T oxygenBefore = wmax(sp.y, T(0));
+          T availableFuel = wmin(fuelBefore, oxygenBefore * T(0.25));
+          T safeFuel = wmax(fuelBefore, T(0.00001));
+          T safeOxygen = wmax(oxygenBefore, T(0.00001));
+          T activation = Math<T>::exp(-T(15000) / wmax(fl.w, ambient));
+          T arrhenius = wclamp(T(2.5e6) * activation * Math<T>::pow(safeFuel, T(-0.30)) * Math<T>::pow(safeOxygen, T(1.30)),
+                               T(0), T(80));
+          T ignition = wsmooth(T(690), T(930), fl.w);
+          T requestedBurn = availableFuel * (T(1) - Math<T>::exp(-arrhenius * ignition * dt)) * Math<T>::exp(-water * T(18));
+          T burn = wclamp(requestedBurn, T(0), availableFuel);
+          T richness = wclamp(T(1) - oxygenBefore / wmax(fuelBefore * T(4), T(0.00001)), T(0), T(1));
+          sp.x = wmax(T(0), fuelBefore - burn);
+          sp.y = wmax(T(0), oxygenBefore - burn * T(4));
+          sp.z = wclamp(sp.z + burn * (T(1) + richness * T(1.8)), T(0), T(3));
+          sp.w = wclamp(sp.w + burn * T(0.62), T(0), T(3));
+          fl.w += burn * (T(2150) + T(1450) * chemZ);
+          T react = wclamp(burn / wmax(dt, T(0.0001)) * T(0.22), T(0), T(4));
+
+          T curlx = T(0.5) * ((Tp.z - B.z) * my - (F.y - K.y) * mz);
+          T curly = T(0.5) * ((F.x - K.x) * mz - (R.z - L.z) * mx);
+          T curlz = T(0.5) * ((R.y - L.y) * mx - (Tp.x - B.x) * my);
+          T gdx = T(0.5) * ((R.w - L.w) * mx) + T(0.00001);
+          T gdy = T(0.5) * ((Tp.w - B.w) * my) + T(0.00001);
+          T gdz = T(0.5) * ((F.w - K.w) * mz) + T(0.00001);
+          T glen = Math<T>::sqrt(gdx * gdx + gdy * gdy + gdz * gdz);
+          T dirx = gdx / glen, diry = gdy / glen, dirz = gdz / glen;
+          T vk = T(0.00135) * chemW * dt;
+          fl.x = fl.x + (diry * curlz - dirz * curly) * vk;
+          fl.y = fl.y + (dirz * curlx - dirx * curlz) * vk;
+          fl.z = fl.z + (dirx * curly - diry * curlx) * vk;
+
+          T hotPlume = wsmooth(T(520), T(1650), fl.w);
+          T heightEnvelope = wsmooth(T(0.045), T(0.15), npy) * (T(1) - wsmooth(T(0.80), T(1.0), npy));
+          T ltx = Math<T>::sin(npy * T(31) + npz * T(19) + time * T(3.1)) + T(0.45) * Math<T>::sin(npy * T(67) - time * T(5.3));
+          T lty = T(0.12) * Math<T>::sin(npx * T(29) + npz * T(23) - time * T(4.1));
+          T ltz = Math<T>::cos(npy * T(27) - npx * T(21) + time * T(2.7)) + T(0.45) * Math<T>::cos(npy * T(59) + time * T(4.7));
+          T turbulenceStrength = (T(0.075) + T(0.16) * chemW) * hotPlume * heightEnvelope;
+          T tk = turbulenceStrength * dt;
+          fl.x = fl.x + ltx * tk;
+          fl.y = fl.y + lty * tk;
+          fl.z = fl.z + ltz * tk;
+
+          T buoyancy = wclamp((fl.w - ambient) / T(1500), T(-0.1), T(1.2));
+          fl.y += buoyancy * dt * T(0.72) * wclamp(fireY, T(0), T(4));
+
+          sp.z *= Math<T>::exp(-water * dt * T(4.8));
+          T nox = T(0.035) * ex, noy = T(0.50) * ey, noz = T(0.5) * ez;
+          T stx = T(0.52) * ex, sty = controlsY * ey, stz = T(0.5) * ez;
+          T smx = stx - nox, smy = sty - noy, smz = stz - noz;
+          T slen = Math<T>::sqrt(smx * smx + smy * smy + smz * smz);
+          smx = smx / slen; smy = smy / slen; smz = smz / slen;
+          fl.x = fl.x + smx * water * dt * T(0.82);
+          fl.y = fl.y + smy * water * dt * T(0.82);
+          fl.z = fl.z + smz * water * dt * T(0.82);
+
+          T persistence = wclamp(fireZ, T(0.4), T(3));
+          T radiation = T(5.67e-8) * (Math<T>::pow(ambient, T(4)) - Math<T>::pow(wmax(fl.w, T(0)), T(4)))
+                        / (T(1005) * persistence);
+          fl.w = wmax(ambient, fl.w + dt * radiation);
+          sp.z *= Math<T>::exp(-dt * T(0.075) / persistence);
+          sp.w *= Math<T>::exp(-dt * T(0.14) / persistence);
+
+          T outflow = wsmooth(T(0.84), T(0.995), npy);
+          sp.x *= T(1) - outflow;
+          T outflowDecay = Math<T>::exp(-outflow * dt * T(15) / persistence);
+          sp.z *= outflowDecay;
+          sp.w *= outflowDecay;
+          fl.w = wmix(fl.w, ambient, outflow * T(0.16) / persistence);
+          react *= T(1) - outflow;
+          if (y == ny - 1) {
+            sp.x = T(0); sp.z = T(0); sp.w = T(0);
+            fl.w = ambient;
+            react = T(0);
+          }
+          if (x == 0 || x == nx - 1) fl.x = T(0);
+          if (z == 0 || z == nz - 1) fl.z = T(0);
+          if (y == 0) fl.y = T(0);
+          if (y == ny - 1) fl.y = wmax(T(0), fl.y);
+          fl.x = wclamp(fl.x, T(-1.4), T(1.4));
+          fl.y = wclamp(fl.y, T(-1.4), T(1.4));
+          fl.z = wclamp(fl.z, T(-1.4), T(1.4));
+          fl.w = wclamp(fl.w, ambient, T(2900));
+          sp.x = wclamp(sp.x, T(0), T(3)); sp.y = wclamp(sp.y, T(0), T(3));
+          sp.z = wclamp(sp.z, T(0), T(3)); sp.w = wclamp(sp.w, T(0), T(3));
+          flowOut[id] = fl;
+          specOut[id] = store_species(sp);
+          reaction[id] = react;
+        }
+      }
+    }
+  }
+
+  V4<T> store_species(const V4<T>& s) const;
+
+  // computeDivergence, shaders.ts:344-364.
+  void compute_divergence(const std::vector<V4<T>>& f) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const V4<T>& c = f[idx(x, y, z)];
+          T left = f[idx(x - 1, y, z)].x;
+          T bottom = f[idx(x, y - 1, z)].y;
+          T back = f[idx(x, y, z - 1)].z;
+          divergence[idx(x, y, z)] = (c.x - left) * mx + (c.y - bottom) * my + (c.z - back) * mz;
+        }
+  }
+
+  // loadPressure, shaders.ts:366-373 (pressureShaders.ts:49-56 for f16 storage).
+  T load_p(const std::vector<T>& p, int x, int y, int z) const {
+    if (y >= ny) return T(0);
+    return p[idx(x, y, z)];
+  }
+
+  // solvePressureGlobal / solvePressureTiled / solvePressureF16Tiled (one sweep).
+  // The f32 form multiplies by the f32 reciprocal of 2 (wx + wy + wz), one of
+  // the results WGSL's f32 division (2.5 ULP) admits; FP64 divides.
+  void jacobi_sweep(const std::vector<T>& pin, std::vector<T>& pout, bool f16Store) {
+    T wx = mx * mx, wy = my * my, wz = mz * mz;
+    T denom = T(2) * (wx + wy + wz);
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          size_t id = idx(x, y, z);
+          if (y == ny - 1) { pout[id] = T(0); continue; }
+          T nb = wx * (load_p(pin, x - 1, y, z) + load_p(pin, x + 1, y, z))
+               + wy * (load_p(pin, x, y - 1, z) + load_p(pin, x, y + 1, z))
+               + wz * (load_p(pin, x, y, z - 1) + load_p(pin, x, y, z + 1));
+          T v = jacobi_divide(nb - divergence[id], denom);
+          pout[id] = f16Store ? store_pressure_f16(v) : v;
+        }
+  }
+  T jacobi_divide(T num, T denom) const;
+  T store_pressure_f16(T v) const;
+
+  // pressureGradient + projectVelocity, shaders.ts:461-501.
+  void project(const std::vector<V4<T>>& fin, const std::vector<T>& p, std::vector<V4<T>>& fout) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          size_t id = idx(x, y, z);
+          auto grad = [&](int cx, int cy, int cz, T& gxo, T& gyo, T& gzo) {
+            T c = load_p(p, cx, cy, cz);
+            gxo = (load_p(p, cx + 1, cy, cz) - c) * mx;
+            gyo = (load_p(p, cx, cy + 1, cz) - c) * my;
+            gzo = (load_p(p, cx, cy, cz + 1) - c) * mz;
+          };
+          T g0, g1, g2;
+          grad(x, y, z, g0, g1, g2);
+          V4<T> f = fin[id];
+          f.x = f.x - g0; f.y = f.y - g1; f.z = f.z - g2;
+          if (y == ny - 1) {
+            const V4<T>& bf = fin[idx(x, y - 1, z)];
+            grad(x, y - 1, z, g0, g1, g2);
+            f.x = bf.x - g0; f.y = bf.y - g1; f.z = bf.z - g2;
+          }
+          if (x == 0 || x == nx - 1) f.x = T(0);
+          if (z == 0 || z == nz - 1) f.z = T(0);
+          if (y == 0) f.y = T(0);
+          if (y == ny - 1) f.y = wmax(T(0), f.y);
+          fout[id] = f;
+        }
+  }
+
+  // encodeSimulationStep, FireXEngine.ts:2115-2276 (water passes skipped:
+  // the liquid field is an input here).
+  void step(double deltaTime) {
+    write_uniforms(deltaTime);
+    log.clear();
+    log += "simulate;";
+    simulate(speciesIndex);
+    speciesIndex = 1 - speciesIndex;
+    log += "divergence;";
+    compute_divergence(flow[1]);
+    if (heavy()) {
+      if (f16Pressure()) {
+        for (int i = 0; i < pressureIterations; ++i) {
+          jacobi_sweep(pressure[pressureIndex], pressure[1 - pressureIndex], true);
+          pressureIndex = 1 - pressureIndex;
+        }
+        log += "jacobi_f16x" + std::to_string(pressureIterations) + ";project;";
+        project(flow[1], pressure[pressureIndex], flow[0]);
+        log += "divergence;clear_corr;";
+        compute_divergence(flow[0]);
+        std::fill(corr[0].begin(), corr[0].end(), T(0));
+        std::fill(corr[1].begin(), corr[1].end(), T(0));
+        int ci = 0;
+        for (int i = 0; i < correctionIterations; ++i) {
+          jacobi_sweep(corr[ci], corr[1 - ci], true);
+          ci = 1 - ci;
+        }
+        log += "jacobi_f16x" + std::to_string(correctionIterations) + ";project;";
+        project(flow[0], corr[ci], flow[1]);
+      } else {
+        // compact f32 fallback: two shared pressure fields, no warm start.
+        std::fill(pressure[0].begin(), pressure[0].end(), T(0));
+        std::fill(pressure[1].begin(), pressure[1].end(), T(0));
+        pressureIndex = 0;
+        log += "clear_pressure;";
+        for (int i = 0; i < pressureIterations; ++i) {
+          jacobi_sweep(pressure[pressureIndex], pressure[1 - pressureIndex], false);
+          pressureIndex = 1 - pressureIndex;
+        }
+        log += "jacobix" + std::to_string(pressureIterations) + ";project;";
+        project(flow[1], pressure[pressureIndex], flow[0]);
+        log += "divergence;clear_pressure;";
+        compute_divergence(flow[0]);
+        std::fill(pressure[0].begin(), pressure[0].end(), T(0));
+        std::fill(pressure[1].begin(), pressure[1].end(), T(0));
+        int ci = 0;
+        for (int i = 0; i < correctionIterations; ++i) {
+          jacobi_sweep(pressure[ci], pressure[1 - ci], false);
+          ci = 1 - ci;
+        }
+        log += "jacobix" + std::to_string(correctionIterations) + ";project;";
+        project(flow[0], pressure[ci], flow[1]);
+      }
+    } else {
+      for (int i = 0; i < pressureIterations; ++i) {
+        jacobi_sweep(pressure[pressureIndex], pressure[1 - pressureIndex], false);
+        pressureIndex = 1 - pressureIndex;
+      }
+      log += "jacobix" + std::to_string(pressureIterations) + ";project;";
+      project(flow[1], pressure[pressureIndex], flow[0]);
+      log += "divergence;clear_corr;";
+      compute_divergence(flow[0]);
+      std::fill(corr[0].begin(), corr[0].end(), T(0));
+      std::fill(corr[1].begin(), corr[1].end(), T(0));
+      int ci = 0;
+      for (int i = 0; i < correctionIterations; ++i) {
+        jacobi_sweep(corr[ci], corr[1 - ci], false);
+        ci = 1 - ci;
+      }
+      log += "jacobix" + std::to_string(correctionIterations) + ";project;";
+      project(flow[0], corr[ci], flow[1]);
+    }
+    log += "copy_flow;";
+    flow[0] = flow[1];
+    simulationTime += deltaTime;
+    frameIndex += 1;
+  }
+};
+
+template <> V4<float> State<float>::store_species(const V4<float>& s) const {
+  if (speciesStorage == SPECIES_F16)
+    return {round_to_f16(s.x), round_to_f16(s.y), round_to_f16(s.z), round_to_f16(s.w)};
+  if (speciesStorage == SPECIES_BF16)
+    return {round_to_bf16(s.x), round_to_bf16(s.y), round_to_bf16(s.z), round_to_bf16(s.w)};
+  return s;
+}
+template <> V4<double> State<double>::store_species(const V4<double>& s) const { return s; }
+
+template <> float State<float>::jacobi_divide(float num, float denom) const {
+  return num * (1.0f / denom);
+}
+template <> double State<double>::jacobi_divide(double num, double denom) const { return num / denom; }
+template <> float State<float>::store_pressure_f16(float v) const { return round_to_f16(v); }
+template <> double State<double>::store_pressure_f16(double v) const { return v; }
+
+struct Handle {
+  int precision;  // 32 or 64
+  State<float> f;
+  State<double> d;
+};
+
+enum Field {
+  F_FLOW = 0, F_SPECIES = 1, F_PRESSURE = 2, F_CORRECTION = 3, F_DIVERGENCE = 4,
+  F_REACTION = 5, F_LIQUID = 6, F_FLOW_PREDICTED = 7
+};
+
+template <typename T> size_t field_components(int field) {
+  return (field == F_FLOW || field == F_SPECIES || field == F_FLOW_PREDICTED) ? 4 : 1;
+}
+
+template <typename T> int read_field(State<T>& s, int field, double* out) {
+  size_t n = s.n;
+  switch (field) {
+    case F_FLOW: case F_FLOW_PREDICTED: {
+      const auto& f = s.flow[field == F_FLOW ? 0 : 1];
+      for (size_t i = 0; i < n; ++i) { out[4*i] = f[i].x; out[4*i+1] = f[i].y; out[4*i+2] = f[i].z; out[4*i+3] = f[i].w; }
+      return 0;
+    }
+    case F_SPECIES: {
+      const auto& f = s.species[s.speciesIndex];
+      for (size_t i = 0; i < n; ++i) { out[4*i] = f[i].x; out[4*i+1] = f[i].y; out[4*i+2] = f[i].z; out[4*i+3] = f[i].w; }
+      return 0;
+    }
+    case F_PRESSURE: for (size_t i = 0; i < n; ++i) out[i] = s.pressure[s.pressureIndex][i]; return 0;
+    case F_CORRECTION: {
+      // last residual solution: correctionIndex = correctionIterations % 2
+      // (f32 heavy fallback shares the pressure fields).
+      int ci = s.correctionIterations % 2;
+      const auto& c = (s.heavy() && !s.f16Pressure()) ? s.pressure[ci] : s.corr[ci];
+      for (size_t i = 0; i < n; ++i) out[i] = c[i];
+      return 0;
+    }
+    case F_DIVERGENCE: for (size_t i = 0; i < n; ++i) out[i] = s.divergence[i]; return 0;
+    case F_REACTION: for (size_t i = 0; i < n; ++i) out[i] = s.reaction[i]; return 0;
+    case F_LIQUID: for (size_t i = 0; i < n; ++i) out[i] = (double)s.liquid[i]; return 0;
+  }
+  return -1;
+}
+
+template <typename T> int write_field(State<T>& s, int field, const double* in) {
+  size_t n = s.n;
+  switch (field) {
+    case F_FLOW: case F_FLOW_PREDICTED: {
+      auto& f = s.flow[field == F_FLOW ? 0 : 1];
+      for (size_t i = 0; i < n; ++i) f[i] = V4<T>{T(in[4*i]), T(in[4*i+1]), T(in[4*i+2]), T(in[4*i+3])};
+      return 0;
+    }
+    case F_SPECIES: {
+      auto& f = s.species[s.speciesIndex];
+      for (size_t i = 0; i < n; ++i) f[i] = s.store_species(V4<T>{T(in[4*i]), T(in[4*i+1]), T(in[4*i+2]), T(in[4*i+3])});
+      return 0;
+    }
+    case F_PRESSURE: for (size_t i = 0; i < n; ++i) s.pressure[s.pressureIndex][i] = T(in[i]); return 0;
+    case F_DIVERGENCE: for (size_t i = 0; i < n; ++i) s.divergence[i] = T(in[i]); return 0;
+    case F_REACTION: for (size_t i = 0; i < n; ++i) s.reaction[i] = T(in[i]); return 0;
+    case F_LIQUID: for (size_t i = 0; i < n; ++i) s.liquid[i] = (uint32_t)in[i]; return 0;
+  }
+  return -1;
+}
+
+}  // namespace
+
+// =========================================================================== C API
+extern "C" {
+
+void* orc_create(int nx, int ny, int nz, int precision, int pressureIterations, int correctionIterations) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || (precision != 32 && precision != 64)) return nullptr;
+  Handle* h = new Handle();
+  h->precision = precision;
+  auto init = [&](auto& s) {
+    s.nx = nx; s.ny = ny; s.nz = nz; s.n = (size_t)nx * ny * nz;
+    s.pressureIterations = pressureIterations;
+    s.correctionIterations = correctionIterations;
+    s.initialize_fields();
+  };
+  if (precision == 32) init(h->f); else init(h->d);
+  return h;
+}
+
+void orc_destroy(void* p) { delete static_cast<Handle*>(p); }
+
+#define ORC_DISPATCH(h, expr) ((h)->precision == 32 ? ([&](auto& s) { return expr; })((h)->f) \
+                                                     : ([&](auto& s) { return expr; })((h)->d))
+
+int orc_set_parameter(void* p, const char* name, double value) {
+  Handle* h = static_cast<Handle*>(p);
+  return ORC_DISPATCH(h, set_parameter(s.params, name, value) ? 0 : 1);
+}
+
+int orc_set_bool(void* p, const char* name, int value) {
+  Handle* h = static_cast<Handle*>(p);
+  if (std::string(name) == "waterEnabled") { h->f.params.waterEnabled = value != 0; h->d.params.waterEnabled = value != 0; return 0; }
+  if (std::string(name) == "cinematic") { h->f.cinematic = value != 0; h->d.cinematic = value != 0; return 0; }
+  return 1;
+}
+
+double orc_get_parameter(void* p, const char* name) {
+  Handle* h = static_cast<Handle*>(p);
+  const Params& q = h->precision == 32 ? h->f.params : h->d.params;
+  std::string n(name);
+  if (n == "burnerSize") return q.burnerSize;
+  if (n == "burnerDepth") return q.burnerDepth;
+  if (n == "sourceThickness") return q.sourceThickness;
+  if (n == "firePower") return q.firePower;
+  if (n == "sourceLift") return q.sourceLift;
+  if (n == "buoyancyScale") return q.buoyancyScale;
+  if (n == "flamePersistence") return q.flamePersistence;
+  if (n == "airEntrainment") return q.airEntrainment;
+  if (n == "fuelRate") return q.fuelRate;
+  if (n == "oxygenRate") return q.oxygenRate;
+  if (n == "heatEfficiency") return q.heatEfficiency;
+  if (n == "vorticity") return q.vorticity;
+  if (n == "aimHeight") return q.aimHeight;
+  if (n == "domainWidth") return q.domainWidth;
+  if (n == "domainHeight") return q.domainHeight;
+  if (n == "domainDepth") return q.domainDepth;
+  return NAN;
+}
+
+void orc_set_time_step(void* p, double ts) { Handle* h = static_cast<Handle*>(p); h->f.timeStep = ts; h->d.timeStep = ts; }
+void orc_set_species_storage(void* p, int mode) { static_cast<Handle*>(p)->f.speciesStorage = mode; }
+void orc_set_heavy_mode(void* p, int mode) { Handle* h = static_cast<Handle*>(p); h->f.heavyMode = mode; h->d.heavyMode = mode; }
+
+int orc_set_emitters(void* p, int count, const float* data /* count*5: cx cy cz radius amplitude */,
+                     float jitter, float velocityNoise, uint64_t seed) {
+  if (count < 0 || count > 8) return 1;
+  Handle* h = static_cast<Handle*>(p);
+  EmitterSet e;
+  e.count = count;
+  for (int i = 0; i < count; ++i) e.e[i] = SeededEmitter{data[5*i], data[5*i+1], data[5*i+2], data[5*i+3], data[5*i+4]};
+  e.jitter = jitter; e.velocityNoise = velocityNoise; e.seed = seed;
+  h->f.emitters = e; h->d.emitters = e;
+  return 0;
+}
+
+void orc_reset(void* p) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.simulationTime = 0, s.frameIndex = 0, s.speciesIndex = 0, s.pressureIndex = 0,
+                   s.initialize_fields(), 0));
+}
+
+void orc_step(void* p, double dt, int count) {
+  Handle* h = static_cast<Handle*>(p);
+  for (int i = 0; i < count; ++i) ORC_DISPATCH(h, (s.step(dt), 0));
+}
+
+int orc_read_field(void* p, int field, double* out) {
+  Handle* h = static_cast<Handle*>(p);
+  return ORC_DISPATCH(h, read_field(s, field, out));
+}
+
+int orc_write_field(void* p, int field, const double* in) {
+  Handle* h = static_cast<Handle*>(p);
+  return ORC_DISPATCH(h, write_field(s, field, in));
+}
+
+double orc_simulation_time(void* p) { Handle* h = static_cast<Handle*>(p); return h->precision == 32 ? h->f.simulationTime : h->d.simulationTime; }
+int orc_species_index(void* p) { Handle* h = static_cast<Handle*>(p); return h->precision == 32 ? h->f.speciesIndex : h->d.speciesIndex; }
+int orc_pressure_index(void* p) { Handle* h = static_cast<Handle*>(p); return h->precision == 32 ? h->f.pressureIndex : h->d.pressureIndex; }
+
+int orc_pass_log(void* p, char* buf, int cap) {
+  Handle* h = static_cast<Handle*>(p);
+  const std::string& s = h->precision == 32 ? h->f.log : h->d.log;
+  int len = (int)s.size();
+  if (cap > 0) { int c = std::min(len, cap - 1); std::memcpy(buf, s.data(), c); buf[c] = 0; }
+  return len;
+}
+
+// The 44-float uniform block of the last step (writeUniforms).
+void orc_uniforms(void* p, double dt, float* out44) {
+  Handle* h = static_cast<Handle*>(p);
+  h->f.write_uniforms(dt);
+  std::memcpy(out44, h->f.u, sizeof h->f.u);
+}
+
+// ---- single-kernel entry points used by the unit tests -------------------
+// Apply one kernel to the state's current buffers, with uniforms for dt.
+void orc_kernel_divergence(void* p, double dt, int fromPredicted) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.write_uniforms(dt), s.compute_divergence(s.flow[fromPredicted ? 1 : 0]), 0));
+}
+// One Jacobi sweep pressure[pi] -> pressure[1-pi], then flip.
+void orc_kernel_jacobi(void* p, double dt, int f16) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.write_uniforms(dt), s.jacobi_sweep(s.pressure[s.pressureIndex], s.pressure[1 - s.pressureIndex], f16 != 0),
+                   s.pressureIndex = 1 - s.pressureIndex, 0));
+}
+// projectVelocity(flow[src], pressure[pi]) -> flow[1-src].
+void orc_kernel_project(void* p, double dt, int src) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.write_uniforms(dt), s.project(s.flow[src], s.pressure[s.pressureIndex], s.flow[1 - src]), 0));
+}
+// simulate(flow[0], species[si]) -> flow[1], species[1-si], reaction; flips si.
+void orc_kernel_simulate(void* p, double dt) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.write_uniforms(dt), s.simulate(s.speciesIndex), s.speciesIndex = 1 - s.speciesIndex, 0));
+}
+// trilinearFlow at an arbitrary position (cell units) on flow[0].
+void orc_trilinear_flow(void* p, double dt, double px, double py, double pz, double* out4) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.write_uniforms(dt), [&] {
+    using T = std::remove_reference_t<decltype(s.gx)>;
+    V4<T> v = s.trilinear(s.flow[0], T(px), T(py), T(pz));
+    out4[0] = v.x; out4[1] = v.y; out4[2] = v.z; out4[3] = v.w;
+  }(), 0));
+}
+// emitter() at a normalised position (reference analytic emitter).
+double orc_emitter(void* p, double dt, double npx, double npy, double npz) {
+  Handle* h = static_cast<Handle*>(p);
+  return ORC_DISPATCH(h, (s.write_uniforms(dt), (double)s.emitter_reference(
+      (decltype(s.gx))npx, (decltype(s.gx))npy, (decltype(s.gx))npz)));
+}
+
+float orc_round_f16(float v) { return round_to_f16(v); }
+float orc_round_bf16(float v) { return round_to_bf16(v); }
+float orc_seeded_uniform(uint64_t seed, uint64_t frame, uint64_t cell, uint32_t stream) {
+  return seeded_uniform(seed, frame, cell, stream);
+}
+float orc_f32_exp(float x) { return fxm_exp(x); }
+float orc_f32_sin(float x) { return fxm_sin(x); }
+float orc_f32_cos(float x) { return fxm_cos(x); }
+float orc_f32_pow(float x, float y) { return fxm_pow(x, y); }
+float orc_f32_log2(float x) { return fxm_log2(x); }
+float orc_f32_exp2(float x) { return fxm_exp2(x); }
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+"""ctypes front for the CPU oracle (test infrastructure only).
+
+The oracle restates the reference Fire-X step (src/sim/firex/shaders.ts,
+src/sim/firex/FireXEngine.ts) on the CPU in FP32 (WGSL operation order) and
+FP64.  Only the tests, the smoke check and the CPU-baseline leg of bench.py
+use it; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_firex.so")
+
+FIELD_FLOW, FIELD_SPECIES, FIELD_PRESSURE, FIELD_CORRECTION = 0, 1, 2, 3
+FIELD_DIVERGENCE, FIELD_REACTION, FIELD_LIQUID, FIELD_FLOW_PREDICTED = 4, 5, 6, 7
+VECTOR_FIELDS = (FIELD_FLOW, FIELD_SPECIES, FIELD_FLOW_PREDICTED)
+
+SPECIES_F32, SPECIES_F16, SPECIES_BF16 = 0, 1, 2
+HEAVY_AUTO, HEAVY_OFF, HEAVY_F16, HEAVY_F32 = 0, 1, 2, 3
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i, d, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float
+    u64, u32 = ctypes.c_uint64, ctypes.c_uint32
+    dp = ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
+    sig = {
+        "orc_create": (vp, [i, i, i, i, i, i]),
+        "orc_destroy": (None, [vp]),
+        "orc_set_parameter": (i, [vp, ctypes.c_char_p, d]),
+        "orc_set_bool": (i, [vp, ctypes.c_char_p, i]),
+        "orc_get_parameter": (d, [vp, ctypes.c_char_p]),
+        "orc_set_time_step": (None, [vp, d]),
+        "orc_set_species_storage": (None, [vp, i]),
+        "orc_set_heavy_mode": (None, [vp, i]),
+        "orc_set_emitters": (i, [vp, i, fp, f, f, u64]),
+        "orc_reset": (None, [vp]),
+        "orc_step": (None, [vp, d, i]),
+        "orc_read_field": (i, [vp, i, dp]),
+        "orc_write_field": (i, [vp, i, dp]),
+        "orc_simulation_time": (d, [vp]),
+        "orc_species_index": (i, [vp]),
+        "orc_pressure_index": (i, [vp]),
+        "orc_pass_log": (i, [vp, ctypes.c_char_p, i]),
+        "orc_uniforms": (None, [vp, d, fp]),
+        "orc_kernel_divergence": (None, [vp, d, i]),
+        "orc_kernel_jacobi": (None, [vp, d, i]),
+        "orc_kernel_project": (None, [vp, d, i]),
+        "orc_kernel_simulate": (None, [vp, d]),
+        "orc_trilinear_flow": (None, [vp, d, d, d, d, dp]),
+        "orc_emitter": (d, [vp, d, d, d, d]),
+        "orc_round_f16": (f, [f]),
+        "orc_round_bf16": (f, [f]),
+        "orc_seeded_uniform": (f, [u64, u64, u64, u32]),
+        "orc_f32_exp": (f, [f]),
+        "orc_f32_sin": (f, [f]),
+        "orc_f32_cos": (f, [f]),
+        "orc_f32_pow": (f, [f, f]),
+        "orc_f32_log2": (f, [f]),
+        "orc_f32_exp2": (f, [f]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class Oracle:
+    """One oracle solver instance (FP32 or FP64)."""
+
+    def __init__(self, dims, precision=64, pressure_iterations=24, correction_iterations=24,
+                 time_step=1.0 / 60.0):
+        self.lib = load()
+        self.dims = tuple(int(v) for v in dims)
+        self.cells = self.dims[0] * self.dims[1] * self.dims[2]
+        self.precision = precision
+        self.h = self.lib.orc_create(*self.dims, precision, pressure_iterations, correction_iterations)
+        if not self.h:
+            raise ValueError("invalid oracle configuration")
+        self.lib.orc_set_time_step(self.h, time_step)
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_parameter(self, name: str, value: float) -> None:
+        self.lib.orc_set_parameter(self.h, name.encode(), float(value))
+
+    def set_bool(self, name: str, value: bool) -> None:
+        self.lib.orc_set_bool(self.h, name.encode(), int(bool(value)))
+
+    def get_parameter(self, name: str) -> float:
+        return self.lib.orc_get_parameter(self.h, name.encode())
+
+    def set_species_storage(self, mode: int) -> None:
+        self.lib.orc_set_species_storage(self.h, mode)
+
+    def set_heavy_mode(self, mode: int) -> None:
+        self.lib.orc_set_heavy_mode(self.h, mode)
+
+    def set_emitters(self, emitters, jitter=0.0, velocity_noise=0.0, seed=0) -> None:
+        arr = np.ascontiguousarray(np.asarray(emitters, dtype=np.float32).reshape(-1, 5))
+        rc = self.lib.orc_set_emitters(self.h, arr.shape[0], arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       float(jitter), float(velocity_noise), int(seed))
+        if rc:
+            raise ValueError("at most 8 emitters")
+
+    def reset(self) -> None:
+        self.lib.orc_reset(self.h)
+
+    def step(self, dt: float, count: int = 1) -> None:
+        self.lib.orc_step(self.h, float(dt), int(count))
+
+    def _shape(self, field):
+        nx, ny, nz = self.dims
+        return (nz, ny, nx, 4) if field in VECTOR_FIELDS else (nz, ny, nx)
+
+    def read(self, field: int) -> np.ndarray:
+        shape = self._shape(field)
+        out = np.empty(shape, dtype=np.float64)
+        rc = self.lib.orc_read_field(self.h, field, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc:
+            raise ValueError(f"unknown field {field}")
+        return out
+
+    def write(self, field: int, values) -> None:
+        arr = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(self._shape(field)))
+        rc = self.lib.orc_write_field(self.h, field, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc:
+            raise ValueError(f"field {field} is not writable")
+
+    @property
+    def simulation_time(self) -> float:
+        return self.lib.orc_simulation_time(self.h)
+
+    @property
+    def species_index(self) -> int:
+        return self.lib.orc_species_index(self.h)
+
+    @property
+    def pressure_index(self) -> int:
+        return self.lib.orc_pressure_index(self.h)
+
+    def pass_log(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        self.lib.orc_pass_log(self.h, buf, 4096)
+        return buf.value.decode()
+
+    def uniforms(self, dt: float) -> np.ndarray:
+        out = np.zeros(44, dtype=np.float32)
+        self.lib.orc_uniforms(self.h, float(dt), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return out
+
+    # single kernels -------------------------------------------------------
+    def kernel_divergence(self, dt, from_predicted=False):
+        self.lib.orc_kernel_divergence(self.h, float(dt), int(from_predicted))
+
+    def kernel_jacobi(self, dt, f16=False):
+        self.lib.orc_kernel_jacobi(self.h, float(dt), int(f16))
+
+    def kernel_project(self, dt, src=0):
+        self.lib.orc_kernel_project(self.h, float(dt), int(src))
+
+    def kernel_simulate(self, dt):
+        self.lib.orc_kernel_simulate(self.h, float(dt))
+
+    def trilinear_flow(self, dt, position):
+        out = (ctypes.c_double * 4)()
+        self.lib.orc_trilinear_flow(self.h, float(dt), *[float(v) for v in position], out)
+        return np.array(out[:])
+
+    def emitter(self, dt, normalized_position):
+        return self.lib.orc_emitter(self.h, float(dt), *[float(v) for v in normalized_position])

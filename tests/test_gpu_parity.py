@@ -1,0 +1,194 @@
+"""GPU parity: the HIP solver against the FP32 oracle, bit for bit.
+
+The FP32 oracle restates the reference step (src/sim/firex/shaders.ts,
+FireXEngine.ts:2115-2276) in WGSL operation order; the HIP kernels keep the
+same f32 operations, so every field must match exactly (bitwise on the f32
+pattern; +0 and -0 count as different).  Grids are small enough for the
+oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+
+from fire_amd import FireXHipSolver, device_count, seeded_emitters
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+FIELD_PAIRS = [
+    ("flow", orc.FIELD_FLOW),
+    ("species", orc.FIELD_SPECIES),
+    ("pressure", orc.FIELD_PRESSURE),
+    ("correction", orc.FIELD_CORRECTION),
+    ("divergence", orc.FIELD_DIVERGENCE),
+    ("reaction", orc.FIELD_REACTION),
+]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if device_count() < 1:
+        pytest.fail("no HIP device visible to the GPU parity tests")
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def assert_fields_equal(gpu, oracle, label, names=None):
+    for name, fid in FIELD_PAIRS:
+        if names and name not in names:
+            continue
+        g = gpu.read_field(name)
+        o = oracle.read(fid).astype(np.float32)
+        if not np.array_equal(_bits(g), _bits(o)):
+            diff = np.abs(g.astype(np.float64) - o.astype(np.float64))
+            bad = np.argwhere(_bits(g) != _bits(o))
+            raise AssertionError(
+                f"{label}: field {name} differs in {len(bad)} of {g.size} values; max |diff| {diff.max():.3e}; "
+                f"first at {bad[0].tolist()} gpu={g[tuple(bad[0])]!r} oracle={o[tuple(bad[0])]!r}")
+
+
+def make_pair(dims, iters=(8, 8), plan="reference", storage="f32", heavy="off", sweeps=0, emitters=None,
+              params=None):
+    o = orc.Oracle(dims, precision=32, pressure_iterations=iters[0], correction_iterations=iters[1])
+    o.set_bool("cinematic", True)
+    if storage != "f32":
+        o.set_species_storage({"f16": orc.SPECIES_F16, "bf16": orc.SPECIES_BF16}[storage])
+        o.reset()
+    o.set_heavy_mode({"off": orc.HEAVY_OFF, "f16": orc.HEAVY_F16, "f32": orc.HEAVY_F32, "auto": orc.HEAVY_AUTO}[heavy])
+    g = FireXHipSolver(dims, tier="hd", pressure_iterations=iters[0], correction_iterations=iters[1], plan=plan,
+                       species_storage=storage, heavy_mode=heavy, jacobi_sweeps_per_launch=sweeps)
+    if emitters is not None:
+        em, jitter, noise, seed = emitters
+        o.set_emitters(em, jitter, noise, seed)
+        g.set_emitters(em, jitter, noise, seed)
+    for k, v in (params or {}).items():
+        o.set_parameter(k, v)
+        g.set_parameter(k, v)
+    return g, o
+
+
+def test_initial_state_matches():
+    g, o = make_pair((16, 24, 16))
+    assert_fields_equal(g, o, "after reset")
+    g.close()
+
+
+@pytest.mark.parametrize("dims", [(16, 24, 16), (20, 28, 12), (17, 29, 13)])
+def test_reference_plan_steps_bitwise(dims):
+    g, o = make_pair(dims, iters=(8, 8))
+    for step in range(3):
+        o.step(DT)
+        g.step(DT)
+        assert_fields_equal(g, o, f"reference plan {dims} step {step + 1}")
+    g.close()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_fused_plan_sweep_depths_bitwise(sweeps):
+    dims = (40, 72, 36)
+    g, o = make_pair(dims, iters=(11, 9), plan="fused", sweeps=sweeps)
+    for step in range(2):
+        o.step(DT)
+        g.step(DT)
+    assert_fields_equal(g, o, f"fused plan K={sweeps}", names=["flow", "species", "divergence", "reaction"])
+    # pressure/correction live in whichever buffer the last launch wrote
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    np.testing.assert_array_equal(_bits(g.read_field("correction")), _bits(o.read(orc.FIELD_CORRECTION)))
+    g.close()
+
+
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_hd_shape_many_steps_bitwise(plan):
+    dims = (64, 96, 64)
+    g, o = make_pair(dims, iters=(24, 24), plan=plan)
+    o.step(DT, 10)
+    g.step(DT, 10)
+    assert_fields_equal(g, o, f"{plan} 64x96x64 after 10 steps")
+    g.close()
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_reduced_precision_species_bitwise(storage, plan):
+    dims = (24, 36, 24)
+    g, o = make_pair(dims, iters=(8, 8), plan=plan, storage=storage)
+    o.step(DT, 4)
+    g.step(DT, 4)
+    assert_fields_equal(g, o, f"{storage} species {plan}")
+    g.close()
+
+
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_seeded_emitters_bitwise(plan):
+    dims = (32, 48, 32)
+    em = seeded_emitters(dims, 3, radius_range=(3.0, 7.0), seed=7)
+    g, o = make_pair(dims, iters=(8, 8), plan=plan, emitters=(em, 0.1, 0.5, 1234))
+    o.step(DT, 4)
+    g.step(DT, 4)
+    assert_fields_equal(g, o, f"seeded emitters {plan}")
+    g.close()
+
+
+@pytest.mark.parametrize("heavy", ["f16", "f32"])
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_heavy_paths_forced_on_small_grid_bitwise(heavy, plan):
+    dims = (24, 32, 20)
+    g, o = make_pair(dims, iters=(8, 8), plan=plan, heavy=heavy)
+    o.step(DT, 3)
+    g.step(DT, 3)
+    assert_fields_equal(g, o, f"heavy {heavy} {plan}", names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    np.testing.assert_array_equal(_bits(g.read_field("correction")), _bits(o.read(orc.FIELD_CORRECTION)))
+    g.close()
+
+
+def test_scene_parameters_bitwise():
+    dims = (24, 32, 24)
+    params = {"burnerSize": 1.0, "burnerDepth": 1.0, "sourceThickness": 1.0, "firePower": 1.0, "sourceLift": 1.0,
+              "fuelRate": 0.92, "oxygenRate": 0.42, "heatEfficiency": 0.62, "vorticity": 1.6, "buoyancyScale": 1.0,
+              "flamePersistence": 1.0, "airEntrainment": 0.0, "domainWidth": 1.0, "domainHeight": 1.0,
+              "domainDepth": 1.0}
+    g, o = make_pair(dims, iters=(6, 6), params=params)
+    # the oxygen level of the reset state follows oxygenRate
+    g.reset()
+    o.reset()
+    o.step(DT, 3)
+    g.step(DT, 3)
+    assert_fields_equal(g, o, "rich scene")
+    g.close()
+
+
+def test_liquid_input_bitwise():
+    dims = (16, 24, 16)
+    g, o = make_pair(dims, iters=(6, 6))
+    rng = np.random.default_rng(3)
+    liquid = rng.integers(0, 4096 * 2, size=(dims[2], dims[1], dims[0])).astype(np.uint32)
+    g.write_field("liquid", liquid)
+    o.write(orc.FIELD_LIQUID, liquid.astype(np.float64))
+    o.step(DT, 2)
+    g.step(DT, 2)
+    assert_fields_equal(g, o, "liquid cooling")
+    g.close()
+
+
+def test_seeded_state_write_and_step_bitwise():
+    dims = (20, 24, 16)
+    g, o = make_pair(dims, iters=(5, 7))
+    rng = np.random.default_rng(11)
+    flow = np.empty((dims[2], dims[1], dims[0], 4))
+    flow[..., :3] = rng.uniform(-1.2, 1.2, size=flow[..., :3].shape)
+    flow[..., 3] = rng.uniform(300, 2000, size=flow[..., 3].shape)
+    flow = flow.astype(np.float32).astype(np.float64)
+    species = rng.uniform(0, 1.5, size=(dims[2], dims[1], dims[0], 4)).astype(np.float32).astype(np.float64)
+    pressure = rng.normal(0, 1e-3, size=(dims[2], dims[1], dims[0])).astype(np.float32).astype(np.float64)
+    for name, fid, arr in (("flow", orc.FIELD_FLOW, flow), ("species", orc.FIELD_SPECIES, species),
+                           ("pressure", orc.FIELD_PRESSURE, pressure)):
+        o.write(fid, arr)
+        g.write_field(name, arr)
+    assert_fields_equal(g, o, "after write", names=["flow", "species", "pressure"])
+    o.step(DT)
+    g.step(DT)
+    assert_fields_equal(g, o, "random state, one step")
+    g.close()
