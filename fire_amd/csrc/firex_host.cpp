@@ -68,30 +68,38 @@ bool params_set_bool(Params& p, const char* name, bool value) {
   return false;
 }
 
-bool params_get(const Params& p, const char* name, double* value) {
+double* params_field(Params& p, const char* name) {
   const std::string n(name ? name : "");
-  const struct {
+  struct Entry {
     const char* key;
-    double v;
-  } table[] = {
-      {"burnerSize", p.burnerSize}, {"burnerDepth", p.burnerDepth}, {"sourceThickness", p.sourceThickness},
-      {"firePower", p.firePower}, {"sourceLift", p.sourceLift}, {"buoyancyScale", p.buoyancyScale},
-      {"flamePersistence", p.flamePersistence}, {"airEntrainment", p.airEntrainment}, {"fuelRate", p.fuelRate},
-      {"oxygenRate", p.oxygenRate}, {"heatEfficiency", p.heatEfficiency}, {"vorticity", p.vorticity},
-      {"waterFlow", p.waterFlow}, {"sprayAngle", p.sprayAngle}, {"waterEnabled", p.waterEnabled ? 1.0 : 0.0},
-      {"cameraYaw", p.cameraYaw}, {"cameraPitch", p.cameraPitch}, {"cameraDistance", p.cameraDistance},
-      {"viewMode", p.viewMode}, {"aimHeight", p.aimHeight}, {"nozzleType", p.nozzleType},
-      {"exposure", p.exposure}, {"frontDefinition", p.frontDefinition},
-      {"opticalDetailTarget", p.opticalDetailTarget}, {"domainWidth", p.domainWidth},
-      {"domainHeight", p.domainHeight}, {"domainDepth", p.domainDepth}, {"viewZoom", p.viewZoom},
+    double* v;
   };
-  for (const auto& e : table) {
-    if (n == e.key) {
-      *value = e.v;
-      return true;
-    }
+  const Entry table[] = {
+      {"burnerSize", &p.burnerSize}, {"burnerDepth", &p.burnerDepth}, {"sourceThickness", &p.sourceThickness},
+      {"firePower", &p.firePower}, {"sourceLift", &p.sourceLift}, {"buoyancyScale", &p.buoyancyScale},
+      {"flamePersistence", &p.flamePersistence}, {"airEntrainment", &p.airEntrainment}, {"fuelRate", &p.fuelRate},
+      {"oxygenRate", &p.oxygenRate}, {"heatEfficiency", &p.heatEfficiency}, {"vorticity", &p.vorticity},
+      {"waterFlow", &p.waterFlow}, {"sprayAngle", &p.sprayAngle}, {"cameraYaw", &p.cameraYaw},
+      {"cameraPitch", &p.cameraPitch}, {"cameraDistance", &p.cameraDistance}, {"viewMode", &p.viewMode},
+      {"aimHeight", &p.aimHeight}, {"nozzleType", &p.nozzleType}, {"exposure", &p.exposure},
+      {"frontDefinition", &p.frontDefinition}, {"opticalDetailTarget", &p.opticalDetailTarget},
+      {"domainWidth", &p.domainWidth}, {"domainHeight", &p.domainHeight}, {"domainDepth", &p.domainDepth},
+      {"viewZoom", &p.viewZoom},
+  };
+  for (const auto& e : table)
+    if (n == e.key) return e.v;
+  return nullptr;
+}
+
+bool params_get(const Params& p, const char* name, double* value) {
+  if (name && std::string(name) == "waterEnabled") {
+    *value = p.waterEnabled ? 1.0 : 0.0;
+    return true;
   }
-  return false;
+  double* f = params_field(const_cast<Params&>(p), name);
+  if (!f) return false;
+  *value = *f;
+  return true;
 }
 
 // ---------------------------------------------------------------- uniforms
@@ -392,6 +400,43 @@ int firex_split_offline_delta(double delta, double maximum_step, double* steps, 
   for (int32_t i = 0; i < k && i < cap; ++i) steps[i] = step;
   *count = k;
   return FIREX_OK;
+}
+
+int32_t firex_fixed_step_clock_advance(double step_seconds, int32_t maximum_steps, double* accumulator,
+                                       double delta_seconds) {
+  if (!accumulator || !std::isfinite(step_seconds) || !(step_seconds > 0) || maximum_steps < 1) {
+    set_error(FIREX_ERR_INVALID, "Fixed step must be finite and positive; maximum step count a positive integer.");
+    return -1;
+  }
+  FixedStepClock c;
+  c.step_seconds = step_seconds;
+  c.maximum_steps = maximum_steps;
+  c.accumulator = *accumulator;
+  int32_t n = c.advance(delta_seconds);
+  *accumulator = c.accumulator;
+  return n;
+}
+
+int firex_clamp_parameter(const char* name, double value, double previous, double* out) {
+  if (!name || !out) return set_error(FIREX_ERR_INVALID, "name and out must be non-null");
+  Params p;
+  double* field = params_field(p, name);
+  if (!field) return set_error(FIREX_ERR_INVALID, std::string("unknown parameter: ") + name);
+  *field = previous;  // the stored value, as it stands before the call
+  params_set(p, name, value);
+  *out = *field;
+  return FIREX_OK;
+}
+
+int firex_resolve_offline_dims(const char* tier, const int32_t* override_dims, int32_t out[3]) {
+  if (!tier || !out) return set_error(FIREX_ERR_INVALID, "tier and out must be non-null");
+  for (const Tier& t : kTiers) {
+    if (strcmp(t.name, tier) == 0 && t.time_step == 1.0 / 60.0) {
+      for (int i = 0; i < 3; ++i) out[i] = override_dims ? std::max(t.dims[i], override_dims[i]) : t.dims[i];
+      return FIREX_OK;
+    }
+  }
+  return set_error(FIREX_ERR_INVALID, std::string("unknown offline tier: ") + tier);
 }
 
 }  // extern "C"

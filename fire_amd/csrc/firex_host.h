@@ -31,6 +31,7 @@ struct Params {
 bool params_set(Params& p, const char* name, double value);
 bool params_set_bool(Params& p, const char* name, bool value);
 bool params_get(const Params& p, const char* name, double* value);
+double* params_field(Params& p, const char* name);  // numeric keys only
 
 // writeUniforms, FireXEngine.ts:2043-2113.
 void write_uniforms(const Params& p, const int32_t dims[3], double dt, double simulation_time, double frame_index,

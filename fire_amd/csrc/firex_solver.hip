@@ -344,7 +344,9 @@ int firex_create(const firex_config* cfg, firex_solver** out) {
     return hip_fail(nullptr, e, "hipStreamCreate");
   }
   const uint64_t c = s->cells;
-  const size_t pbytes = c * (f16p ? 2 : 4);
+  // Rounded up to whole 32-bit words: the fill kernel clears words
+  // (the reference rounds the same way, FireXEngine.ts:1560).
+  const size_t pbytes = (c * (f16p ? 2 : 4) + 3) / 4 * 4;
   int rc = FIREX_OK;
   for (int k = 0; k < 2 && !rc; ++k) {
     rc = alloc(s, (void**)&s->flow[k], c * 16, "flow");

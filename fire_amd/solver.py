@@ -142,6 +142,12 @@ _SIGNATURES = {
                                        ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32)]),
     "firex_split_offline_delta": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                                  ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "firex_fixed_step_clock_advance": (ctypes.c_int32, [ctypes.c_double, ctypes.c_int32,
+                                                        ctypes.POINTER(ctypes.c_double), ctypes.c_double]),
+    "firex_clamp_parameter": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                             ctypes.POINTER(ctypes.c_double)]),
+    "firex_resolve_offline_dims": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
+                                                  ctypes.POINTER(ctypes.c_int32)]),
     "firex_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
     "firex_destroy": (None, [_P]),
     "firex_reset": (ctypes.c_int, [_P]),
@@ -252,6 +258,47 @@ def split_offline_delta(delta_seconds: float, maximum_step_seconds: float) -> li
     _check(lib.firex_split_offline_delta(float(delta_seconds), float(maximum_step_seconds), steps, count.value,
                                          ctypes.byref(count)))
     return [steps[i] for i in range(count.value)]
+
+
+class FixedStepClock:
+    """FixedStepClock, src/sim/FixedStepClock.ts:1-38 (state kept here, logic native)."""
+
+    def __init__(self, step_seconds: float, maximum_steps_per_frame: int):
+        if not (isinstance(step_seconds, (int, float)) and math.isfinite(step_seconds) and step_seconds > 0):
+            raise ValueError("Fixed step must be finite and positive.")
+        if not (isinstance(maximum_steps_per_frame, int) and maximum_steps_per_frame > 0):
+            raise ValueError("Maximum step count must be a positive integer.")
+        self.step_seconds = float(step_seconds)
+        self.maximum_steps_per_frame = maximum_steps_per_frame
+        self._acc = ctypes.c_double(0.0)
+
+    def advance(self, delta_seconds: float, step=None) -> int:
+        n = load_library().firex_fixed_step_clock_advance(self.step_seconds, self.maximum_steps_per_frame,
+                                                          ctypes.byref(self._acc), float(delta_seconds))
+        if n < 0:
+            raise ValueError(load_library().firex_last_error().decode())
+        if step is not None:
+            for _ in range(n):
+                step(self.step_seconds)
+        return n
+
+    def reset(self) -> None:
+        self._acc.value = 0.0
+
+
+def clamp_parameter(name: str, value: float, previous: float) -> float:
+    """The value setParameter stores for a numeric key (FireXEngine.ts:937-1032)."""
+    out = ctypes.c_double()
+    _check(load_library().firex_clamp_parameter(name.encode(), float(value), float(previous), ctypes.byref(out)))
+    return out.value
+
+
+def resolve_offline_dims(tier: str, override: Optional[Sequence[int]] = None) -> tuple:
+    """resolveFireXOfflineDimensions, FireXEngine.ts:279-290."""
+    out = (ctypes.c_int32 * 3)()
+    ov = (ctypes.c_int32 * 3)(*[int(v) for v in override]) if override is not None else None
+    _check(load_library().firex_resolve_offline_dims(tier.encode(), ov, out))
+    return tuple(out)
 
 
 # ------------------------------------------------------------------- solver

@@ -175,6 +175,18 @@ int firex_preflight(const int32_t dims[3], const firex_device_limits* limits, in
  * cap steps; *count receives the full step count. */
 int firex_split_offline_delta(double delta_seconds, double maximum_step_seconds, double* steps, int32_t cap,
                               int32_t* count);
+/* FixedStepClock.advance, src/sim/FixedStepClock.ts:16-33: adds the bounded
+ * delta to *accumulator and returns the number of steps to run; -1 (and
+ * FIREX_ERR_INVALID in firex_last_error) for an invalid clock. */
+int32_t firex_fixed_step_clock_advance(double step_seconds, int32_t maximum_steps, double* accumulator,
+                                       double delta_seconds);
+/* The clamp setParameter applies to one numeric key (FireXEngine.ts:937-1032):
+ * *out receives the stored value given the previous one; FIREX_ERR_INVALID
+ * for a key the solver does not know. */
+int firex_clamp_parameter(const char* name, double value, double previous, double* out);
+/* resolveFireXOfflineDimensions, FireXEngine.ts:279-290: per-axis maximum of
+ * the tier preset and the override (override_dims may be NULL). */
+int firex_resolve_offline_dims(const char* tier, const int32_t* override_dims, int32_t out[3]);
 
 /* ---- solver lifetime --------------------------------------------------- */
 /* Allocate and initialise a plume (createResourceSet + initializeFields,

@@ -927,4 +927,31 @@ float orc_f32_pow(float x, float y) { return fxm_pow(x, y); }
 float orc_f32_log2(float x) { return fxm_log2(x); }
 float orc_f32_exp2(float x) { return fxm_exp2(x); }
 
+// Vectorised form of the f32 built-ins: fn 0 exp, 1 exp2, 2 log2, 3 sin,
+// 4 cos, 5 pow(x, y), 6 smoothstep(y0, y1, x) with y = (y0, y1) pairs.
+void orc_f32_eval(int fn, const float* x, const float* y, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (fn) {
+      case 0: out[i] = fxm_exp(x[i]); break;
+      case 1: out[i] = fxm_exp2(x[i]); break;
+      case 2: out[i] = fxm_log2(x[i]); break;
+      case 3: out[i] = fxm_sin(x[i]); break;
+      case 4: out[i] = fxm_cos(x[i]); break;
+      case 5: out[i] = fxm_pow(x[i], y[i]); break;
+      default: out[i] = fxm_smoothstep(y[2 * i], y[2 * i + 1], x[i]); break;
+    }
+  }
+}
+
+// One Jacobi sweep of the FP64 stencil is checked against a dense solve;
+// this exports the grid's stencil weights (wx, wy, wz) for a test to build
+// the matrix from the same uniforms.
+void orc_stencil_weights(void* p, double dt, double* out3) {
+  Handle* h = static_cast<Handle*>(p);
+  h->d.write_uniforms(dt);
+  out3[0] = h->d.mx * h->d.mx;
+  out3[1] = h->d.my * h->d.my;
+  out3[2] = h->d.mz * h->d.mz;
+}
+
 }  // extern "C"

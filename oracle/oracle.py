@@ -70,6 +70,8 @@ def load() -> ctypes.CDLL:
         "orc_f32_pow": (f, [f, f]),
         "orc_f32_log2": (f, [f]),
         "orc_f32_exp2": (f, [f]),
+        "orc_f32_eval": (None, [i, fp, fp, fp, ctypes.c_int64]),
+        "orc_stencil_weights": (None, [vp, d, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -77,6 +79,20 @@ def load() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+F32_FUNCS = {"exp": 0, "exp2": 1, "log2": 2, "sin": 3, "cos": 4, "pow": 5, "smoothstep": 6}
+
+
+def f32_eval(name: str, x, y=None) -> np.ndarray:
+    """Evaluate an f32 WGSL built-in of the oracle's FP32 layer over an array."""
+    lib = load()
+    xa = np.ascontiguousarray(x, dtype=np.float32)
+    ya = np.ascontiguousarray(y if y is not None else np.zeros(1), dtype=np.float32)
+    out = np.empty_like(xa)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.orc_f32_eval(F32_FUNCS[name], xa.ctypes.data_as(fp), ya.ctypes.data_as(fp), out.ctypes.data_as(fp), xa.size)
+    return out
 
 
 class Oracle:
@@ -189,6 +205,11 @@ class Oracle:
         out = (ctypes.c_double * 4)()
         self.lib.orc_trilinear_flow(self.h, float(dt), *[float(v) for v in position], out)
         return np.array(out[:])
+
+    def stencil_weights(self, dt=0.0):
+        out = (ctypes.c_double * 3)()
+        self.lib.orc_stencil_weights(self.h, float(dt), out)
+        return tuple(out[:])
 
     def emitter(self, dt, normalized_position):
         return self.lib.orc_emitter(self.h, float(dt), *[float(v) for v in normalized_position])
