@@ -59,7 +59,34 @@ def build_library(verbose: bool = False) -> str:
     os.replace(tmp, LIBRARY)
     if verbose:
         print(f"built {LIBRARY}")
+    build_napi_addon(verbose)
     return LIBRARY
+
+
+NAPI_SOURCE = os.path.join(HERE, "napi", "firex_napi.c")
+NAPI_ADDON = os.path.join(HERE, "firex_napi.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+
+
+def build_napi_addon(verbose: bool = False) -> str:
+    """Node-API addon over the C ABI (plain C, no node-gyp): firex_napi.node.
+
+    Skipped (returns "") when the Node-API headers are not installed."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print("node_api.h not found: N-API addon not built")
+        return ""
+    tmp = NAPI_ADDON + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c11", "-shared", "-fPIC", "-Wall", f"-I{NODE_INCLUDE}",
+           "-DNODE_GYP_MODULE_NAME=firex_napi", NAPI_SOURCE, "-L" + HERE, "-lfirex_hip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"N-API addon build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, NAPI_ADDON)
+    if verbose:
+        print(f"built {NAPI_ADDON}")
+    return NAPI_ADDON
 
 
 if __name__ == "__main__":

@@ -1,0 +1,70 @@
+// Type declarations for firex_hip_solver.js (the TypeScript view of the
+// MI355X Fire-X solver, drop-in for FireXEngine's compute path,
+// src/sim/firex/FireXEngine.ts:370).
+
+export type FireXFieldName =
+  | 'flow' | 'species' | 'flowPredicted' | 'pressure' | 'correction' | 'divergence' | 'reaction' | 'liquid';
+
+export type FireXReductionName =
+  | 'sumFuel' | 'sumOxygen' | 'sumProducts' | 'sumVapor' | 'sumTemperature' | 'sumReaction'
+  | 'residualL2' | 'flowDivergenceL2';
+
+export interface FireXHipSolverOptions {
+  tier?: 'low' | 'balanced' | 'high' | 'maximum' | 'cinematic' | 'hd' | 'qhd' | 'uhd';
+  dims?: readonly [number, number, number];
+  pressureIterations?: number;
+  correctionIterations?: number;
+  timeStep?: number;
+  maximumSubsteps?: number;
+  speciesStorage?: 'f32' | 'f16' | 'bf16';
+  heavyMode?: 'auto' | 'off' | 'f16' | 'f32';
+  plan?: 'reference' | 'fused';
+  cinematic?: boolean;
+  device?: number;
+  memoryBudgetBytes?: number;
+  jacobiSweepsPerLaunch?: number;
+  scene?: Readonly<Record<string, number | boolean>>;
+}
+
+export interface FireXHipGridInfo {
+  dimensions: [number, number, number];
+  domainMeters: [number, number, number];
+  cellSizeMeters: [number, number, number];
+  cellCount: number;
+  estimatedBytes: number;
+  deviceBytes: number;
+  simulationTime: number;
+  frameIndex: number;
+  custom: boolean;
+}
+
+export declare class FireXHipSolver {
+  readonly backend: string;
+  constructor(options?: FireXHipSolverOptions);
+  initialize(): Promise<void>;
+  setParameter(name: string, value: number | boolean): void;
+  getParameter(name: string): number;
+  setEmitters(
+    emitters: ReadonlyArray<readonly [number, number, number, number, number]>,
+    jitter?: number, velocityNoise?: number, seed?: number,
+  ): void;
+  reset(): void;
+  frame(timeSeconds: number, deltaSeconds: number): number;
+  step(dt: number, count?: number): void;
+  renderOfflineFrame(deltaSeconds: number, present?: false, signal?: AbortSignal): Promise<void>;
+  sync(): void;
+  readField<T extends Float32Array | Float64Array | Uint32Array = Float32Array>(name: FireXFieldName, out?: T): T;
+  writeField(name: FireXFieldName, data: Float32Array | Float64Array | Uint32Array): void;
+  reduce(name: FireXReductionName): number;
+  getGridInfo(): FireXHipGridInfo;
+  dispose(): void;
+}
+
+export declare function estimateFireXGridMemory(
+  dimensions: readonly [number, number, number], particleCount?: number,
+): {
+  cellCount: number; gridBytes: number; particleBytes: number; fixedBytes: number; totalBytes: number;
+  largestStorageBufferBytes: number;
+};
+
+export declare function splitOfflineSimulationDelta(deltaSeconds: number, maximumStepSeconds: number): number[];
