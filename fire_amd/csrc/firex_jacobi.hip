@@ -43,8 +43,8 @@ template <> struct PIO<0> {
 template <> struct PIO<1> {
   using Word = _Float16;
   __device__ static inline float load(const Word* p, uint32_t i) { return (float)p[i]; }
-  __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = (_Float16)v; }
-  __device__ static inline float round(float v) { return (float)(_Float16)v; }
+  __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = (_Float16)fx_keep_f32(v); }
+  __device__ static inline float round(float v) { return (float)(_Float16)fx_keep_f32(v); }
 };
 
 __device__ inline uint32_t xcd_remap(uint32_t hw, uint32_t total) {

@@ -37,7 +37,10 @@ template <> struct SpeciesIO<FX_STORAGE_F16> {
   }
   __device__ static inline void store(Word* p, uint32_t i, float4 v) {
     half4_t h;
-    h.x = (_Float16)v.x; h.y = (_Float16)v.y; h.z = (_Float16)v.z; h.w = (_Float16)v.w;
+    h.x = (_Float16)fx_keep_f32(v.x);
+    h.y = (_Float16)fx_keep_f32(v.y);
+    h.z = (_Float16)fx_keep_f32(v.z);
+    h.w = (_Float16)fx_keep_f32(v.w);
     p[i] = h;
   }
 };
@@ -66,7 +69,7 @@ template <> struct PressureIO<0> {
 template <> struct PressureIO<1> {
   using Word = _Float16;
   __device__ static inline float load(const Word* p, uint32_t i) { return (float)p[i]; }
-  __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = (_Float16)v; }
+  __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = (_Float16)fx_keep_f32(v); }
 };
 
 // ------------------------------------------------------------ cell mapping

@@ -42,6 +42,7 @@ struct firex_solver {
   double* scratch = nullptr;
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
   uint64_t bytes = 0;
   JacobiPlan jacobi;
   int corr_index = 0;      // buffer holding the last residual solution
@@ -54,6 +55,14 @@ struct firex_solver {
   std::vector<hipEvent_t> event_pool;
   std::vector<TimedLaunch> pending;
   firex_pass_timing totals;
+  // Captured rest-of-step graphs, one per buffer-index state.
+  struct RestGraph {
+    int flow_a, pressure_index;
+    int post_flow_a, post_pressure_index, post_corr_index;
+    hipGraphExec_t exec;
+  };
+  std::vector<RestGraph> graphs;
+  uint64_t graph_key = 0;
 };
 
 namespace {
@@ -196,20 +205,25 @@ void fill(firex_solver* s, void* p, uint64_t words) {
   timed(s, FIREX_PASS_FILL_COPY, [&] { launch_fill_u32(p, words, 0, s->stream); });
 }
 
-// One solver step with the given uniform block (encodeSimulationStep,
-// FireXEngine.ts:2158-2275).
-int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
-  const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
+// simulate, the first pass of encodeSimulationStep (FireXEngine.ts:2172-2173).
+void encode_simulate(firex_solver* s, const FxStep& st) {
+  SimulateArgs a{flow0(s), s->species[s->species_index], flow1(s), s->species[1 - s->species_index], s->reaction,
+                 s->liquid, st};
+  timed(s, FIREX_PASS_SIMULATE, [&] { launch_simulate(s->cfg.species_storage, a, s->stream); });
+  s->species_index = 1 - s->species_index;
+}
+
+// The rest of the step (FireXEngine.ts:2174-2275): divergence, both Jacobi
+// solves, both projections, clears and copyFlow.  Its launches depend only
+// on the grid metric and on the buffer-index state (flow_a, pressure_index),
+// never on the time, so it can be captured once per state and replayed.
+void encode_rest(firex_solver* s, const FxStep& st) {
   hipStream_t q = s->stream;
   const bool fused = s->cfg.plan == FIREX_PLAN_FUSED;
   const int N = s->cfg.pressure_iterations, M = s->cfg.correction_iterations;
   const uint64_t pwords = ((uint64_t)s->cells * (s->f16p ? 2 : 4) + 3) / 4;
   const int f16 = s->f16p ? 1 : 0;
 
-  SimulateArgs a{flow0(s), s->species[s->species_index], flow1(s), s->species[1 - s->species_index], s->reaction,
-                 s->liquid, st};
-  timed(s, FIREX_PASS_SIMULATE, [&] { launch_simulate(s->cfg.species_storage, a, q); });
-  s->species_index = 1 - s->species_index;
   timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow1(s), s->divergence, st, q); });
 
   if (s->heavy && s->compact) {
@@ -252,6 +266,79 @@ int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
     timed(s, FIREX_PASS_PROJECT, [&] { launch_project(f16, flow0(s), s->corr[s->corr_index], flow1(s), st, q); });
     timed(s, FIREX_PASS_FILL_COPY, [&] { launch_copy_u128(flow0(s), flow1(s), s->cells, q); });
   }
+}
+
+// Key of the values the captured launches bake in (metric and stencil).
+uint64_t graph_key(const FxStep& st) {
+  uint64_t h = 0x9e3779b97f4a7c15ull;
+  const float v[] = {st.mx, st.my, st.mz, st.wx, st.wy, st.wz, st.jacobi_inv};
+  for (float f : v) h = (h ^ fxm_float_to_bits(f)) * 0x100000001b3ull;
+  return h;
+}
+
+void drop_graphs(firex_solver* s) {
+  for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
+// Replays (capturing on first use) the rest of the step as one graph launch.
+// Returns false when capture is unavailable; the caller then runs it eagerly.
+bool launch_rest_graph(firex_solver* s, const FxStep& st) {
+  const uint64_t key = graph_key(st);
+  if (key != s->graph_key) {
+    drop_graphs(s);
+    s->graph_key = key;
+  }
+  for (const auto& g : s->graphs) {
+    if (g.flow_a == s->flow_a && g.pressure_index == s->pressure_index) {
+      if (hipGraphLaunch(g.exec, s->stream) != hipSuccess) return false;
+      s->flow_a = g.post_flow_a;
+      s->pressure_index = g.post_pressure_index;
+      s->corr_index = g.post_corr_index;
+      return true;
+    }
+  }
+  firex_solver::RestGraph g;
+  g.flow_a = s->flow_a;
+  g.pressure_index = s->pressure_index;
+  const int corr_before = s->corr_index;
+  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  encode_rest(s, st);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    s->flow_a = g.flow_a;  // nothing ran: restore the state for the eager path
+    s->pressure_index = g.pressure_index;
+    s->corr_index = corr_before;
+    return false;
+  }
+  g.post_flow_a = s->flow_a;
+  g.post_pressure_index = s->pressure_index;
+  g.post_corr_index = s->corr_index;
+  s->graphs.push_back(g);
+  return hipGraphLaunch(g.exec, s->stream) == hipSuccess;
+}
+
+// One solver step with the given uniform block (encodeSimulationStep,
+// FireXEngine.ts:2158-2275).
+int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
+  const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
+  encode_simulate(s, st);
+  if (s->cfg.capture_graphs && !s->timing) {
+    if (!launch_rest_graph(s, st)) {
+      s->cfg.capture_graphs = 0;  // capture unavailable: stay eager from now on
+      drop_graphs(s);
+      encode_rest(s, st);
+    }
+  } else {
+    encode_rest(s, st);
+  }
   if (s->timing) {
     s->totals.steps += 1;
     if (s->pending.size() > 8192) {
@@ -280,9 +367,7 @@ int correction_buffer(firex_solver* s, void** p) {
 
 }  // namespace
 
-extern "C" {
-
-int firex_device_count(void) {
+extern "C" int firex_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) {
     (void)hipGetLastError();
@@ -291,10 +376,11 @@ int firex_device_count(void) {
   return n;
 }
 
-int firex_create(const firex_config* cfg, firex_solver** out) {
-  clear_error();
-  if (!cfg || !out) return set_error(FIREX_ERR_INVALID, "config and out must be non-null");
-  *out = nullptr;
+namespace fx {
+
+// Validates a configuration; fills the path flags and the device bytes.
+int solver_plan(const firex_config* cfg, bool* heavy_out, bool* f16p_out, bool* compact_out, uint64_t* bytes_out) {
+  if (!cfg) return set_error(FIREX_ERR_INVALID, "config is null");
   for (int i = 0; i < 3; ++i) {
     if (cfg->dims[i] < 2 || cfg->dims[i] > 1024)
       return set_error(FIREX_ERR_UNSUPPORTED, "each grid axis must be in [2, 1024]");
@@ -317,6 +403,24 @@ int firex_create(const firex_config* cfg, firex_solver** out) {
     return set_error(FIREX_ERR_BUDGET, "Transactional rebuild peak requires " + std::to_string(need) +
                                            " bytes, above the selected " + std::to_string(cfg->memory_budget_bytes) +
                                            " byte guard.");
+  if (heavy_out) *heavy_out = heavy;
+  if (f16p_out) *f16p_out = f16p;
+  if (compact_out) *compact_out = compact;
+  if (bytes_out) *bytes_out = need;
+  return FIREX_OK;
+}
+
+// Creates a plume.  With a non-null stream the solver issues its work there
+// and neither creates nor destroys a stream of its own, and the initial
+// field reset is left in flight on that stream.
+int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** out) {
+  clear_error();
+  if (!cfg || !out) return set_error(FIREX_ERR_INVALID, "config and out must be non-null");
+  *out = nullptr;
+  bool heavy = false, f16p = false, compact = false;
+  uint64_t need = 0;
+  int prc = solver_plan(cfg, &heavy, &f16p, &compact, &need);
+  if (prc) return prc;
 
   int ndev = firex_device_count();
   if (ndev <= 0) return set_error(FIREX_ERR_DEVICE, "No HIP device is available for the Fire-X solver.");
@@ -338,10 +442,15 @@ int firex_create(const firex_config* cfg, firex_solver** out) {
     delete s;
     return hip_fail(nullptr, e, "hipSetDevice");
   }
-  e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    delete s;
-    return hip_fail(nullptr, e, "hipStreamCreate");
+  s->own_stream = stream == nullptr;
+  if (stream) {
+    s->stream = stream;
+  } else {
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete s;
+      return hip_fail(nullptr, e, "hipStreamCreate");
+    }
   }
   const uint64_t c = s->cells;
   // Rounded up to whole 32-bit words: the fill kernel clears words
@@ -359,22 +468,41 @@ int firex_create(const firex_config* cfg, firex_solver** out) {
   if (!rc) rc = alloc(s, (void**)&s->scratch, kReduceScratchBytes, "reduction scratch");
   if (!rc) {
     s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch);
+    if (cfg->jacobi_chunk_z > 0) s->jacobi.chunk_z = cfg->jacobi_chunk_z;
     rc = initialize_fields(s);
   }
-  if (!rc) {
+  if (!rc && s->own_stream) {
     e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) rc = hip_fail(s, e, "field initialization");
   }
   if (rc) {
     std::string msg = firex_last_error();
+    if (!s->own_stream) hipStreamSynchronize(s->stream);
     free_all(s);
-    hipStreamDestroy(s->stream);
+    if (s->own_stream) hipStreamDestroy(s->stream);
     delete s;
     return set_error(rc, msg);
   }
   *out = s;
   return FIREX_OK;
 }
+
+// Enqueues one step of dt seconds without waiting.
+int solver_step_async(firex_solver* s, double dt) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  return step_with_dt(s, dt);
+}
+
+hipStream_t solver_stream(firex_solver* s) { return s->stream; }
+uint64_t solver_bytes(firex_solver* s) { return s->bytes; }
+uint32_t solver_cells(firex_solver* s) { return s->cells; }
+
+}  // namespace fx
+
+extern "C" {
+
+int firex_create(const firex_config* cfg, firex_solver** out) { return fx::solver_create(cfg, nullptr, out); }
 
 void firex_destroy(firex_solver* s) {
   if (!s) return;
@@ -385,8 +513,9 @@ void firex_destroy(firex_solver* s) {
     hipEventDestroy(t.stop);
   }
   for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
+  drop_graphs(s);
   free_all(s);
-  hipStreamDestroy(s->stream);
+  if (s->own_stream) hipStreamDestroy(s->stream);
   delete s;
 }
 

@@ -69,6 +69,18 @@ struct FxStep {
   float wx, wy, wz, jacobi_inv;        // metric^2, 1 / (2 (wx + wy + wz))
 };
 
+#if defined(__HIP__)
+// Keeps an f32 value as computed.  Without it the compiler may fold
+// "f32 multiply, then convert to f16" into one mixed-precision FMA that
+// rounds once, straight to f16, where the reference rounds to f32 first and
+// then to f16 (pressureShaders.ts:115-116).  The empty asm only pins the
+// register; it emits no instruction.
+__device__ static inline float fx_keep_f32(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+#endif
+
 // Counter-based seeded noise: splitmix64 finaliser on (seed, frame, cell, stream).
 #if defined(__HIP__)
 __host__ __device__

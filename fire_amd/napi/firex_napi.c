@@ -157,6 +157,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.device = (int32_t)num_or(env, argv[0], "device", cfg.device);
     cfg.memory_budget_bytes = (uint64_t)num_or(env, argv[0], "memoryBudgetBytes", 0);
     cfg.jacobi_sweeps_per_launch = (int32_t)num_or(env, argv[0], "jacobiSweepsPerLaunch", 0);
+    cfg.capture_graphs = (int32_t)num_or(env, argv[0], "captureGraphs", 0);
     if (str_prop(env, argv[0], "speciesStorage", buf, sizeof buf)) cfg.species_storage = lookup(buf, storage_names, 3);
     if (str_prop(env, argv[0], "heavyMode", buf, sizeof buf)) cfg.heavy_mode = lookup(buf, heavy_names, 4);
     if (str_prop(env, argv[0], "plan", buf, sizeof buf)) cfg.plan = lookup(buf, plan_names, 2);

@@ -81,7 +81,9 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("memory_budget_bytes", ctypes.c_uint64),
         ("jacobi_sweeps_per_launch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("capture_graphs", ctypes.c_int32),
+        ("jacobi_chunk_z", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -310,7 +312,7 @@ class FireXHipSolver:
                  time_step: Optional[float] = None, maximum_substeps: Optional[int] = None,
                  species_storage: str = "f32", heavy_mode: str = "auto", plan: str = "reference",
                  cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
-                 jacobi_sweeps_per_launch: int = 0):
+                 jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -332,6 +334,8 @@ class FireXHipSolver:
         cfg.device = int(device)
         cfg.memory_budget_bytes = int(memory_budget_bytes)
         cfg.jacobi_sweeps_per_launch = int(jacobi_sweeps_per_launch)
+        cfg.capture_graphs = int(bool(capture_graphs))
+        cfg.jacobi_chunk_z = int(jacobi_chunk_z)
         self.config = cfg
         self.dims = tuple(int(v) for v in cfg.dims)
         self.cells = self.dims[0] * self.dims[1] * self.dims[2]

@@ -87,7 +87,10 @@ typedef struct firex_config {
   int32_t device;                /* HIP device ordinal                               */
   uint64_t memory_budget_bytes;  /* guard on current + replacement bytes; 0 = none   */
   int32_t jacobi_sweeps_per_launch; /* FUSED plan temporal depth; 0 = tuned default  */
-  int32_t reserved[7];
+  int32_t capture_graphs;        /* 1: everything after simulate runs as a captured
+                                    hipGraph (one launch per step for that part)     */
+  int32_t jacobi_chunk_z;        /* FUSED plan: z planes per block; 0 = tuned default */
+  int32_t reserved[5];
 } firex_config;
 
 /* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */

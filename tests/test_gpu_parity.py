@@ -192,3 +192,22 @@ def test_seeded_state_write_and_step_bitwise():
     g.step(DT)
     assert_fields_equal(g, o, "random state, one step")
     g.close()
+
+
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_captured_graph_replay_bitwise(plan):
+    """The post-simulate part of the step replayed from captured hipGraphs
+    (one graph per buffer-index state) gives the eager results, step after
+    step, with eager host work and synchronisation between the replays."""
+    dims = (24, 40, 20)
+    o = orc.Oracle(dims, precision=32, pressure_iterations=7, correction_iterations=5)
+    o.set_bool("cinematic", True)
+    o.set_heavy_mode(orc.HEAVY_OFF)
+    g = FireXHipSolver(dims, tier="hd", pressure_iterations=7, correction_iterations=5, plan=plan,
+                       heavy_mode="off", capture_graphs=True, jacobi_sweeps_per_launch=3)
+    for step in range(5):
+        o.step(DT)
+        g.step(DT)
+        g.sync()
+        assert_fields_equal(g, o, f"graph replay {plan} step {step + 1}")
+    g.close()
