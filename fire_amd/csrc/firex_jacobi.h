@@ -9,14 +9,20 @@
 namespace fx {
 
 // Launch geometry of the temporally blocked sweep kernel.
+enum { JACOBI_KERNEL_AUTO = 0, JACOBI_KERNEL_SHIFT = 1, JACOBI_KERNEL_RING = 2 };
+
 struct JacobiPlan {
   int32_t sweeps_per_launch = 4;  // K: sweeps advanced per launch
   int32_t rows = 4;               // R: y rows per thread
   int32_t waves = 8;              // W: waves per block (stacked in y)
   int32_t chunk_z = 16;           // output planes per block
+  int32_t kernel = JACOBI_KERNEL_SHIFT;  // SHIFT: any grid; RING: nx >= 64, ny >= W * R
 };
 
-JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch);
+// kernel: JACOBI_KERNEL_* (AUTO picks RING where the grid allows it);
+// sweeps_per_launch and chunk_z: 0 = default.
+JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z = 0,
+                            int32_t kernel = JACOBI_KERNEL_AUTO);
 
 // One launch of the temporally blocked kernel: `sweeps` sweeps from
 // buffers[src] (or from the zero field, without reading it) into

@@ -33,18 +33,36 @@ namespace fx {
 
 namespace {
 
+// Word 3 of a gfx9 buffer resource for raw 32-bit accesses.
+constexpr int kBufferWord3 = 0x00020000;
+
 template <int P> struct PIO;
 template <> struct PIO<0> {
   using Word = float;
+  static constexpr uint32_t kBytes = 4;
   __device__ static inline float load(const Word* p, uint32_t i) { return p[i]; }
   __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = v; }
   __device__ static inline float round(float v) { return v; }
+  __device__ static inline float bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  }
+  __device__ static inline void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+  }
 };
 template <> struct PIO<1> {
   using Word = _Float16;
+  static constexpr uint32_t kBytes = 2;
   __device__ static inline float load(const Word* p, uint32_t i) { return (float)p[i]; }
   __device__ static inline void store(Word* p, uint32_t i, float v) { p[i] = (_Float16)fx_keep_f32(v); }
   __device__ static inline float round(float v) { return (float)(_Float16)fx_keep_f32(v); }
+  __device__ static inline float bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
+  }
+  __device__ static inline void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
+    const _Float16 h = (_Float16)fx_keep_f32(v);
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, (int)voff, (int)soff, 0);
+  }
 };
 
 __device__ inline uint32_t xcd_remap(uint32_t hw, uint32_t total) {
@@ -183,6 +201,205 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_tb(const typename PIO<P>::Wor
   }
 }
 
+// ------------------------------------------------------------- ring kernel
+// The same K-level pipeline with three changes that cut the instructions
+// per cell update:
+//   * every level runs every iteration and the z loop is unrolled by U (a
+//     multiple of 3 and >= K + 1), so each level's three planes and the div
+//     planes live in register rings addressed at compile time (no copies);
+//     a level's values outside its valid z range are never consumed;
+//   * x neighbours come from DPP wave shifts, whose out-of-range lane keeps
+//     its own value;
+//   * tiles are aligned to the domain walls (the first tile starts at 0, the
+//     last one ends at n), so the wall column / floor row sees itself as its
+//     missing neighbour, which is the reference's clamped (Neumann) read,
+//     without a select.  Tiles may overlap; overlapping outputs are computed
+//     from identical inputs and are equal.
+// Needs nx >= 64 and ny >= W * R.
+
+// lane i <- lane i - 1 (lane 0 keeps v)
+__device__ inline float lane_from_left(float v) {
+  int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));
+}
+// lane i <- lane i + 1 (lane 63 keeps v)
+__device__ inline float lane_from_right(float v) {
+  int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));
+}
+
+__host__ __device__ inline int wall_tiles(int n, int T, int K) {
+  return n <= T ? 1 : 1 + (n - T + (T - 2 * K) - 1) / (T - 2 * K);
+}
+
+// Origin and output range [lo, hi) of tile i of `count` along an axis of n
+// cells, tile width T, halo K.
+__device__ inline void wall_tile(int i, int count, int n, int T, int K, int& origin, int& lo, int& hi) {
+  if (i == count - 1) {
+    origin = n - T;
+    hi = n;
+  } else {
+    origin = i * (T - 2 * K);
+    hi = origin + T - K;
+  }
+  lo = i == 0 ? 0 : origin + K;
+}
+
+template <int P, int K, int R, int W>
+__global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::Word* __restrict__ pin,
+                                                        typename PIO<P>::Word* __restrict__ pout,
+                                                        const float* __restrict__ div, FxStep st, int chunk_z,
+                                                        int tiles_x, int tiles_y, int zero_start) {
+  constexpr int H = W * R;
+  constexpr int U = 3 * ((K + 3) / 3);  // >= K + 1, multiple of 3
+  __shared__ float edge[2][K][W][2][64];  // [parity][level][wave][row 0 | row R-1][lane]
+
+  const int nx = st.nx, ny = st.ny, nz = st.nz;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const uint32_t chunks = (uint32_t)(nz + chunk_z - 1) / (uint32_t)chunk_z;
+  const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)tiles_x * (uint32_t)tiles_y * chunks);
+  const int tile_x = (int)(b % (uint32_t)tiles_x);
+  const uint32_t rest = b / (uint32_t)tiles_x;
+  const int tile_y = (int)(rest % (uint32_t)tiles_y);
+  const int chunk = (int)(rest / (uint32_t)tiles_y);
+  int ox, xlo, xhi, oy, ylo, yhi;
+  wall_tile(tile_x, tiles_x, nx, 64, K, ox, xlo, xhi);
+  wall_tile(tile_y, tiles_y, ny, H, K, oy, ylo, yhi);
+  const int gx = ox + lane;
+  const int gy0 = oy + wave * R;
+  const int z0 = chunk * chunk_z;
+  const int z1 = min(z0 + chunk_z, nz);
+  const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+  const uint32_t cells = plane * (uint32_t)nz;
+  // Buffer resources: per-row byte offsets in VGPRs, the plane offset in an
+  // SGPR; accesses past the field read 0 / are dropped by the hardware.
+  const __amdgpu_buffer_rsrc_t rin =
+      __builtin_amdgcn_make_buffer_rsrc((void*)pin, (short)0, (int)(cells * PIO<P>::kBytes), kBufferWord3);
+  const __amdgpu_buffer_rsrc_t rout =
+      __builtin_amdgcn_make_buffer_rsrc((void*)pout, (short)0, (int)(cells * PIO<P>::kBytes), kBufferWord3);
+  const __amdgpu_buffer_rsrc_t rdiv =
+      __builtin_amdgcn_make_buffer_rsrc((void*)div, (short)0, (int)(cells * 4u), kBufferWord3);
+  uint32_t rowoff[R];
+  bool top[R], out_row[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    rowoff[r] = (uint32_t)gx + (uint32_t)nx * (uint32_t)(gy0 + r);
+    top[r] = gy0 + r == ny - 1;
+    out_row[r] = gy0 + r >= ylo && gy0 + r < yhi && gx >= xlo && gx < xhi;
+  }
+  const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.jacobi_inv;
+
+  float Pr[K][3][R], Dv[U][R];
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) Pr[s][q][r] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < U; ++q)
+#pragma unroll
+    for (int r = 0; r < R; ++r) Dv[q][r] = 0.0f;
+
+  const int t0 = z0 - K;
+  const int n_it = (z1 - z0) + 2 * K;
+  for (int tb = 0; tb < n_it; tb += U) {
+#pragma unroll
+    for (int ph = 0; ph < U; ++ph) {
+      const int t = t0 + tb + ph;
+      const int par = (tb + ph) & 1;
+      {
+        // The clamped plane index makes the virtual planes -1 and nz copies
+        // of the wall planes: the reference's clamped z reads at level 0.
+        const uint32_t pbase = (uint32_t)min(max(t, 0), nz - 1) * plane;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          Pr[0][ph % 3][r] = zero_start ? 0.0f : PIO<P>::bload(rin, rowoff[r] * PIO<P>::kBytes, pbase * PIO<P>::kBytes);
+          Dv[ph][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)(rowoff[r] * 4u),
+                                                                                    (int)(pbase * 4u), 0));
+        }
+      }
+#pragma unroll
+      for (int s = 1; s <= K; ++s) {
+        const int z = t - s;
+        const int sc = ((ph - s) % 3 + 3) % 3;      // slot of plane z
+        const int sa = ((ph - s - 1) % 3 + 3) % 3;  // plane z - 1
+        const int sb = ((ph - s + 1) % 3 + 3) % 3;  // plane z + 1
+        const int sd = ((ph - s) % U + U) % U;      // div of plane z
+        float res[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float ctr = Pr[s - 1][sc][r];
+          const float left = lane_from_left(ctr);
+          const float right = lane_from_right(ctr);
+          float down, up;
+          if (r > 0) down = Pr[s - 1][sc][r - 1];
+          else down = wave > 0 ? edge[par][s - 1][wave - 1][1][lane] : ctr;
+          if (r < R - 1) up = Pr[s - 1][sc][r + 1];
+          else up = wave < W - 1 ? edge[par][s - 1][wave + 1][0][lane] : ctr;
+          const float nb = wx * (left + right) + wy * (down + up) + wz * (Pr[s - 1][sa][r] + Pr[s - 1][sb][r]);
+          float v = (nb - Dv[sd][r]) * inv;
+          if (top[r]) v = 0.0f;
+          res[r] = PIO<P>::round(v);
+        }
+        if (s < K) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) Pr[s][sc][r] = res[r];
+          // Keep the virtual planes beside the z walls equal to the wall
+          // plane for the level above (its clamped z neighbour).
+          if (z == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) Pr[s][sa][r] = res[r];
+          } else if (z == nz) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) Pr[s][sc][r] = Pr[s][sa][r];
+          }
+        } else if (z >= z0 && z < z1) {
+          const uint32_t pbase = (uint32_t)z * plane * PIO<P>::kBytes;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (out_row[r]) PIO<P>::bstore(rout, rowoff[r] * PIO<P>::kBytes, pbase, res[r]);
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        const int sn = ((ph - s) % 3 + 3) % 3;  // newest plane of level s
+        edge[par ^ 1][s][wave][0][lane] = Pr[s][sn][0];
+        edge[par ^ 1][s][wave][1][lane] = Pr[s][sn][R - 1];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int P, int K, int R, int W>
+void launch_ring(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
+                 int zero_start, hipStream_t s) {
+  const int tiles_x = wall_tiles(st.nx, 64, K);
+  const int tiles_y = wall_tiles(st.ny, W * R, K);
+  const int chunks = (st.nz + plan.chunk_z - 1) / plan.chunk_z;
+  const uint32_t blocks = (uint32_t)tiles_x * (uint32_t)tiles_y * (uint32_t)chunks;
+  using Word = typename PIO<P>::Word;
+  k_jacobi_ring<P, K, R, W><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, st, plan.chunk_z, tiles_x,
+                                                       tiles_y, zero_start);
+}
+
+template <int P>
+void launch_ring_k(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
+                   int zero_start, hipStream_t s) {
+  switch (k) {
+    case 1: launch_ring<P, 1, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    case 2: launch_ring<P, 2, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    case 3: launch_ring<P, 3, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    case 4: launch_ring<P, 4, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    case 5: launch_ring<P, 5, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    case 6: launch_ring<P, 6, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    case 7: launch_ring<P, 7, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+    default: launch_ring<P, 8, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+  }
+}
+
 template <int P, int K>
 void launch_tb(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st, int zero_start,
                hipStream_t s) {
@@ -280,17 +497,35 @@ __global__ __launch_bounds__(256) void k_project_divergence(const float4* __rest
 
 bool jacobi_plan_valid(const JacobiPlan& plan) {
   return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= 8 && plan.rows == 4 && plan.waves == 8 &&
-         plan.chunk_z >= 1;
+         plan.chunk_z >= 1 && (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);
 }
 
-JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch) {
+JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel) {
   JacobiPlan p;
-  p.sweeps_per_launch = sweeps_per_launch > 0 ? (sweeps_per_launch > 8 ? 8 : sweeps_per_launch) : 4;
-  // Enough z chunks to give every CU a couple of blocks.
+  const bool ring_ok = dims[0] >= 64 && dims[1] >= p.waves * p.rows;
+  p.kernel = (kernel == JACOBI_KERNEL_RING && ring_ok) ? JACOBI_KERNEL_RING
+           : kernel == JACOBI_KERNEL_SHIFT ? JACOBI_KERNEL_SHIFT
+           : ring_ok ? JACOBI_KERNEL_RING : JACOBI_KERNEL_SHIFT;
+  const int dflt = p.kernel == JACOBI_KERNEL_RING ? 2 : 4;
+  p.sweeps_per_launch = sweeps_per_launch > 0 ? (sweeps_per_launch > 8 ? 8 : sweeps_per_launch) : dflt;
   const int K = p.sweeps_per_launch;
-  const int tiles = ((dims[0] + 63 - 2 * K) / (64 - 2 * K)) * ((dims[1] + 31 - 2 * K) / (32 - 2 * K));
+  if (chunk_z > 0) {
+    p.chunk_z = chunk_z;
+    return p;
+  }
+  // Enough z chunks to give every CU a couple of blocks.
+  const int tiles = p.kernel == JACOBI_KERNEL_RING
+      ? wall_tiles(dims[0], 64, K) * wall_tiles(dims[1], p.waves * p.rows, K)
+      : ((dims[0] + 63 - 2 * K) / (64 - 2 * K)) * ((dims[1] + 31 - 2 * K) / (32 - 2 * K));
   int chunk = dims[2];
   while (chunk > 8 && (long)tiles * ((dims[2] + chunk - 1) / chunk) < 512) chunk = (chunk + 1) / 2;
+  if (p.kernel == JACOBI_KERNEL_RING) {
+    // the ring kernel runs whole groups of U planes: prefer chunk + 2K % U == 0
+    const int U = 3 * ((K + 3) / 3);
+    int c = chunk;
+    while (c > 1 && (c + 2 * K) % U != 0) --c;
+    chunk = c;
+  }
   p.chunk_z = chunk;
   return p;
 }
@@ -316,10 +551,17 @@ void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, vo
                       const float* div, const FxStep& st, hipStream_t s) {
   const void* pin = buffers[l.src];
   void* pout = buffers[1 - l.src];
-  if (f16)
-    launch_tb_k<1>(l.sweeps, plan, pin, pout, div, st, l.zero_start ? 1 : 0, s);
-  else
-    launch_tb_k<0>(l.sweeps, plan, pin, pout, div, st, l.zero_start ? 1 : 0, s);
+  const int zs = l.zero_start ? 1 : 0;
+  if (plan.kernel == JACOBI_KERNEL_RING) {
+    if (f16)
+      launch_ring_k<1>(l.sweeps, plan, pin, pout, div, st, zs, s);
+    else
+      launch_ring_k<0>(l.sweeps, plan, pin, pout, div, st, zs, s);
+  } else if (f16) {
+    launch_tb_k<1>(l.sweeps, plan, pin, pout, div, st, zs, s);
+  } else {
+    launch_tb_k<0>(l.sweeps, plan, pin, pout, div, st, zs, s);
+  }
 }
 
 void launch_project_divergence(int f16, const float4* fin, const void* p, float4* fout, float* div, const FxStep& st,

@@ -467,8 +467,7 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
   if (!rc) rc = alloc(s, (void**)&s->reaction, c * 4, "reaction");
   if (!rc) rc = alloc(s, (void**)&s->scratch, kReduceScratchBytes, "reduction scratch");
   if (!rc) {
-    s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch);
-    if (cfg->jacobi_chunk_z > 0) s->jacobi.chunk_z = cfg->jacobi_chunk_z;
+    s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z, cfg->jacobi_kernel);
     rc = initialize_fields(s);
   }
   if (!rc && s->own_stream) {

@@ -83,7 +83,8 @@ class Config(ctypes.Structure):
         ("jacobi_sweeps_per_launch", ctypes.c_int32),
         ("capture_graphs", ctypes.c_int32),
         ("jacobi_chunk_z", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("jacobi_kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -312,7 +313,8 @@ class FireXHipSolver:
                  time_step: Optional[float] = None, maximum_substeps: Optional[int] = None,
                  species_storage: str = "f32", heavy_mode: str = "auto", plan: str = "reference",
                  cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
-                 jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0):
+                 jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0,
+                 jacobi_kernel: str = "auto"):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -336,6 +338,7 @@ class FireXHipSolver:
         cfg.jacobi_sweeps_per_launch = int(jacobi_sweeps_per_launch)
         cfg.capture_graphs = int(bool(capture_graphs))
         cfg.jacobi_chunk_z = int(jacobi_chunk_z)
+        cfg.jacobi_kernel = {"auto": 0, "shift": 1, "ring": 2}[jacobi_kernel]
         self.config = cfg
         self.dims = tuple(int(v) for v in cfg.dims)
         self.cells = self.dims[0] * self.dims[1] * self.dims[2]

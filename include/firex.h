@@ -90,7 +90,9 @@ typedef struct firex_config {
   int32_t capture_graphs;        /* 1: everything after simulate runs as a captured
                                     hipGraph (one launch per step for that part)     */
   int32_t jacobi_chunk_z;        /* FUSED plan: z planes per block; 0 = tuned default */
-  int32_t reserved[5];
+  int32_t jacobi_kernel;         /* FUSED plan: 0 auto, 1 register-shift kernel (any
+                                    grid), 2 register-ring kernel (nx >= 64, ny >= 32) */
+  int32_t reserved[4];
 } firex_config;
 
 /* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */

@@ -211,3 +211,46 @@ def test_captured_graph_replay_bitwise(plan):
         g.sync()
         assert_fields_equal(g, o, f"graph replay {plan} step {step + 1}")
     g.close()
+
+
+def _ring_pair(dims, iters, sweeps, heavy="off", chunk=0):
+    o = orc.Oracle(dims, precision=32, pressure_iterations=iters[0], correction_iterations=iters[1])
+    o.set_bool("cinematic", True)
+    o.set_heavy_mode({"off": orc.HEAVY_OFF, "f16": orc.HEAVY_F16}[heavy])
+    g = FireXHipSolver(dims, tier="hd", pressure_iterations=iters[0], correction_iterations=iters[1], plan="fused",
+                       heavy_mode=heavy, jacobi_sweeps_per_launch=sweeps, jacobi_kernel="ring",
+                       jacobi_chunk_z=chunk)
+    return g, o
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_ring_kernel_sweep_depths_bitwise(sweeps):
+    """Register-ring Jacobi with wall-aligned, overlapping tiles (nx, ny not
+    multiples of the tile interior) and chunks that do not divide nz."""
+    dims = (80, 72, 30)
+    g, o = _ring_pair(dims, (11, 9), sweeps, chunk=7)
+    o.step(DT, 2)
+    g.step(DT, 2)
+    assert_fields_equal(g, o, f"ring K={sweeps}", names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    np.testing.assert_array_equal(_bits(g.read_field("correction")), _bits(o.read(orc.FIELD_CORRECTION)))
+    g.close()
+
+
+@pytest.mark.parametrize("dims,chunk", [((64, 32, 16), 0), ((64, 32, 16), 5), ((128, 96, 40), 0), ((96, 160, 8), 3)])
+def test_ring_kernel_tile_shapes_bitwise(dims, chunk):
+    g, o = _ring_pair(dims, (8, 8), 3, chunk=chunk)
+    o.step(DT, 2)
+    g.step(DT, 2)
+    assert_fields_equal(g, o, f"ring {dims} chunk {chunk}")
+    g.close()
+
+
+def test_ring_kernel_f16_pressure_bitwise():
+    dims = (64, 48, 24)
+    g, o = _ring_pair(dims, (9, 7), 4, heavy="f16", chunk=6)
+    o.step(DT, 3)
+    g.step(DT, 3)
+    assert_fields_equal(g, o, "ring f16 pressure", names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    g.close()
