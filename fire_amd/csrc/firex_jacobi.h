@@ -16,7 +16,8 @@ struct JacobiPlan {
   int32_t rows = 4;               // R: y rows per thread
   int32_t waves = 8;              // W: waves per block (stacked in y)
   int32_t chunk_z = 16;           // output planes per block
-  int32_t kernel = JACOBI_KERNEL_SHIFT;  // SHIFT: any grid; RING: nx >= 64, ny >= W * R
+  int32_t kernel = JACOBI_KERNEL_SHIFT;  // SHIFT: any grid, K <= 8; RING: nx >= 64, ny >= W * R, K <= 5
+  int32_t prefetch = 0;           // RING: load plane t + 1 one iteration before its first use
 };
 
 // kernel: JACOBI_KERNEL_* (AUTO picks RING where the grid allows it);
@@ -32,6 +33,9 @@ struct JacobiLaunch {
   int32_t src;
   bool zero_start;
 };
+
+// Blocks one launch of the plan runs on a grid of dims.
+int jacobi_plan_blocks(const JacobiPlan& plan, const int32_t dims[3]);
 
 // Splits `sweeps` sweeps starting at buffers[start] into launches; returns
 // the number of launches written to out (at most cap) and stores the index

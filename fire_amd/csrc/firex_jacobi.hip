@@ -202,56 +202,85 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_tb(const typename PIO<P>::Wor
 }
 
 // ------------------------------------------------------------- ring kernel
-// The same K-level pipeline with three changes that cut the instructions
-// per cell update:
-//   * every level runs every iteration and the z loop is unrolled by U (a
-//     multiple of 3 and >= K + 1), so each level's three planes and the div
-//     planes live in register rings addressed at compile time (no copies);
-//     a level's values outside its valid z range are never consumed;
-//   * x neighbours come from DPP wave shifts, whose out-of-range lane keeps
-//     its own value;
-//   * tiles are aligned to the domain walls (the first tile starts at 0, the
-//     last one ends at n), so the wall column / floor row sees itself as its
-//     missing neighbour, which is the reference's clamped (Neumann) read,
-//     without a select.  Tiles may overlap; overlapping outputs are computed
-//     from identical inputs and are equal.
-// Needs nx >= 64 and ny >= W * R.
+// The same K-level pipeline, arranged to spend as few instructions as
+// possible per cell update:
+//   * every level runs every iteration and the z loop is unrolled by U, so
+//     each level's planes live in register rings addressed at compile time
+//     (level 0: 3 + PF planes, levels 1..K-1: 3 planes, div: D planes);
+//   * with PF = 1, level s works on plane t - s - 1 while plane t is being
+//     loaded, so the loads of a plane have a whole iteration to arrive (the
+//     barrier that ends an iteration waits for LDS only); with PF = 0 it
+//     works on plane t - s, which needs fewer registers;
+//   * x neighbours come from zero-filling DPP wave shifts, y neighbours at
+//     wave edges from LDS, both without selects;
+//   * the clamped (Neumann) walls -- x = 0, x = nx - 1, the floor y = 0 and
+//     both z walls -- are mirror images: halo cells outside the grid load
+//     the cell mirrored across the wall (-1 - i for cell -1 - i, 2n - 1 - i
+//     for cell i >= n).  The plain update then keeps every such cell equal
+//     to its mirror image level after level (at the wall both sums add the
+//     same two values, and f32 addition commutes), which is exactly the
+//     reference's clamped read, so wall tiles need no per-level fix-ups and
+//     lose validity from the outside in like any other halo.  The top row
+//     y = ny - 1 is set to zero after every level; the last y tile ends
+//     there.
+// Tiles overlap by 2K columns/rows (K at the floor); an output cell is
+// only written by the tile whose valid range holds it.  Needs nx >= 64,
+// ny >= W * R and nz > K.
 
-// lane i <- lane i - 1 (lane 0 keeps v)
-__device__ inline float lane_from_left(float v) {
-  int b = __builtin_bit_cast(int, v);
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));
+// lane i <- lane i - 1 (lane 0 <- 0)
+__device__ inline float shl_from_left(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }
-// lane i <- lane i + 1 (lane 63 keeps v)
-__device__ inline float lane_from_right(float v) {
-  int b = __builtin_bit_cast(int, v);
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));
+// lane i <- lane i + 1 (lane 63 <- 0)
+__device__ inline float shr_from_right(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+}
+// Tiles of width T along an axis of n cells with halo K: tile i starts at
+// -K + i (T - 2K); the last tile ends at n - 1 + K (top = 0) or at n - 1
+// (top = 1: no halo above the last cell).
+__host__ __device__ inline int ring_tiles(int n, int T, int K, int top) {
+  const int span = n + K + (top ? 0 : K);
+  return span <= T ? 1 : 1 + (span - T + (T - 2 * K) - 1) / (T - 2 * K);
 }
 
-__host__ __device__ inline int wall_tiles(int n, int T, int K) {
-  return n <= T ? 1 : 1 + (n - T + (T - 2 * K) - 1) / (T - 2 * K);
-}
-
-// Origin and output range [lo, hi) of tile i of `count` along an axis of n
-// cells, tile width T, halo K.
-__device__ inline void wall_tile(int i, int count, int n, int T, int K, int& origin, int& lo, int& hi) {
+// Origin and output range [lo, hi) of tile i of `count` (count >= 2).
+__device__ inline void ring_tile(int i, int count, int n, int T, int K, int top, int& origin, int& lo, int& hi) {
   if (i == count - 1) {
-    origin = n - T;
+    origin = n + (top ? 0 : K) - T;
     hi = n;
   } else {
-    origin = i * (T - 2 * K);
+    origin = -K + i * (T - 2 * K);
     hi = origin + T - K;
   }
   lo = i == 0 ? 0 : origin + K;
 }
 
-template <int P, int K, int R, int W>
+// Mirror of cell v across the walls of an axis of n cells (-K <= v < n + K, n > K).
+__device__ inline int mirror(int v, int n) { return v < 0 ? -1 - v : v >= n ? 2 * n - 1 - v : v; }
+
+constexpr int kRingMaxSweeps = 5;  // K = 6 no longer unrolls fully (spills to scratch)
+
+// Ring sizes: level 0 keeps M0 planes, div D planes (D >= K + 1 + PF); the
+// z loop is unrolled by U, a multiple of 2 (LDS parity), 3, M0 and D.
+template <int K, int PF>
+struct RingDims {
+  static constexpr int M0 = 3 + PF;
+  static constexpr int D = PF ? (K + 2 <= 4 ? 4 : K + 2 <= 6 ? 6 : 12) : (K + 1 <= 3 ? 3 : 6);
+  static constexpr int U = PF ? 12 : 6;
+  static_assert(K >= 1 && K + 1 + PF <= D, "sweeps per launch out of range");
+  static_assert(U % 2 == 0 && U % 3 == 0 && U % M0 == 0 && U % D == 0, "ring sizes");
+};
+
+template <int P, int K, int R, int W, int PF>
 __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::Word* __restrict__ pin,
                                                         typename PIO<P>::Word* __restrict__ pout,
                                                         const float* __restrict__ div, FxStep st, int chunk_z,
                                                         int tiles_x, int tiles_y, int zero_start) {
   constexpr int H = W * R;
-  constexpr int U = 3 * ((K + 3) / 3);  // >= K + 1, multiple of 3
+  constexpr int M0 = RingDims<K, PF>::M0;
+  constexpr int D = RingDims<K, PF>::D;
+  constexpr int U = RingDims<K, PF>::U;
+  constexpr int LV = K > 1 ? K - 1 : 1;   // levels 1..K-1 kept in rings
   __shared__ float edge[2][K][W][2][64];  // [parity][level][wave][row 0 | row R-1][lane]
 
   const int nx = st.nx, ny = st.ny, nz = st.nz;
@@ -264,8 +293,9 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
   const int tile_y = (int)(rest % (uint32_t)tiles_y);
   const int chunk = (int)(rest / (uint32_t)tiles_y);
   int ox, xlo, xhi, oy, ylo, yhi;
-  wall_tile(tile_x, tiles_x, nx, 64, K, ox, xlo, xhi);
-  wall_tile(tile_y, tiles_y, ny, H, K, oy, ylo, yhi);
+  ring_tile(tile_x, tiles_x, nx, 64, K, 0, ox, xlo, xhi);
+  ring_tile(tile_y, tiles_y, ny, H, K, 1, oy, ylo, yhi);
+  const bool fix_top = tile_y == tiles_y - 1 && wave == W - 1;  // row R - 1 is y = ny - 1
   const int gx = ox + lane;
   const int gy0 = oy + wave * R;
   const int z0 = chunk * chunk_z;
@@ -273,130 +303,167 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
   const uint32_t cells = plane * (uint32_t)nz;
   // Buffer resources: per-row byte offsets in VGPRs, the plane offset in an
-  // SGPR; accesses past the field read 0 / are dropped by the hardware.
-  const __amdgpu_buffer_rsrc_t rin =
-      __builtin_amdgcn_make_buffer_rsrc((void*)pin, (short)0, (int)(cells * PIO<P>::kBytes), kBufferWord3);
+  // SGPR.  A zero start reads the pressure through an empty resource, which
+  // returns 0 for every access.
+  const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)pin, (short)0, zero_start ? 0 : (int)(cells * PIO<P>::kBytes), kBufferWord3);
   const __amdgpu_buffer_rsrc_t rout =
       __builtin_amdgcn_make_buffer_rsrc((void*)pout, (short)0, (int)(cells * PIO<P>::kBytes), kBufferWord3);
   const __amdgpu_buffer_rsrc_t rdiv =
       __builtin_amdgcn_make_buffer_rsrc((void*)div, (short)0, (int)(cells * 4u), kBufferWord3);
-  uint32_t rowoff[R];
-  bool top[R], out_row[R];
+  // Addressing: the (clamped) column is the one per-lane offset; the
+  // (clamped) row offsets are wave-uniform and go with the plane offset.
+  const uint32_t cx = (uint32_t)mirror(gx, nx);
+  uint32_t rowoff[R];  // wave-uniform
+  bool y_out[R];       // wave-uniform
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    rowoff[r] = (uint32_t)gx + (uint32_t)nx * (uint32_t)(gy0 + r);
-    top[r] = gy0 + r == ny - 1;
-    out_row[r] = gy0 + r >= ylo && gy0 + r < yhi && gx >= xlo && gx < xhi;
+    const int y = gy0 + r;
+    rowoff[r] = (uint32_t)nx * (uint32_t)mirror(y, ny);
+    y_out[r] = y >= ylo && y < yhi;
   }
+  const bool x_out = gx >= xlo && gx < xhi;
   const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.jacobi_inv;
+  // wave-edge neighbours; wave 0's "down" and wave W-1's "up" are halo rows
+  // or the zeroed top row, so any in-bounds slot will do
+  const int wdn = wave > 0 ? wave - 1 : W - 1;
+  const int wup = wave < W - 1 ? wave + 1 : 0;
 
-  float Pr[K][3][R], Dv[U][R];
+  float P0[M0][R], Pr[LV][3][R], Dv[D][R];
 #pragma unroll
-  for (int s = 0; s < K; ++s)
+  for (int q = 0; q < M0; ++q)
+#pragma unroll
+    for (int r = 0; r < R; ++r) P0[q][r] = 0.0f;
+#pragma unroll
+  for (int s = 0; s < LV; ++s)
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
       for (int r = 0; r < R; ++r) Pr[s][q][r] = 0.0f;
 #pragma unroll
-  for (int q = 0; q < U; ++q)
+  for (int q = 0; q < D; ++q)
 #pragma unroll
     for (int r = 0; r < R; ++r) Dv[q][r] = 0.0f;
 
+  // Iteration t loads plane t and level s computes plane t - s - PF, so the
+  // chunk [z0, z1) takes planes z0 - K .. z1 + K - 1 and (z1 - z0) + 2K + PF
+  // iterations.
   const int t0 = z0 - K;
-  const int n_it = (z1 - z0) + 2 * K;
-  for (int tb = 0; tb < n_it; tb += U) {
+  const int n_it = (z1 - z0) + 2 * K + PF;
+  for (int tb = 0;; tb += U) {
 #pragma unroll
     for (int ph = 0; ph < U; ++ph) {
+      if (tb + ph >= n_it) return;  // block-uniform: every wave leaves here
       const int t = t0 + tb + ph;
-      const int par = (tb + ph) & 1;
+      const int par = ph & 1;  // U is even: the parity of the iteration
       {
-        // The clamped plane index makes the virtual planes -1 and nz copies
-        // of the wall planes: the reference's clamped z reads at level 0.
-        const uint32_t pbase = (uint32_t)min(max(t, 0), nz - 1) * plane;
+        // Planes outside the grid load their mirror image.  (With PF = 1
+        // the last iteration's load is not used.  It stays unconditional: a
+        // load that may or may not have been issued makes the compiler's
+        // memory counter waits stall on the newest loads.)
+        const uint32_t pbase = (uint32_t)min(max(mirror(t, nz), 0), nz - 1) * plane;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          Pr[0][ph % 3][r] = zero_start ? 0.0f : PIO<P>::bload(rin, rowoff[r] * PIO<P>::kBytes, pbase * PIO<P>::kBytes);
-          Dv[ph][r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)(rowoff[r] * 4u),
-                                                                                    (int)(pbase * 4u), 0));
+          P0[ph % M0][r] = PIO<P>::bload(rin, cx * PIO<P>::kBytes, (pbase + rowoff[r]) * PIO<P>::kBytes);
+          Dv[ph % D][r] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)(cx * 4u), (int)((pbase + rowoff[r]) * 4u), 0));
         }
       }
 #pragma unroll
       for (int s = 1; s <= K; ++s) {
-        const int z = t - s;
-        const int sc = ((ph - s) % 3 + 3) % 3;      // slot of plane z
-        const int sa = ((ph - s - 1) % 3 + 3) % 3;  // plane z - 1
-        const int sb = ((ph - s + 1) % 3 + 3) % 3;  // plane z + 1
-        const int sd = ((ph - s) % U + U) % U;      // div of plane z
+        const int z = t - s - PF;
+        // ring slots (plane - t0 mod ring size) of planes z - 1, z, z + 1 of
+        // level s - 1, and of z's div
+        const int m0 = s == 1 ? M0 : 3;
+        const int sa = ((ph - s - PF - 1) % m0 + m0) % m0;
+        const int sc = ((ph - s - PF) % m0 + m0) % m0;
+        const int sb = ((ph - s - PF + 1) % m0 + m0) % m0;
+        const int sd = ((ph - s - PF) % D + D) % D;
+        const int lv = s >= 2 ? s - 2 : 0;
+        const float* below = s == 1 ? P0[sa] : Pr[lv][sa];
+        const float* centre = s == 1 ? P0[sc] : Pr[lv][sc];
+        const float* above = s == 1 ? P0[sb] : Pr[lv][sb];
         float res[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const float ctr = Pr[s - 1][sc][r];
-          const float left = lane_from_left(ctr);
-          const float right = lane_from_right(ctr);
-          float down, up;
-          if (r > 0) down = Pr[s - 1][sc][r - 1];
-          else down = wave > 0 ? edge[par][s - 1][wave - 1][1][lane] : ctr;
-          if (r < R - 1) up = Pr[s - 1][sc][r + 1];
-          else up = wave < W - 1 ? edge[par][s - 1][wave + 1][0][lane] : ctr;
-          const float nb = wx * (left + right) + wy * (down + up) + wz * (Pr[s - 1][sa][r] + Pr[s - 1][sb][r]);
-          float v = (nb - Dv[sd][r]) * inv;
-          if (top[r]) v = 0.0f;
-          res[r] = PIO<P>::round(v);
+          const float ctr = centre[r];
+          const float left = shl_from_left(ctr);
+          const float right = shr_from_right(ctr);
+          const float down = r > 0 ? centre[r - 1] : edge[par][s - 1][wdn][1][lane];
+          const float up = r < R - 1 ? centre[r + 1] : edge[par][s - 1][wup][0][lane];
+          const float nb = wx * (left + right) + wy * (down + up) + wz * (below[r] + above[r]);
+          res[r] = PIO<P>::round((nb - Dv[sd][r]) * inv);
         }
+        if (fix_top) res[R - 1] = 0.0f;
         if (s < K) {
+          // level s keeps planes in slots (plane - t0) % 3
+          const int qc = ((ph - s - PF) % 3 + 3) % 3;
 #pragma unroll
-          for (int r = 0; r < R; ++r) Pr[s][sc][r] = res[r];
-          // Keep the virtual planes beside the z walls equal to the wall
-          // plane for the level above (its clamped z neighbour).
-          if (z == 0) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) Pr[s][sa][r] = res[r];
-          } else if (z == nz) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) Pr[s][sc][r] = Pr[s][sa][r];
-          }
+          for (int r = 0; r < R; ++r) Pr[s - 1][qc][r] = res[r];
         } else if (z >= z0 && z < z1) {
           const uint32_t pbase = (uint32_t)z * plane * PIO<P>::kBytes;
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            if (out_row[r]) PIO<P>::bstore(rout, rowoff[r] * PIO<P>::kBytes, pbase, res[r]);
+            if (y_out[r] && x_out)
+              PIO<P>::bstore(rout, cx * PIO<P>::kBytes, pbase + rowoff[r] * PIO<P>::kBytes, res[r]);
         }
       }
+      // Publish rows 0 and R-1 of the plane each level's neighbours use next
+      // iteration (level q: plane t - q - PF), then swap parity.
+      {
+        const int q0 = ((ph - PF) % M0 + M0) % M0;
+        edge[par ^ 1][0][wave][0][lane] = P0[q0][0];
+        edge[par ^ 1][0][wave][1][lane] = P0[q0][R - 1];
+      }
 #pragma unroll
-      for (int s = 0; s < K; ++s) {
-        const int sn = ((ph - s) % 3 + 3) % 3;  // newest plane of level s
-        edge[par ^ 1][s][wave][0][lane] = Pr[s][sn][0];
-        edge[par ^ 1][s][wave][1][lane] = Pr[s][sn][R - 1];
+      for (int s = 1; s < K; ++s) {
+        const int q = ((ph - s - PF) % 3 + 3) % 3;
+        edge[par ^ 1][s][wave][0][lane] = Pr[s - 1][q][0];
+        edge[par ^ 1][s][wave][1][lane] = Pr[s - 1][q][R - 1];
       }
       __syncthreads();
     }
   }
 }
 
-template <int P, int K, int R, int W>
+template <int P, int K, int R, int W, int PF>
 void launch_ring(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
                  int zero_start, hipStream_t s) {
-  const int tiles_x = wall_tiles(st.nx, 64, K);
-  const int tiles_y = wall_tiles(st.ny, W * R, K);
+  const int tiles_x = ring_tiles(st.nx, 64, K, 0);
+  const int tiles_y = ring_tiles(st.ny, W * R, K, 1);
   const int chunks = (st.nz + plan.chunk_z - 1) / plan.chunk_z;
   const uint32_t blocks = (uint32_t)tiles_x * (uint32_t)tiles_y * (uint32_t)chunks;
   using Word = typename PIO<P>::Word;
-  k_jacobi_ring<P, K, R, W><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, st, plan.chunk_z, tiles_x,
+  k_jacobi_ring<P, K, R, W, PF><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, st, plan.chunk_z, tiles_x,
                                                        tiles_y, zero_start);
 }
 
+template <int P, int R, int W, int PF>
+void launch_ring_rw(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
+                    int zero_start, hipStream_t s) {
+  switch (k) {
+    case 1: launch_ring<P, 1, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
+    case 2: launch_ring<P, 2, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
+    case 3: launch_ring<P, 3, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
+    case 4: launch_ring<P, 4, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
+    default: launch_ring<P, 5, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
+  }
+}
+
+// Ring variants: tile shape (rows per thread x waves per block, 64 x 32
+// cells either way) and prefetch distance.
 template <int P>
 void launch_ring_k(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
                    int zero_start, hipStream_t s) {
-  switch (k) {
-    case 1: launch_ring<P, 1, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    case 2: launch_ring<P, 2, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    case 3: launch_ring<P, 3, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    case 4: launch_ring<P, 4, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    case 5: launch_ring<P, 5, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    case 6: launch_ring<P, 6, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    case 7: launch_ring<P, 7, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
-    default: launch_ring<P, 8, 4, 8>(plan, pin, pout, div, st, zero_start, s); break;
+  if (plan.rows == 8) {
+    if (plan.prefetch)
+      launch_ring_rw<P, 8, 4, 1>(k, plan, pin, pout, div, st, zero_start, s);
+    else
+      launch_ring_rw<P, 8, 4, 0>(k, plan, pin, pout, div, st, zero_start, s);
+  } else if (plan.prefetch) {
+    launch_ring_rw<P, 4, 8, 1>(k, plan, pin, pout, div, st, zero_start, s);
+  } else {
+    launch_ring_rw<P, 4, 8, 0>(k, plan, pin, pout, div, st, zero_start, s);
   }
 }
 
@@ -496,37 +563,43 @@ __global__ __launch_bounds__(256) void k_project_divergence(const float4* __rest
 }  // namespace
 
 bool jacobi_plan_valid(const JacobiPlan& plan) {
-  return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= 8 && plan.rows == 4 && plan.waves == 8 &&
-         plan.chunk_z >= 1 && (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);
+  const int max_k = plan.kernel == JACOBI_KERNEL_RING ? kRingMaxSweeps : 8;
+  const bool shape = plan.kernel == JACOBI_KERNEL_RING
+      ? ((plan.rows == 4 && plan.waves == 8) || (plan.rows == 8 && plan.waves == 4)) && (plan.prefetch == 0 || plan.prefetch == 1)
+      : plan.rows == 4 && plan.waves == 8;
+  return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= max_k && shape && plan.chunk_z >= 1 &&
+         (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);
+}
+
+int jacobi_plan_blocks(const JacobiPlan& p, const int32_t dims[3]) {
+  const int K = p.sweeps_per_launch;
+  const int tiles = p.kernel == JACOBI_KERNEL_RING
+      ? ring_tiles(dims[0], 64, K, 0) * ring_tiles(dims[1], p.waves * p.rows, K, 1)
+      : ((dims[0] + 63 - 2 * K) / (64 - 2 * K)) * ((dims[1] + 31 - 2 * K) / (32 - 2 * K));
+  return tiles * ((dims[2] + p.chunk_z - 1) / p.chunk_z);
 }
 
 JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel) {
   JacobiPlan p;
-  const bool ring_ok = dims[0] >= 64 && dims[1] >= p.waves * p.rows;
+  // The ring kernel addresses fields through buffer resources with 32-bit
+  // byte offsets: fields up to 2 GiB.
+  const uint64_t field_bytes = (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2] * 4u;
+  const bool ring_ok = dims[0] >= 64 && dims[1] >= p.waves * p.rows && dims[2] >= 3 && field_bytes < (1ull << 31);
   p.kernel = (kernel == JACOBI_KERNEL_RING && ring_ok) ? JACOBI_KERNEL_RING
            : kernel == JACOBI_KERNEL_SHIFT ? JACOBI_KERNEL_SHIFT
            : ring_ok ? JACOBI_KERNEL_RING : JACOBI_KERNEL_SHIFT;
-  const int dflt = p.kernel == JACOBI_KERNEL_RING ? 2 : 4;
-  p.sweeps_per_launch = sweeps_per_launch > 0 ? (sweeps_per_launch > 8 ? 8 : sweeps_per_launch) : dflt;
-  const int K = p.sweeps_per_launch;
+  // the ring kernel's mirrored z halo needs nz >= K + 2
+  const int max_k = p.kernel == JACOBI_KERNEL_RING ? (dims[2] - 2 < kRingMaxSweeps ? dims[2] - 2 : kRingMaxSweeps) : 8;
+  const int dflt = p.kernel == JACOBI_KERNEL_RING ? 3 : 4;
+  const int k = sweeps_per_launch > 0 ? sweeps_per_launch : dflt;
+  p.sweeps_per_launch = k > max_k ? max_k : k;
   if (chunk_z > 0) {
     p.chunk_z = chunk_z;
     return p;
   }
   // Enough z chunks to give every CU a couple of blocks.
-  const int tiles = p.kernel == JACOBI_KERNEL_RING
-      ? wall_tiles(dims[0], 64, K) * wall_tiles(dims[1], p.waves * p.rows, K)
-      : ((dims[0] + 63 - 2 * K) / (64 - 2 * K)) * ((dims[1] + 31 - 2 * K) / (32 - 2 * K));
-  int chunk = dims[2];
-  while (chunk > 8 && (long)tiles * ((dims[2] + chunk - 1) / chunk) < 512) chunk = (chunk + 1) / 2;
-  if (p.kernel == JACOBI_KERNEL_RING) {
-    // the ring kernel runs whole groups of U planes: prefer chunk + 2K % U == 0
-    const int U = 3 * ((K + 3) / 3);
-    int c = chunk;
-    while (c > 1 && (c + 2 * K) % U != 0) --c;
-    chunk = c;
-  }
-  p.chunk_z = chunk;
+  p.chunk_z = dims[2];
+  while (p.chunk_z > 8 && jacobi_plan_blocks(p, dims) < 512) p.chunk_z = (p.chunk_z + 1) / 2;
   return p;
 }
 
