@@ -33,8 +33,11 @@ DT = 1.0 / 60.0
 # species 16 + reaction 4 (no liquid input); divergence 16 + 4; one Jacobi
 # sweep reads p and div and writes p; projection reads flow 16 + p 4 and
 # writes flow 16; the fused projection also writes div 4; copyFlow 32.
+# Algorithmic HBM bytes per cell and launch (each input read once, each
+# output written once).  A temporally blocked Jacobi launch reads p and div
+# and writes p once per cell whatever its number of sweeps: 12 B (f32 p).
 BYTES_PER_CELL = {
-    "simulate_f32": 68, "simulate_f16": 52, "divergence": 20, "jacobi_sweep": 12,
+    "simulate_f32": 68, "simulate_f16": 52, "divergence": 20, "jacobi_launch": 12,
     "project": 36, "project_divergence": 40, "fill_copy": 32,
 }
 
@@ -48,12 +51,12 @@ WORKLOADS = {
 }
 
 
-def make_solver(wl: dict, plan: str, sweeps: int = 0):
+def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False):
     from fire_amd import FireXHipSolver, seeded_emitters
 
     s = FireXHipSolver(wl["dims"], tier="uhd", pressure_iterations=wl["iters"][0],
                        correction_iterations=wl["iters"][1], species_storage=wl["storage"], plan=plan,
-                       heavy_mode="off", jacobi_sweeps_per_launch=sweeps)
+                       heavy_mode="off", jacobi_sweeps_per_launch=sweeps, jacobi_autotune=autotune)
     em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
     s.set_emitters(em, jitter=wl["jitter"], velocity_noise=wl["velocity_noise"], seed=wl["seed"])
     return s
@@ -81,15 +84,16 @@ def pass_profile(solver, steps: int, cells: int, storage: str) -> dict:
         if name == "simulate":
             bpc = BYTES_PER_CELL["simulate_f16" if storage != "f32" else "simulate_f32"]
             units = n
-        elif name == "jacobi":
-            bpc = BYTES_PER_CELL["jacobi_sweep"]
-            units = t["jacobi_sweeps"]
         else:
-            bpc = BYTES_PER_CELL[name]
+            bpc = BYTES_PER_CELL["jacobi_launch" if name == "jacobi" else name]
             units = n
         gbs = bpc * cells * units / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         out[name] = {"ms_per_step": ms / t["steps"], "launches_per_step": n / t["steps"],
                      "avg_launch_us": ms / n * 1e3, "algorithmic_GBps": round(gbs, 1)}
+        if name == "jacobi" and ms > 0:
+            # what one-sweep-per-launch kernels would have to stream to match
+            out[name]["sweep_equivalent_GBps"] = round(
+                BYTES_PER_CELL["jacobi_launch"] * cells * t["jacobi_sweeps"] / (ms * 1e-3) / 1e9, 1)
     out["jacobi_sweeps_per_step"] = t["jacobi_sweeps"] / max(1, t["steps"])
     out["_raw"] = t
     return out
@@ -127,7 +131,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="uhd-mixed", choices=sorted(WORKLOADS))
     ap.add_argument("--plan", default="fused", choices=["fused", "reference"])
-    ap.add_argument("--sweeps", type=int, default=0, help="Jacobi sweeps per launch (0 = tuned default)")
+    ap.add_argument("--sweeps", type=int, default=0, help="Jacobi sweeps per launch (0 = autotuned)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default Jacobi plan instead of tuning")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=60, help="steps timed on the plain f32 path (0 = skip)")
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
@@ -149,7 +154,11 @@ def main() -> int:
     print(f"bench: workload {args.workload} {wl['dims']} plan {args.plan} steps {args.steps} warmup {args.warmup}",
           file=sys.stderr, flush=True)
 
-    solver = make_solver(wl, args.plan, args.sweeps)
+    t_setup = time.perf_counter()
+    solver = make_solver(wl, args.plan, args.sweeps, autotune=not args.no_autotune and args.plan == "fused")
+    jacobi_plan = solver.info["jacobi"]
+    print(f"bench: solver ready in {time.perf_counter() - t_setup:.2f} s, Jacobi plan {jacobi_plan}",
+          file=sys.stderr, flush=True)
     elapsed = time_steps(solver, args.steps, args.warmup)
     ms_per_step = elapsed / args.steps * 1e3
     value = cells * args.steps / elapsed
@@ -162,11 +171,12 @@ def main() -> int:
     raw = passes.pop("_raw")
     sweeps_per_launch = raw["jacobi_sweeps"] / max(1, raw["jacobi"]["launches"])
     avg_launch_s = jac.get("avg_launch_us", 0.0) * 1e-6
-    bytes_per_launch = BYTES_PER_CELL["jacobi_sweep"] * cells * sweeps_per_launch
+    bytes_per_launch = BYTES_PER_CELL["jacobi_launch"] * cells
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                "kernel": f"k_jacobi_tb (Jacobi, {sweeps_per_launch:g} sweeps per launch)",
+                "kernel": f"k_jacobi_{jacobi_plan['kernel']} (Jacobi, {sweeps_per_launch:g} sweeps per launch, "
+                          f"z chunk {jacobi_plan['chunk_z']})",
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "avg_launch_us": round(jac.get("avg_launch_us", 0.0), 2)}
 
@@ -200,7 +210,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": args.workload, "grid": list(wl["dims"]), "cells": cells,
                    "jacobi": list(wl["iters"]), "species_storage": wl["storage"], "dt": DT,
-                   "emitters": wl["emitters"], "seed": wl["seed"], "plan": args.plan,
+                   "emitters": wl["emitters"], "seed": wl["seed"], "plan": args.plan, "jacobi_plan": jacobi_plan,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

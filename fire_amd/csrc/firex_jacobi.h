@@ -56,4 +56,14 @@ void launch_project_divergence(int f16, const float4* fin, const void* p, float4
 // Host-side check of a plan against the kernel's assumptions.
 bool jacobi_plan_valid(const JacobiPlan& plan);
 
+// Autotuner: times candidate plans for a grid of dims on the current device
+// and returns the fastest in *out (and its time per sweep in microseconds).
+// sweeps_per_launch / chunk_z > 0 and kernel != AUTO are kept fixed; the
+// rest is searched.  buffers (two pressure fields in the given storage) and
+// div are scratch: their contents are overwritten.  Results are cached per
+// device, shape, storage and fixed parameters, so a second solver of the
+// same shape does not time again.  Returns a hipError_t (0 on success).
+int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
+                     void* const buffers[2], float* div, hipStream_t s, JacobiPlan* out, float* us_per_sweep);
+
 }  // namespace fx

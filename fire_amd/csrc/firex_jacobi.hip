@@ -26,6 +26,13 @@
 // separate sweeps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include "firex_jacobi.h"
 #include "firex_kernels.h"
 
@@ -300,6 +307,14 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
   const int gy0 = oy + wave * R;
   const int z0 = chunk * chunk_z;
   const int z1 = min(z0 + chunk_z, nz);
+  // Odd chunks stream downwards: the pipeline runs on stream coordinates
+  // (planes z0 - K .. z1 + K - 1 in order) and zr = z0 + z1 - 1 maps them to
+  // grid planes back to front.  Neighbouring chunks then read their shared
+  // halo planes at the same time (both at the start or both at the end of
+  // their streams), which lets the second read hit in L2.  The update is
+  // symmetric in z (below + above commutes), so the result is the same.
+  const int zr = (chunk & 1) ? z0 + z1 - 1 : 0;
+  const int zsign = (chunk & 1) ? -1 : 1;
   const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
   const uint32_t cells = plane * (uint32_t)nz;
   // Buffer resources: per-row byte offsets in VGPRs, the plane offset in an
@@ -361,7 +376,7 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
         // the last iteration's load is not used.  It stays unconditional: a
         // load that may or may not have been issued makes the compiler's
         // memory counter waits stall on the newest loads.)
-        const uint32_t pbase = (uint32_t)min(max(mirror(t, nz), 0), nz - 1) * plane;
+        const uint32_t pbase = (uint32_t)min(max(mirror(zr + zsign * t, nz), 0), nz - 1) * plane;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           P0[ph % M0][r] = PIO<P>::bload(rin, cx * PIO<P>::kBytes, (pbase + rowoff[r]) * PIO<P>::kBytes);
@@ -401,7 +416,7 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
 #pragma unroll
           for (int r = 0; r < R; ++r) Pr[s - 1][qc][r] = res[r];
         } else if (z >= z0 && z < z1) {
-          const uint32_t pbase = (uint32_t)z * plane * PIO<P>::kBytes;
+          const uint32_t pbase = (uint32_t)(zr + zsign * z) * plane * PIO<P>::kBytes;
 #pragma unroll
           for (int r = 0; r < R; ++r)
             if (y_out[r] && x_out)
@@ -635,6 +650,144 @@ void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, vo
   } else {
     launch_tb_k<0>(l.sweeps, plan, pin, pout, div, st, zs, s);
   }
+}
+
+// ------------------------------------------------------------------ tuner
+
+namespace {
+
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;  // device, dims, f16, K, chunk, kernel
+struct TuneEntry {
+  JacobiPlan plan;
+  float us_per_sweep;
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, TuneEntry> g_tune_cache;
+
+// z chunk sizes worth timing: about 10% apart, giving 200..3000 blocks.
+std::vector<int> chunk_candidates(int nz, long tiles) {
+  std::vector<int> out;
+  for (int n = 1; n <= nz; ++n) {
+    const int c = (nz + n - 1) / n;
+    if (c < 4) break;
+    const long blocks = tiles * ((nz + c - 1) / c);
+    if (blocks < 200 || blocks > 3000) continue;
+    if (!out.empty() && c * 10 > out.back() * 9) continue;
+    out.push_back(c);
+  }
+  if (out.empty()) out.push_back(std::max(1, std::min(nz, 16)));
+  return out;
+}
+
+}  // namespace
+
+int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
+                     void* const buffers[2], float* div, hipStream_t s, JacobiPlan* out, float* us_per_sweep) {
+  int device = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return e;
+  const TuneKey key{device, dims[0], dims[1], dims[2], f16, sweeps_per_launch, chunk_z, kernel};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      *out = it->second.plan;
+      if (us_per_sweep) *us_per_sweep = it->second.us_per_sweep;
+      return hipSuccess;
+    }
+  }
+  // Candidates: the default plan first, then every kernel / depth / chunk /
+  // prefetch combination the fixed parameters allow.
+  std::vector<JacobiPlan> cands;
+  cands.push_back(make_jacobi_plan(dims, sweeps_per_launch, chunk_z, kernel));
+  const int kernels[2] = {JACOBI_KERNEL_RING, JACOBI_KERNEL_SHIFT};
+  for (int kern : kernels) {
+    if (kernel != JACOBI_KERNEL_AUTO && kernel != kern) continue;
+    const JacobiPlan probe = make_jacobi_plan(dims, 1, 1, kern);
+    if (probe.kernel != kern) continue;  // this grid does not allow it
+    // the ring kernel, where allowed, is always the faster one
+    if (kern == JACOBI_KERNEL_SHIFT && kernel == JACOBI_KERNEL_AUTO && cands[0].kernel == JACOBI_KERNEL_RING) continue;
+    const int kmax = kern == JACOBI_KERNEL_RING ? kRingMaxSweeps : 4;
+    for (int K = sweeps_per_launch > 0 ? sweeps_per_launch : 2; K <= (sweeps_per_launch > 0 ? sweeps_per_launch : kmax);
+         ++K) {
+      JacobiPlan base = make_jacobi_plan(dims, K, 1, kern);
+      if (base.sweeps_per_launch != K) continue;
+      base.chunk_z = 1;
+      const long tiles = jacobi_plan_blocks(base, dims) / dims[2];
+      std::vector<int> chunks = chunk_z > 0 ? std::vector<int>{chunk_z} : chunk_candidates(dims[2], tiles);
+      for (int c : chunks) {
+        for (int pf = 0; pf <= (kern == JACOBI_KERNEL_RING && K <= 3 ? 1 : 0); ++pf) {
+          JacobiPlan p = base;
+          p.chunk_z = c;
+          p.prefetch = pf;
+          if (jacobi_plan_valid(p)) cands.push_back(p);
+        }
+      }
+    }
+  }
+
+  FxStep st;
+  memset(&st, 0, sizeof st);
+  st.nx = dims[0];
+  st.ny = dims[1];
+  st.nz = dims[2];
+  st.wx = st.wy = st.wz = 1.0f;
+  st.jacobi_inv = 1.0f / 6.0f;
+  const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+  const size_t pbytes = (cells * (f16 ? 2 : 4) + 3) / 4 * 4;
+  if ((e = hipMemsetAsync(buffers[0], 0, pbytes, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(buffers[1], 0, pbytes, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(div, 0, cells * 4, s)) != hipSuccess) return e;
+  hipEvent_t ev[2];
+  if ((e = hipEventCreate(&ev[0])) != hipSuccess) return e;
+  if ((e = hipEventCreate(&ev[1])) != hipSuccess) {
+    hipEventDestroy(ev[0]);
+    return e;
+  }
+  auto time_plan = [&](const JacobiPlan& p, int reps, float* us) -> hipError_t {
+    const JacobiLaunch l0{p.sweeps_per_launch, 0, false}, l1{p.sweeps_per_launch, 1, false};
+    launch_jacobi_tb(p, f16, l0, buffers, div, st, s);
+    hipError_t err = hipEventRecord(ev[0], s);
+    for (int r = 0; r < reps && err == hipSuccess; ++r) launch_jacobi_tb(p, f16, (r & 1) ? l0 : l1, buffers, div, st, s);
+    if (err == hipSuccess) err = hipGetLastError();
+    if (err == hipSuccess) err = hipEventRecord(ev[1], s);
+    if (err == hipSuccess) err = hipEventSynchronize(ev[1]);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    *us = ms * 1000.0f / (float)(reps * p.sweeps_per_launch);
+    return err;
+  };
+  // Warm the clocks, then a quick pass over all candidates and a longer
+  // one over the best few.
+  std::vector<std::pair<float, int>> score;
+  float us = 0;
+  e = time_plan(cands[0], 12, &us);
+  for (size_t i = 0; i < cands.size() && e == hipSuccess; ++i) {
+    e = time_plan(cands[i], 3, &us);
+    score.emplace_back(us, (int)i);
+  }
+  if (e == hipSuccess) {
+    std::sort(score.begin(), score.end());
+    const size_t top = std::min<size_t>(score.size(), 6);
+    for (size_t j = 0; j < top && e == hipSuccess; ++j) {
+      float a = 0, b = 0;
+      e = time_plan(cands[score[j].second], 8, &a);
+      if (e == hipSuccess) e = time_plan(cands[score[j].second], 8, &b);
+      score[j].first = std::min(a, b);
+    }
+    std::sort(score.begin(), score.begin() + top);
+  }
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  if (e != hipSuccess) return e;
+  const TuneEntry best{cands[score[0].second], score[0].first};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune_cache[key] = best;
+  }
+  *out = best.plan;
+  if (us_per_sweep) *us_per_sweep = best.us_per_sweep;
+  return hipSuccess;
 }
 
 void launch_project_divergence(int f16, const float4* fin, const void* p, float4* fout, float* div, const FxStep& st,

@@ -112,6 +112,7 @@ firex_config plume_config(const firex_server* s, const firex_request& r) {
   c.plan = s->cfg.plan;
   c.device = s->cfg.device;
   c.capture_graphs = s->cfg.capture_graphs;
+  c.jacobi_autotune = s->cfg.jacobi_autotune;
   return c;
 }
 

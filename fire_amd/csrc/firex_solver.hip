@@ -45,6 +45,7 @@ struct firex_solver {
   bool own_stream = true;
   uint64_t bytes = 0;
   JacobiPlan jacobi;
+  float jacobi_tuned_us = 0;  // autotuner's time per sweep (0: not tuned)
   int corr_index = 0;      // buffer holding the last residual solution
   // HIP-event timing of launches (firex_set_pass_timing)
   struct TimedLaunch {
@@ -467,8 +468,18 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
   if (!rc) rc = alloc(s, (void**)&s->reaction, c * 4, "reaction");
   if (!rc) rc = alloc(s, (void**)&s->scratch, kReduceScratchBytes, "reduction scratch");
   if (!rc) {
-    s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z, cfg->jacobi_kernel);
-    rc = initialize_fields(s);
+    if (cfg->jacobi_autotune && cfg->plan == FIREX_PLAN_FUSED) {
+      // Times candidate plans on the solver's own pressure and divergence
+      // fields, which initialize_fields() clears afterwards.
+      void* bufs[2] = {s->pressure[0], s->pressure[1]};
+      e = (hipError_t)tune_jacobi_plan(s->dims, f16p ? 1 : 0, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z,
+                                       cfg->jacobi_kernel, bufs, s->divergence, s->stream, &s->jacobi,
+                                       &s->jacobi_tuned_us);
+      if (e != hipSuccess) rc = hip_fail(s, e, "Jacobi plan autotuning");
+    } else {
+      s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z, cfg->jacobi_kernel);
+    }
+    if (!rc) rc = initialize_fields(s);
   }
   if (!rc && s->own_stream) {
     e = hipStreamSynchronize(s->stream);
@@ -827,6 +838,11 @@ int firex_get_info(firex_solver* s, firex_info* info) {
   info->species_storage = s->cfg.species_storage;
   info->plan = s->cfg.plan;
   info->device_bytes = s->bytes;
+  info->jacobi_kernel = s->jacobi.kernel;
+  info->jacobi_sweeps_per_launch = s->jacobi.sweeps_per_launch;
+  info->jacobi_chunk_z = s->jacobi.chunk_z;
+  info->jacobi_prefetch = s->jacobi.prefetch;
+  info->jacobi_tuned_us_per_sweep = s->jacobi_tuned_us;
   return FIREX_OK;
 }
 

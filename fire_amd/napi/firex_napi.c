@@ -427,6 +427,8 @@ static napi_value js_get_info(napi_env env, napi_callback_info info) {
   set_num(env, o, "heavy", fi.heavy);
   set_num(env, o, "f16Pressure", fi.f16_pressure);
   set_num(env, o, "deviceBytes", (double)fi.device_bytes);
+  set_num(env, o, "jacobiSweepsPerLaunch", fi.jacobi_sweeps_per_launch);
+  set_num(env, o, "jacobiChunkZ", fi.jacobi_chunk_z);
   return o;
 }
 

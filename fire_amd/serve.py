@@ -29,7 +29,8 @@ class ServerConfig(ctypes.Structure):
         ("plan", ctypes.c_int32),
         ("capture_graphs", ctypes.c_int32),
         ("max_inflight_steps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("jacobi_autotune", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -133,7 +134,8 @@ def make_request(dims: Sequence[int], steps: int, *, pressure_iterations: int = 
 
 class FireXServer:
     def __init__(self, memory_budget_bytes: int = 0, max_active: int = 16, streams: int = 4, plan: str = "fused",
-                 capture_graphs: bool = True, max_inflight_steps: int = 2, device: int = 0):
+                 capture_graphs: bool = True, max_inflight_steps: int = 2, device: int = 0,
+                 jacobi_autotune: bool = False):
         self._lib = _lib()
         cfg = ServerConfig()
         cfg.device = int(device)
@@ -143,6 +145,7 @@ class FireXServer:
         cfg.plan = PLANS[plan]
         cfg.capture_graphs = int(bool(capture_graphs))
         cfg.max_inflight_steps = int(max_inflight_steps)
+        cfg.jacobi_autotune = int(bool(jacobi_autotune))
         h = ctypes.c_void_p()
         _check(self._lib.firex_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

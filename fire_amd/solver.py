@@ -84,7 +84,8 @@ class Config(ctypes.Structure):
         ("capture_graphs", ctypes.c_int32),
         ("jacobi_chunk_z", ctypes.c_int32),
         ("jacobi_kernel", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("jacobi_autotune", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -105,6 +106,12 @@ class Info(ctypes.Structure):
         ("species_storage", ctypes.c_int32),
         ("plan", ctypes.c_int32),
         ("device_bytes", ctypes.c_uint64),
+        ("jacobi_kernel", ctypes.c_int32),
+        ("jacobi_sweeps_per_launch", ctypes.c_int32),
+        ("jacobi_chunk_z", ctypes.c_int32),
+        ("jacobi_prefetch", ctypes.c_int32),
+        ("jacobi_tuned_us_per_sweep", ctypes.c_float),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -314,7 +321,7 @@ class FireXHipSolver:
                  species_storage: str = "f32", heavy_mode: str = "auto", plan: str = "reference",
                  cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
                  jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0,
-                 jacobi_kernel: str = "auto"):
+                 jacobi_kernel: str = "auto", jacobi_autotune: bool = False):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -339,6 +346,7 @@ class FireXHipSolver:
         cfg.capture_graphs = int(bool(capture_graphs))
         cfg.jacobi_chunk_z = int(jacobi_chunk_z)
         cfg.jacobi_kernel = {"auto": 0, "shift": 1, "ring": 2}[jacobi_kernel]
+        cfg.jacobi_autotune = int(bool(jacobi_autotune))
         self.config = cfg
         self.dims = tuple(int(v) for v in cfg.dims)
         self.cells = self.dims[0] * self.dims[1] * self.dims[2]
@@ -457,6 +465,10 @@ class FireXHipSolver:
             "frame_index": inf.frame_index, "species_index": inf.species_index,
             "pressure_index": inf.pressure_index, "heavy": bool(inf.heavy), "f16_pressure": bool(inf.f16_pressure),
             "species_storage": inf.species_storage, "plan": inf.plan, "device_bytes": int(inf.device_bytes),
+            "jacobi": {"kernel": {1: "shift", 2: "ring"}.get(inf.jacobi_kernel, inf.jacobi_kernel),
+                       "sweeps_per_launch": inf.jacobi_sweeps_per_launch, "chunk_z": inf.jacobi_chunk_z,
+                       "prefetch": inf.jacobi_prefetch,
+                       "tuned_us_per_sweep": round(float(inf.jacobi_tuned_us_per_sweep), 3)},
         }
 
 

@@ -92,7 +92,10 @@ typedef struct firex_config {
   int32_t jacobi_chunk_z;        /* FUSED plan: z planes per block; 0 = tuned default */
   int32_t jacobi_kernel;         /* FUSED plan: 0 auto, 1 register-shift kernel (any
                                     grid), 2 register-ring kernel (nx >= 64, ny >= 32) */
-  int32_t reserved[4];
+  int32_t jacobi_autotune;       /* FUSED plan: 1 = time candidate Jacobi plans on the
+                                    device at creation and keep the fastest; fields
+                                    above left at 0 are searched (cached per shape)  */
+  int32_t reserved[3];
 } firex_config;
 
 /* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */
@@ -114,6 +117,12 @@ typedef struct firex_info {
   int32_t species_storage;
   int32_t plan;
   uint64_t device_bytes;        /* bytes this solver holds on the device      */
+  int32_t jacobi_kernel;        /* FUSED plan in use: kernel, sweeps per      */
+  int32_t jacobi_sweeps_per_launch; /* launch, z chunk and prefetch distance   */
+  int32_t jacobi_chunk_z;
+  int32_t jacobi_prefetch;
+  float jacobi_tuned_us_per_sweep; /* autotuner's time per sweep; 0 = untuned */
+  int32_t reserved;
 } firex_info;
 
 typedef struct firex_memory_estimate {

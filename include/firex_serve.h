@@ -35,7 +35,8 @@ typedef struct firex_server_config {
   int32_t plan;                  /* FIREX_PLAN_* for every plume                              */
   int32_t capture_graphs;        /* 1: replay the post-simulate part of a step as a hipGraph */
   int32_t max_inflight_steps;    /* steps issued but not finished per plume (>= 1; 0 = 2)    */
-  int32_t reserved[5];
+  int32_t jacobi_autotune;       /* 1: autotune the Jacobi plan per grid shape (cached)      */
+  int32_t reserved[4];
 } firex_server_config;
 
 typedef struct firex_request {

@@ -96,3 +96,39 @@ def test_null_arguments_are_errors_not_crashes():
     lib.firex_destroy(None)
     out = ctypes.c_double()
     assert lib.firex_clamp_parameter(None, 1.0, 1.0, ctypes.byref(out)) == fs.ERR_INVALID
+
+
+# ctypes mirror of each C struct: (ctypes class, C struct name)
+def _struct_pairs():
+    from fire_amd import serve as sv
+    return [
+        (fs.Config, "firex_config"), (fs.Emitter, "firex_emitter"), (fs.Info, "firex_info"),
+        (fs.MemoryEstimate, "firex_memory_estimate"), (fs.DeviceLimits, "firex_device_limits"),
+        (fs.PassTiming, "firex_pass_timing"), (sv.ServerConfig, "firex_server_config"),
+        (sv.Request, "firex_request"), (sv.RequestStatus, "firex_request_status"),
+        (sv.ServerStats, "firex_server_stats"),
+    ]
+
+
+def test_python_structs_match_the_c_layout(tmp_path):
+    """Every ctypes mirror has the C struct's size and field offsets (gcc
+    compiles the headers and prints sizeof / offsetof)."""
+    include = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "firex.h"', '#include "firex_serve.h"',
+             "int main(void) {"]
+    expect = []
+    for cls, cname in _struct_pairs():
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        expect.append((f"sizeof {cname}", ctypes.sizeof(cls)))
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append((f"{cname}.{fname}", getattr(cls, fname).offset))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", include, str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert len(got) == len(expect)
+    bad = [(name, c, py) for (name, py), c in zip(expect, got) if c != py]
+    assert not bad, f"C / ctypes layout mismatches: {bad}"

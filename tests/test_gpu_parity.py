@@ -254,3 +254,26 @@ def test_ring_kernel_f16_pressure_bitwise():
     assert_fields_equal(g, o, "ring f16 pressure", names=["flow", "species", "divergence", "reaction"])
     np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
     g.close()
+
+
+def test_autotuned_plan_bitwise_and_reported():
+    """The autotuner picks some ring plan on a ring-sized grid; whatever it
+    picks, the step stays bit-identical to the oracle, and a second solver of
+    the same shape reuses the cached choice."""
+    dims = (96, 64, 40)
+    o = orc.Oracle(dims, precision=32, pressure_iterations=13, correction_iterations=10)
+    o.set_bool("cinematic", True)
+    g = FireXHipSolver(dims, tier="hd", pressure_iterations=13, correction_iterations=10, plan="fused",
+                       jacobi_autotune=True)
+    info = g.info["jacobi"]
+    assert info["kernel"] == "ring" and 1 <= info["sweeps_per_launch"] <= 5 and info["chunk_z"] >= 1
+    assert info["tuned_us_per_sweep"] > 0
+    o.step(DT, 2)
+    g.step(DT, 2)
+    assert_fields_equal(g, o, f"autotuned {info}", names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    g2 = FireXHipSolver(dims, tier="hd", pressure_iterations=13, correction_iterations=10, plan="fused",
+                        jacobi_autotune=True)
+    assert g2.info["jacobi"] == info
+    g.close()
+    g2.close()
