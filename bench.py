@@ -51,12 +51,13 @@ WORKLOADS = {
 }
 
 
-def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False):
+def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False, chunk: int = 0):
     from fire_amd import FireXHipSolver, seeded_emitters
 
     s = FireXHipSolver(wl["dims"], tier="uhd", pressure_iterations=wl["iters"][0],
                        correction_iterations=wl["iters"][1], species_storage=wl["storage"], plan=plan,
-                       heavy_mode="off", jacobi_sweeps_per_launch=sweeps, jacobi_autotune=autotune)
+                       heavy_mode="off", jacobi_sweeps_per_launch=sweeps, jacobi_chunk_z=chunk,
+                       jacobi_autotune=autotune)
     em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
     s.set_emitters(em, jitter=wl["jitter"], velocity_noise=wl["velocity_noise"], seed=wl["seed"])
     return s
@@ -99,6 +100,23 @@ def pass_profile(solver, steps: int, cells: int, storage: str) -> dict:
     return out
 
 
+def measured_traffic(workload: str, plan: dict):
+    """HBM bytes per Jacobi launch for this plan from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
+    None when the plan was not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_jacobi_ring_uhd_mixed_hbm_traffic.json")
+    if workload != "uhd-mixed" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    for p in table["plans"]:
+        if (p["kernel"], p["sweeps_per_launch"], p["chunk_z"], p["prefetch"]) == (
+                plan["kernel"], plan["sweeps_per_launch"], plan["chunk_z"], plan["prefetch"]):
+            return p["hbm_bytes_per_launch"]
+    return None
+
+
 def cpu_baseline(wl: dict, sample_steps: int) -> dict:
     from fire_amd import seeded_emitters
     from oracle import oracle as orc
@@ -124,6 +142,64 @@ def cpu_baseline(wl: dict, sample_steps: int) -> dict:
                       f"(OpenMP over z, {cores} threads), {el:.1f} s"}
 
 
+SERVE_GRIDS = [(64, 96, 64), (128, 192, 128), (160, 240, 160), (192, 288, 192)]
+
+
+def serving_stream(requests: int, rate: float, seed: int) -> dict:
+    """Mixed-size plumes arriving as a seeded Poisson stream, served with
+    continuous batching (admission by HBM budget and slots, captured steps,
+    up to two steps in flight per plume).  Wall-clock arrivals: a request is
+    submitted when its arrival time has passed."""
+    from fire_amd import FireXHipSolver, seeded_emitters
+    from fire_amd.serve import FireXServer, make_request, poisson_stream
+
+    arrivals = poisson_stream(requests, rate, seed, grids=SERVE_GRIDS)
+    # Autotune every grid shape once up front (the plan cache is per
+    # process), so the timed stream does not include tuning.
+    for g in sorted({a[1] for a in arrivals}):
+        FireXHipSolver(g, tier="uhd", pressure_iterations=96, correction_iterations=96, species_storage="f16",
+                       plan="fused", heavy_mode="off", jacobi_autotune=True).close()
+    srv = FireXServer(max_active=16, streams=4, plan="fused", capture_graphs=True, max_inflight_steps=2,
+                      jacobi_autotune=True)
+    ids = []
+    t0 = time.perf_counter()
+    pending = list(arrivals)
+    busy = True
+    while pending or busy:
+        now = time.perf_counter() - t0
+        while pending and pending[0][0] <= now:
+            _, dims, steps, rseed = pending.pop(0)
+            em = seeded_emitters(dims, 3, radius_range=(6.0, 14.0), seed=rseed)
+            ids.append((srv.submit(make_request(dims, steps, emitters=em, jitter=0.1, seed=rseed)), dims, steps))
+        busy = srv.pump()
+        if not busy and pending:
+            time.sleep(max(0.0, min(0.002, pending[0][0] - (time.perf_counter() - t0))))
+    srv.drain()
+    makespan = time.perf_counter() - t0
+    stats = srv.stats()
+    done = [srv.status(i) for i, _, _ in ids]
+    srv.close()
+    turnaround = sorted(d["finish_time"] - d["submit_time"] for d in done)
+    waits = sorted(d["admit_time"] - d["submit_time"] for d in done)
+
+    def pct(v, q):
+        return float(v[min(len(v) - 1, int(q * len(v)))]) if v else 0.0
+
+    return {
+        "requests": len(ids), "arrival_rate_per_s": rate, "seed": seed, "grids": [list(g) for g in SERVE_GRIDS],
+        "steps_range": [60, 600], "jacobi": [96, 96], "species_storage": "f16",
+        "completed": int(stats["done"]), "failed": int(stats["failed"]),
+        "makespan_s": round(makespan, 3), "steps_completed": int(stats["steps_completed"]),
+        "cell_updates_per_s": stats["cell_updates"] / makespan,
+        "step_latency_ms": {"p50": round(stats["latency_ms_p50"], 3), "p99": round(stats["latency_ms_p99"], 3),
+                            "max": round(stats["latency_ms_max"], 3)},
+        "request_turnaround_s": {"p50": round(pct(turnaround, 0.5), 3), "p99": round(pct(turnaround, 0.99), 3)},
+        "admission_wait_s": {"p50": round(pct(waits, 0.5), 3), "p99": round(pct(waits, 0.99), 3)},
+        "server": {"max_active": 16, "streams": 4, "capture_graphs": True, "max_inflight_steps": 2,
+                   "memory_budget_bytes": int(stats["memory_budget_bytes"])},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,12 +208,32 @@ def main() -> int:
     ap.add_argument("--workload", default="uhd-mixed", choices=sorted(WORKLOADS))
     ap.add_argument("--plan", default="fused", choices=["fused", "reference"])
     ap.add_argument("--sweeps", type=int, default=0, help="Jacobi sweeps per launch (0 = autotuned)")
-    ap.add_argument("--no-autotune", action="store_true", help="use the default Jacobi plan instead of tuning")
+    ap.add_argument("--jacobi-chunk", type=int, default=0, help="Jacobi z planes per block (0 = autotuned)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="no Jacobi plan search (fixed --sweeps / --jacobi-chunk or the defaults): for profiles")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES three times (counter calibration)")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=60, help="steps timed on the plain f32 path (0 = skip)")
     ap.add_argument("--cpu-sample-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=16, help="serving-stream sample size (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
     args = ap.parse_args()
+
+    if args.copy_calibration > 0:
+        # A copy kernel of known size: FETCH_SIZE / WRITE_SIZE of this run
+        # scale the same counters of the solver's kernels to bytes.
+        import torch
+
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the Fire-X solver here is a single-GPU build "
@@ -155,7 +251,8 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     t_setup = time.perf_counter()
-    solver = make_solver(wl, args.plan, args.sweeps, autotune=not args.no_autotune and args.plan == "fused")
+    solver = make_solver(wl, args.plan, args.sweeps, autotune=not args.no_autotune and args.plan == "fused",
+                         chunk=args.jacobi_chunk)
     jacobi_plan = solver.info["jacobi"]
     print(f"bench: solver ready in {time.perf_counter() - t_setup:.2f} s, Jacobi plan {jacobi_plan}",
           file=sys.stderr, flush=True)
@@ -174,7 +271,7 @@ def main() -> int:
     bytes_per_launch = BYTES_PER_CELL["jacobi_launch"] * cells
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(args.workload, jacobi_plan),
                 "kernel": f"k_jacobi_{jacobi_plan['kernel']} (Jacobi, {sweeps_per_launch:g} sweeps per launch, "
                           f"z chunk {jacobi_plan['chunk_z']})",
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
@@ -189,6 +286,12 @@ def main() -> int:
         ps.close()
         plain = {"cell_updates_per_s": cells * args.plain_steps / pel, "ms_per_step": pel / args.plain_steps * 1e3,
                  "steps": args.plain_steps}
+
+    serving = None
+    if args.serve_requests > 0:
+        print(f"bench: serving stream ({args.serve_requests} requests at {args.serve_rate}/s)", file=sys.stderr,
+              flush=True)
+        serving = serving_stream(args.serve_requests, args.serve_rate, seed=2026)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -217,6 +320,7 @@ def main() -> int:
         "passes": passes,
         "plain_f32_gpu": plain,
         "multiple_over_plain_f32": (value / plain["cell_updates_per_s"]) if plain else None,
+        "serving": serving,
         "check": {"sum_fuel_after_run": t_fuel},
     }
     print(json.dumps(line), flush=True)

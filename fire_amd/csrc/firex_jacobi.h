@@ -37,12 +37,6 @@ struct JacobiLaunch {
 // Blocks one launch of the plan runs on a grid of dims.
 int jacobi_plan_blocks(const JacobiPlan& plan, const int32_t dims[3]);
 
-// Splits `sweeps` sweeps starting at buffers[start] into launches; returns
-// the number of launches written to out (at most cap) and stores the index
-// of the buffer holding the result in *result.
-int jacobi_schedule(const JacobiPlan& plan, int start, bool zero_start, int sweeps, JacobiLaunch* out, int cap,
-                    int* result);
-
 // f16 selects f16 pressure storage: every sweep's result is rounded to f16,
 // as pressureShaders.ts:116 stores it.
 void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, void* const buffers[2],

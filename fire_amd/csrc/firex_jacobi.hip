@@ -618,23 +618,6 @@ JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, in
   return p;
 }
 
-int jacobi_schedule(const JacobiPlan& plan, int start, bool zero_start, int sweeps, JacobiLaunch* out, int cap,
-                    int* result) {
-  int cur = start, n = 0;
-  bool zero = zero_start;
-  int left = sweeps;
-  while (left > 0) {
-    const int k = left < plan.sweeps_per_launch ? left : plan.sweeps_per_launch;
-    if (n < cap) out[n] = JacobiLaunch{k, cur, zero};
-    ++n;
-    cur = 1 - cur;
-    zero = false;
-    left -= k;
-  }
-  *result = cur;
-  return n;
-}
-
 void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, void* const buffers[2],
                       const float* div, const FxStep& st, hipStream_t s) {
   const void* pin = buffers[l.src];

@@ -160,8 +160,12 @@ int collect_timing(firex_solver* s) {
   return FIREX_OK;
 }
 
+// Runs launch() (which enqueues `launches` kernels back to back) between one
+// pair of timing events when pass timing is on.  A group of consecutive
+// launches shares one pair: events between kernels would add their own
+// gaps to what is measured.
 template <typename F>
-void timed(firex_solver* s, int pass, F&& launch) {
+void timed(firex_solver* s, int pass, F&& launch, int launches = 1) {
   if (!s->timing) {
     launch();
     return;
@@ -175,7 +179,7 @@ void timed(firex_solver* s, int pass, F&& launch) {
   launch();
   hipEventRecord(b, s->stream);
   s->pending.push_back({pass, a, b});
-  s->totals.launches[pass] += 1;
+  s->totals.launches[pass] += launches;
 }
 
 float4* flow0(firex_solver* s) { return s->flow[s->flow_a]; }
@@ -186,18 +190,34 @@ int sweeps(firex_solver* s, int f16, void* const bufs[2], int start, bool zero_s
            bool tb) {
   if (!tb) {
     int cur = start;
-    for (int i = 0; i < count; ++i) {
-      timed(s, FIREX_PASS_JACOBI, [&] { launch_jacobi(f16, bufs[cur], bufs[1 - cur], s->divergence, st, s->stream); });
-      cur = 1 - cur;
-    }
+    timed(
+        s, FIREX_PASS_JACOBI,
+        [&] {
+          for (int i = 0; i < count; ++i) {
+            launch_jacobi(f16, bufs[cur], bufs[1 - cur], s->divergence, st, s->stream);
+            cur = 1 - cur;
+          }
+        },
+        count);
     s->totals.jacobi_sweeps += s->timing ? count : 0;
     return cur;
   }
-  JacobiLaunch launches[1024];
+  // count sweeps as launches of up to K sweeps, alternating buffers; only
+  // the first reads the zero field instead of bufs[start] when zero_start.
   int result = start;
-  int n = jacobi_schedule(s->jacobi, start, zero_start, count, launches, 1024, &result);
-  for (int i = 0; i < n && i < 1024; ++i)
-    timed(s, FIREX_PASS_JACOBI, [&] { launch_jacobi_tb(s->jacobi, f16, launches[i], bufs, s->divergence, st, s->stream); });
+  timed(
+      s, FIREX_PASS_JACOBI,
+      [&] {
+        bool zero = zero_start;
+        for (int left = count; left > 0;) {
+          const int k = left < s->jacobi.sweeps_per_launch ? left : s->jacobi.sweeps_per_launch;
+          launch_jacobi_tb(s->jacobi, f16, JacobiLaunch{k, result, zero}, bufs, s->divergence, st, s->stream);
+          result = 1 - result;
+          zero = false;
+          left -= k;
+        }
+      },
+      (count + s->jacobi.sweeps_per_launch - 1) / s->jacobi.sweeps_per_launch);
   s->totals.jacobi_sweeps += s->timing ? count : 0;
   return result;
 }

@@ -277,3 +277,15 @@ def test_autotuned_plan_bitwise_and_reported():
     assert g2.info["jacobi"] == info
     g.close()
     g2.close()
+
+
+def test_more_than_1024_jacobi_launches_bitwise():
+    """1100 one-sweep launches in one solve (a schedule longer than any fixed
+    launch table) still run every sweep."""
+    dims = (16, 24, 16)
+    g, o = make_pair(dims, iters=(1100, 3), plan="fused", sweeps=1)
+    o.step(DT, 1)
+    g.step(DT, 1)
+    assert_fields_equal(g, o, "1100 sweeps", names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    g.close()
