@@ -38,7 +38,7 @@ DT = 1.0 / 60.0
 # and writes p once per cell whatever its number of sweeps: 12 B (f32 p).
 BYTES_PER_CELL = {
     "simulate_f32": 68, "simulate_f16": 52, "divergence": 20, "jacobi_launch": 12,
-    "project": 36, "project_divergence": 40, "fill_copy": 32,
+    "project": 36, "fill_copy": 32,
 }
 
 WORKLOADS = {
@@ -48,15 +48,19 @@ WORKLOADS = {
                      seed=2026, jitter=0.1, velocity_noise=0.0),
     "cpu-plain": dict(dims=(64, 96, 64), iters=(24, 24), storage="f32", emitters=1, radius=(4 * 2 ** 0.5,) * 2,
                       seed=2026, jitter=0.1, velocity_noise=0.0),
+    # 131.1 M cells: the >= 64 Mi cell path (f16 pressure), bf16 scalars,
+    # sigma 32 cells, +-0.5 seeded noise on the injected velocity
+    "plume-508": dict(dims=(508, 508, 508), iters=(96, 96), storage="bf16", emitters=1,
+                      radius=(32 * 2 ** 0.5,) * 2, seed=2026, jitter=0.1, velocity_noise=0.5),
 }
 
 
-def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False, chunk: int = 0):
+def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False, chunk: int = 0, heavy: str = "off"):
     from fire_amd import FireXHipSolver, seeded_emitters
 
     s = FireXHipSolver(wl["dims"], tier="uhd", pressure_iterations=wl["iters"][0],
                        correction_iterations=wl["iters"][1], species_storage=wl["storage"], plan=plan,
-                       heavy_mode="off", jacobi_sweeps_per_launch=sweeps, jacobi_chunk_z=chunk,
+                       heavy_mode=heavy, jacobi_sweeps_per_launch=sweeps, jacobi_chunk_z=chunk,
                        jacobi_autotune=autotune)
     em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
     s.set_emitters(em, jitter=wl["jitter"], velocity_noise=wl["velocity_noise"], seed=wl["seed"])
@@ -72,13 +76,15 @@ def time_steps(solver, steps: int, warmup: int) -> float:
     return time.perf_counter() - t0
 
 
-def pass_profile(solver, steps: int, cells: int, storage: str) -> dict:
+def pass_profile(solver, steps: int, cells: int, storage: str, f16_pressure: bool = False) -> dict:
+    # f16 pressure (the >= 64 Mi cell path) moves 2 B instead of 4 per p access
+    pressure_bytes = {"jacobi_launch": 8, "project": 34} if f16_pressure else {}
     solver.set_pass_timing(True)
     solver.step(DT, steps)
     t = solver.pass_timing()
     solver.set_pass_timing(False)
     out = {}
-    for name in ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy"):
+    for name in ("simulate", "divergence", "jacobi", "project", "fill_copy"):
         ms, n = t[name]["ms"], t[name]["launches"]
         if n == 0:
             continue
@@ -86,15 +92,15 @@ def pass_profile(solver, steps: int, cells: int, storage: str) -> dict:
             bpc = BYTES_PER_CELL["simulate_f16" if storage != "f32" else "simulate_f32"]
             units = n
         else:
-            bpc = BYTES_PER_CELL["jacobi_launch" if name == "jacobi" else name]
+            key = "jacobi_launch" if name == "jacobi" else name
+            bpc = pressure_bytes.get(key, BYTES_PER_CELL[key])
             units = n
         gbs = bpc * cells * units / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         out[name] = {"ms_per_step": ms / t["steps"], "launches_per_step": n / t["steps"],
                      "avg_launch_us": ms / n * 1e3, "algorithmic_GBps": round(gbs, 1)}
         if name == "jacobi" and ms > 0:
             # what one-sweep-per-launch kernels would have to stream to match
-            out[name]["sweep_equivalent_GBps"] = round(
-                BYTES_PER_CELL["jacobi_launch"] * cells * t["jacobi_sweeps"] / (ms * 1e-3) / 1e9, 1)
+            out[name]["sweep_equivalent_GBps"] = round(bpc * cells * t["jacobi_sweeps"] / (ms * 1e-3) / 1e9, 1)
     out["jacobi_sweeps_per_step"] = t["jacobi_sweeps"] / max(1, t["steps"])
     out["_raw"] = t
     return out
@@ -118,10 +124,15 @@ def measured_traffic(workload: str, plan: dict):
 
 
 def cpu_baseline(wl: dict, sample_steps: int) -> dict:
+    """The FP32 oracle (the reference step restated on the CPU, same passes
+    and order) on the host: per-step median over sample_steps steps after two
+    warm-up steps on all OpenMP threads, then one step on a single thread."""
+    import statistics
+
     from fire_amd import seeded_emitters
     from oracle import oracle as orc
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = orc.max_threads()
     o = orc.Oracle(wl["dims"], precision=32, pressure_iterations=wl["iters"][0],
                    correction_iterations=wl["iters"][1])
     o.set_bool("cinematic", True)
@@ -131,15 +142,25 @@ def cpu_baseline(wl: dict, sample_steps: int) -> dict:
         o.reset()
     em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
     o.set_emitters(em, wl["jitter"], wl["velocity_noise"], wl["seed"])
-    o.step(DT, 1)  # warm-up (page-in)
+    o.step(DT, 2)  # warm-up (page-in)
+    times = []
+    for _ in range(sample_steps):
+        t0 = time.perf_counter()
+        o.step(DT, 1)
+        times.append(time.perf_counter() - t0)
+    orc.set_threads(1)
     t0 = time.perf_counter()
-    o.step(DT, sample_steps)
-    el = time.perf_counter() - t0
+    o.step(DT, 1)
+    single = time.perf_counter() - t0
+    orc.set_threads(cores)
     cells = wl["dims"][0] * wl["dims"][1] * wl["dims"][2]
     o.close()
-    return {"value": cells * sample_steps / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{sample_steps} steps of the same workload after 1 warm-up step, FP32 CPU oracle "
-                      f"(OpenMP over z, {cores} threads), {el:.1f} s"}
+    med = statistics.median(times)
+    return {"value": cells / med, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"median of {sample_steps} steps of the same workload after 2 warm-up steps, FP32 CPU oracle "
+                      f"(OpenMP over z, {cores} threads; {med:.2f} s per step)",
+            "single_thread": {"value": cells / single, "unit": "cell-updates/s", "cores": 1,
+                              "sample": f"1 step after the sample above, 1 thread, {single:.1f} s"}}
 
 
 SERVE_GRIDS = [(64, 96, 64), (128, 192, 128), (160, 240, 160), (192, 288, 192)]
@@ -215,7 +236,9 @@ def main() -> int:
                     help="only run a plain device-to-device copy of BYTES three times (counter calibration)")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=60, help="steps timed on the plain f32 path (0 = skip)")
-    ap.add_argument("--cpu-sample-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample-steps", type=int, default=5)
+    ap.add_argument("--large-steps", type=int, default=10,
+                    help="steps of the 508^3 plume for its per-pass HBM GB/s (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16, help="serving-stream sample size (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
@@ -287,6 +310,24 @@ def main() -> int:
         plain = {"cell_updates_per_s": cells * args.plain_steps / pel, "ms_per_step": pel / args.plain_steps * 1e3,
                  "steps": args.plain_steps}
 
+    large = None
+    if args.large_steps > 0 and args.workload == "uhd-mixed":
+        lw = WORKLOADS["plume-508"]
+        lcells = lw["dims"][0] * lw["dims"][1] * lw["dims"][2]
+        print(f"bench: 508^3 plume ({lcells} cells), {args.large_steps} steps", file=sys.stderr, flush=True)
+        ls = make_solver(lw, "fused", autotune=True, heavy="auto")
+        linfo = ls.info
+        lel = time_steps(ls, args.large_steps, 2)
+        lpasses = pass_profile(ls, 2, lcells, lw["storage"], f16_pressure=linfo["f16_pressure"])
+        lpasses.pop("_raw")
+        ls.close()
+        large = {"grid": list(lw["dims"]), "cells": lcells, "steps": args.large_steps, "jacobi": list(lw["iters"]),
+                 "species_storage": lw["storage"], "pressure": "f16" if linfo["f16_pressure"] else "f32",
+                 "velocity_noise": lw["velocity_noise"], "ms_per_step": lel / args.large_steps * 1e3,
+                 "cell_updates_per_s": lcells * args.large_steps / lel, "jacobi_plan": linfo["jacobi"],
+                 "passes": {k: dict(v, frac_of_peak=round(v["algorithmic_GBps"] / PEAK_HBM_GBS, 4))
+                            if isinstance(v, dict) else v for k, v in lpasses.items()}}
+
     serving = None
     if args.serve_requests > 0:
         print(f"bench: serving stream ({args.serve_requests} requests at {args.serve_rate}/s)", file=sys.stderr,
@@ -321,6 +362,7 @@ def main() -> int:
         "plain_f32_gpu": plain,
         "multiple_over_plain_f32": (value / plain["cell_updates_per_s"]) if plain else None,
         "serving": serving,
+        "plume_508": large,
         "check": {"sum_fuel_after_run": t_fuel},
     }
     print(json.dumps(line), flush=True)

@@ -1,5 +1,4 @@
-// firex_jacobi.h — temporally blocked Jacobi sweeps and the fused
-// projection + divergence pass.
+// firex_jacobi.h — temporally blocked Jacobi sweeps and their autotuner.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,11 +40,6 @@ int jacobi_plan_blocks(const JacobiPlan& plan, const int32_t dims[3]);
 // as pressureShaders.ts:116 stores it.
 void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, void* const buffers[2],
                       const float* div, const FxStep& st, hipStream_t s);
-
-// projectVelocity (shaders.ts:470-501) fused with computeDivergence of its
-// result (shaders.ts:344-364): writes fout and div.
-void launch_project_divergence(int f16, const float4* fin, const void* p, float4* fout, float* div, const FxStep& st,
-                               hipStream_t s);
 
 // Host-side check of a plan against the kernel's assumptions.
 bool jacobi_plan_valid(const JacobiPlan& plan);

@@ -1,5 +1,5 @@
 // firex_jacobi.hip — temporally blocked Jacobi pressure sweeps for gfx950 and
-// the projection pass fused with the divergence of its result.
+// their autotuner.
 //
 // Jacobi sweep (solvePressureGlobal, src/sim/firex/shaders.ts:375-403):
 //   p'(c) = (wx (p[x-1] + p[x+1]) + wy (p[y-1] + p[y+1]) + wz (p[z-1] + p[z+1]) - div(c))
@@ -512,69 +512,6 @@ void launch_tb_k(int k, const JacobiPlan& plan, const void* pin, void* pout, con
 }
 
 // ----------------------------------------------------- projection + divergence
-template <int P>
-__device__ inline float lp(const typename PIO<P>::Word* p, const FxStep& st, int x, int y, int z) {
-  if (y >= st.ny) return 0.0f;
-  x = min(max(x, 0), st.nx - 1);
-  y = max(y, 0);
-  z = min(max(z, 0), st.nz - 1);
-  return PIO<P>::load(p, (uint32_t)x + (uint32_t)st.nx * ((uint32_t)y + (uint32_t)st.ny * (uint32_t)z));
-}
-
-__device__ inline uint32_t ci(const FxStep& st, int x, int y, int z) {
-  x = min(max(x, 0), st.nx - 1);
-  y = min(max(y, 0), st.ny - 1);
-  z = min(max(z, 0), st.nz - 1);
-  return (uint32_t)x + (uint32_t)st.nx * ((uint32_t)y + (uint32_t)st.ny * (uint32_t)z);
-}
-
-// Component `axis` of projectVelocity's output at cell (x, y, z) (in range).
-template <int P, int AXIS>
-__device__ inline float projected(const float4* fin, const typename PIO<P>::Word* p, const FxStep& st, int x, int y,
-                                  int z) {
-  const bool top = y == st.ny - 1;
-  const int sy = top ? y - 1 : y;  // the top cell copies the cell below
-  const float c = lp<P>(p, st, x, sy, z);
-  const float4 f = fin[ci(st, x, sy, z)];
-  float v;
-  if (AXIS == 0) {
-    v = f.x - (lp<P>(p, st, x + 1, sy, z) - c) * st.mx;
-    if (x == 0 || x == st.nx - 1) v = 0.0f;
-  } else if (AXIS == 1) {
-    v = f.y - (lp<P>(p, st, x, sy + 1, z) - c) * st.my;
-    if (y == 0) v = 0.0f;
-    if (top) v = fxm_max(0.0f, v);
-  } else {
-    v = f.z - (lp<P>(p, st, x, sy, z + 1) - c) * st.mz;
-    if (z == 0 || z == st.nz - 1) v = 0.0f;
-  }
-  return v;
-}
-
-template <int P>
-__global__ __launch_bounds__(256) void k_project_divergence(const float4* __restrict__ fin,
-                                                            const typename PIO<P>::Word* __restrict__ p,
-                                                            float4* __restrict__ fout, float* __restrict__ div,
-                                                            FxStep st) {
-  const uint32_t bx = (uint32_t)(st.nx + 63) / 64, by = (uint32_t)(st.ny + 3) / 4;
-  const uint32_t total = bx * by * (uint32_t)st.nz;
-  const uint32_t b = xcd_remap(blockIdx.x, total);
-  const int x = (int)((b % bx) * 64 + threadIdx.x);
-  const uint32_t rest = b / bx;
-  const int y = (int)((rest % by) * 4 + threadIdx.y);
-  const int z = (int)(rest / by);
-  if (x >= st.nx || y >= st.ny) return;
-  const uint32_t id = ci(st, x, y, z);
-  const float ux = projected<P, 0>(fin, p, st, x, y, z);
-  const float uy = projected<P, 1>(fin, p, st, x, y, z);
-  const float uz = projected<P, 2>(fin, p, st, x, y, z);
-  fout[id] = make_float4(ux, uy, uz, fin[id].w);
-  const float left = x > 0 ? projected<P, 0>(fin, p, st, x - 1, y, z) : ux;
-  const float bottom = y > 0 ? projected<P, 1>(fin, p, st, x, y - 1, z) : uy;
-  const float back = z > 0 ? projected<P, 2>(fin, p, st, x, y, z - 1) : uz;
-  div[id] = (ux - left) * st.mx + (uy - bottom) * st.my + (uz - back) * st.mz;
-}
-
 }  // namespace
 
 bool jacobi_plan_valid(const JacobiPlan& plan) {
@@ -773,13 +710,5 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
   return hipSuccess;
 }
 
-void launch_project_divergence(int f16, const float4* fin, const void* p, float4* fout, float* div, const FxStep& st,
-                               hipStream_t s) {
-  const uint32_t blocks = (uint32_t)((st.nx + 63) / 64) * (uint32_t)((st.ny + 3) / 4) * (uint32_t)st.nz;
-  if (f16)
-    k_project_divergence<1><<<blocks, dim3(64, 4), 0, s>>>(fin, (const _Float16*)p, fout, div, st);
-  else
-    k_project_divergence<0><<<blocks, dim3(64, 4), 0, s>>>(fin, (const float*)p, fout, div, st);
-}
 
 }  // namespace fx

@@ -263,13 +263,14 @@ void encode_rest(firex_solver* s, const FxStep& st) {
     timed(s, FIREX_PASS_PROJECT, [&] { launch_project(0, flow0(s), s->pressure[s->corr_index], flow1(s), st, q); });
     timed(s, FIREX_PASS_FILL_COPY, [&] { launch_copy_u128(flow0(s), flow1(s), s->cells, q); });
   } else if (fused) {
-    // The same passes, merged: temporally blocked sweeps, projection fused
-    // with the residual divergence, the residual solve started from its
-    // known zero field, and copyFlow replaced by swapping the flow buffers.
+    // The same passes, merged: temporally blocked sweeps, the residual
+    // solve started from its known zero field, and copyFlow replaced by
+    // swapping the flow buffers.  (Projection and the residual divergence
+    // stay two kernels: a fused kernel must recompute the neighbours'
+    // projected velocity and measured slower than the pair.)
     s->pressure_index = sweeps(s, f16, s->pressure, s->pressure_index, false, N, st, true);
-    timed(s, FIREX_PASS_PROJECT_DIVERGENCE, [&] {
-      launch_project_divergence(f16, flow1(s), s->pressure[s->pressure_index], flow0(s), s->divergence, st, q);
-    });
+    timed(s, FIREX_PASS_PROJECT, [&] { launch_project(f16, flow1(s), s->pressure[s->pressure_index], flow0(s), st, q); });
+    timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow0(s), s->divergence, st, q); });
     if (M == 0) {
       fill(s, s->corr[0], pwords);
       fill(s, s->corr[1], pwords);

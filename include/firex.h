@@ -251,7 +251,7 @@ enum {
   FIREX_PASS_DIVERGENCE = 1,
   FIREX_PASS_JACOBI = 2,              /* one entry per Jacobi launch              */
   FIREX_PASS_PROJECT = 3,
-  FIREX_PASS_PROJECT_DIVERGENCE = 4,  /* fused projection + residual divergence   */
+  FIREX_PASS_PROJECT_DIVERGENCE = 4,  /* no longer launched (kept for the layout) */
   FIREX_PASS_FILL_COPY = 5,           /* clears and copyFlow                      */
   FIREX_PASS_COUNT = 8
 };

@@ -42,6 +42,8 @@
 #include <vector>
 #include <algorithm>
 
+#include <omp.h>
+
 #include "../fire_amd/csrc/fx_math.h"
 
 namespace {
@@ -953,5 +955,9 @@ void orc_stencil_weights(void* p, double dt, double* out3) {
   out3[1] = h->d.my * h->d.my;
   out3[2] = h->d.mz * h->d.mz;
 }
+
+// Threads of the OpenMP loops (the CPU baseline times 1 thread and all cores).
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

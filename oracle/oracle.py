@@ -72,6 +72,8 @@ def load() -> ctypes.CDLL:
         "orc_f32_exp2": (f, [f]),
         "orc_f32_eval": (None, [i, fp, fp, fp, ctypes.c_int64]),
         "orc_stencil_weights": (None, [vp, d, dp]),
+        "orc_set_threads": (None, [i]),
+        "orc_max_threads": (i, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -79,6 +81,19 @@ def load() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def max_threads() -> int:
+    """OpenMP threads the oracle's loops use (OMP_NUM_THREADS or the host's cores)."""
+    return load().orc_max_threads()
+
+
+def set_threads(n: int) -> int:
+    """Set the OpenMP thread count of the oracle's loops; returns the previous one."""
+    lib = load()
+    prev = lib.orc_max_threads()
+    lib.orc_set_threads(int(n))
+    return prev
 
 
 F32_FUNCS = {"exp": 0, "exp2": 1, "log2": 2, "sin": 3, "cos": 4, "pow": 5, "smoothstep": 6}
