@@ -23,6 +23,10 @@ export interface FireXHipSolverOptions {
   device?: number;
   memoryBudgetBytes?: number;
   jacobiSweepsPerLaunch?: number;
+  /** 1: time candidate Jacobi plans on the device at creation and keep the fastest (cached per shape). */
+  jacobiAutotune?: number;
+  /** 1: replay everything after simulate as a captured hipGraph. */
+  captureGraphs?: number;
   scene?: Readonly<Record<string, number | boolean>>;
 }
 

@@ -123,7 +123,8 @@ static int str_prop(napi_env env, napi_value obj, const char* key, char* buf, si
 
 /* create({tier, dims:[x,y,z], pressureIterations, correctionIterations,
  *         timeStep, maximumSubsteps, speciesStorage, heavyMode, plan,
- *         cinematic, device, memoryBudgetBytes, jacobiSweepsPerLaunch}) */
+ *         cinematic, device, memoryBudgetBytes, jacobiSweepsPerLaunch,
+ *         captureGraphs, jacobiAutotune}) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -158,6 +159,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.memory_budget_bytes = (uint64_t)num_or(env, argv[0], "memoryBudgetBytes", 0);
     cfg.jacobi_sweeps_per_launch = (int32_t)num_or(env, argv[0], "jacobiSweepsPerLaunch", 0);
     cfg.capture_graphs = (int32_t)num_or(env, argv[0], "captureGraphs", 0);
+    cfg.jacobi_autotune = (int32_t)num_or(env, argv[0], "jacobiAutotune", 0);
     if (str_prop(env, argv[0], "speciesStorage", buf, sizeof buf)) cfg.species_storage = lookup(buf, storage_names, 3);
     if (str_prop(env, argv[0], "heavyMode", buf, sizeof buf)) cfg.heavy_mode = lookup(buf, heavy_names, 4);
     if (str_prop(env, argv[0], "plan", buf, sizeof buf)) cfg.plan = lookup(buf, plan_names, 2);
