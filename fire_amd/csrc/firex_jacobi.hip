@@ -576,6 +576,19 @@ void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, vo
 
 namespace {
 
+// Tuning data: a divergence field of hashed values in [-0.3, 0.36) (f32
+// patterns that switch like a real field; an all-zero field runs faster
+// than real data and ranks plans differently).
+__global__ __launch_bounds__(256) void k_tune_pattern(float* __restrict__ div, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) {
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    div[i] = (float)(h >> 16) * 1e-5f - 0.3f;
+  }
+}
+
 using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;  // device, dims, f16, K, chunk, kernel
 struct TuneEntry {
   JacobiPlan plan;
@@ -657,7 +670,8 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
   const size_t pbytes = (cells * (f16 ? 2 : 4) + 3) / 4 * 4;
   if ((e = hipMemsetAsync(buffers[0], 0, pbytes, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(buffers[1], 0, pbytes, s)) != hipSuccess) return e;
-  if ((e = hipMemsetAsync(div, 0, cells * 4, s)) != hipSuccess) return e;
+  k_tune_pattern<<<2048, 256, 0, s>>>(div, (uint64_t)cells);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   hipEvent_t ev[2];
   if ((e = hipEventCreate(&ev[0])) != hipSuccess) return e;
   if ((e = hipEventCreate(&ev[1])) != hipSuccess) {
@@ -677,14 +691,37 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
     *us = ms * 1000.0f / (float)(reps * p.sweeps_per_launch);
     return err;
   };
-  // Warm the clocks, then a quick pass over all candidates and a longer
-  // one over the best few.
+  // Warm the clocks, then a quick pass over all candidates, chunk sizes
+  // within 15 % of the two best (time is not smooth in the chunk size: the
+  // block count steps), and a longer pass over the best few.
   std::vector<std::pair<float, int>> score;
   float us = 0;
   e = time_plan(cands[0], 12, &us);
   for (size_t i = 0; i < cands.size() && e == hipSuccess; ++i) {
     e = time_plan(cands[i], 3, &us);
     score.emplace_back(us, (int)i);
+  }
+  if (e == hipSuccess && chunk_z <= 0) {
+    std::sort(score.begin(), score.end());
+    const size_t first = cands.size();
+    for (size_t j = 0; j < std::min<size_t>(score.size(), 2); ++j) {
+      const JacobiPlan b = cands[score[j].second];
+      const int step = std::max(1, b.chunk_z / 16);
+      for (int c = std::max(1, b.chunk_z * 85 / 100); c <= std::min(dims[2], b.chunk_z * 115 / 100); c += step) {
+        bool seen = false;
+        for (const JacobiPlan& q : cands)
+          seen = seen || (q.kernel == b.kernel && q.sweeps_per_launch == b.sweeps_per_launch &&
+                          q.prefetch == b.prefetch && q.chunk_z == c);
+        if (seen) continue;
+        JacobiPlan p = b;
+        p.chunk_z = c;
+        cands.push_back(p);
+      }
+    }
+    for (size_t i = first; i < cands.size() && e == hipSuccess; ++i) {
+      e = time_plan(cands[i], 3, &us);
+      score.emplace_back(us, (int)i);
+    }
   }
   if (e == hipSuccess) {
     std::sort(score.begin(), score.end());
