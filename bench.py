@@ -252,10 +252,16 @@ def main() -> int:
         n = args.copy_calibration // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        dst.copy_(src)
+        a.record()
         for _ in range(3):
             dst.copy_(src)
+        b.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}), flush=True)
+        ms = a.elapsed_time(b) / 3
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 4, "copy_ms": ms,
+                          "copy_GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -275,7 +275,7 @@ uint64_t device_bytes(const int32_t dims[3], int32_t species_storage, bool f16_p
   uint64_t bytes = cells * (2 * 16 + 2 * species + 2 * pressure + 4 /* div */ + 4 /* reaction */);
   if (!compact_f32) bytes += cells * 2 * pressure;  // residual pair
   if (with_liquid) bytes += cells * 4;
-  return bytes + (uint64_t)kReduceScratchBytes;
+  return bytes + (uint64_t)kReduceScratchBytes + fx_sim_table_floats(dims) * 4;
 }
 
 }  // namespace fx

@@ -16,8 +16,12 @@ struct SimulateArgs {
   void* species_out;
   float* reaction;
   const uint32_t* liquid;  // may be null when st.has_liquid == 0
+  float* tables;           // sim_table_bytes(dims) of scratch: per-step coordinate tables
   FxStep st;
 };
+
+// Bytes of the per-step coordinate tables simulate needs for a grid.
+size_t sim_table_bytes(const int32_t dims[3]);
 
 void launch_initialize(int storage, float4* flow, void* species, uint32_t cells, float ambient, float oxygen,
                        hipStream_t s);

@@ -131,18 +131,97 @@ __device__ inline float4 trilinear(const FxStep& st, float px, float py, float p
   return mix4(z0, z1, fz);
 }
 
-// emitter(), shaders.ts:101-133, with its uniform-only terms from FxStep.
-__device__ inline float emitter_reference(const FxStep& st, float npx, float npy, float npz) {
-  float px = npx * st.ex, py = npy * st.ey, pz = npz * st.ez;
-  float ox = px - st.em_cx, oy = py - st.em_cy, oz = pz - st.em_cz;
+// ------------------------------------------------------- coordinate tables
+// Every term of simulate that depends on at most two cell coordinates is
+// evaluated once per step by k_sim_tables, with exactly the f32 operations
+// the per-cell code would use (so the results are the same bits), and read
+// back per cell: the turbulence sines and cosines, the height envelope, the
+// outflow terms, the swirl, and the reference emitter's horizontal factors.
+// Layout (floats, SimTab offsets): xz4 [nx*nz] float4 (emitter radial,
+// texture, pulse, 0); y4 [ny] float4 (height envelope, 1 - outflow, outflow
+// decay, outflow mix); x [nx] swz; yv [ny] emitter vertical; z [nz] swx;
+// yz [ny*nz] ltx; xz [nx*nz] lty; xy [nx*ny] ltz.
+struct SimTab {
+  uint32_t xz4, y4, x, yv, z, yz, xz, xy, total;
+  __host__ __device__ SimTab(int nx, int ny, int nz) {
+    const uint32_t a = (uint32_t)nx, b = (uint32_t)ny, c = (uint32_t)nz;
+    xz4 = 0;
+    y4 = 4 * a * c;
+    x = y4 + 4 * b;
+    yv = x + a;
+    z = yv + b;
+    yz = z + c;
+    xz = yz + b * c;
+    xy = xz + a * c;
+    total = xy + a * b;
+  }
+};
+
+// normalized cell centres, (i + 0.5) / grid (shaders.ts:153-156)
+__device__ inline float np_of(int i, float g) { return ((float)i + 0.5f) / g; }
+
+// emitter(), shaders.ts:101-133, split into its (x, z) factors and its y
+// factor; the per-cell value is exp(-(radial + vertical)) * texture * pulse.
+__device__ inline float4 emitter_xz(const FxStep& st, float npx, float npz) {
+  float px = npx * st.ex, pz = npz * st.ez;
+  float ox = px - st.em_cx, oz = pz - st.em_cz;
   float radial = ox * ox / st.em_rx2 + oz * oz / st.em_rz2;
-  float vertical = oy * oy / st.em_vdenom;
   float cellular = 0.76f + 0.24f * fxm_sin(px * 49.0f + st.et52) * fxm_sin(pz * 43.0f - st.et44);
   float lobeArg = fxm_sin(ox / st.em_rx_safe * 6.2f + st.et17) * fxm_sin(oz / st.em_rz_safe * 5.4f - st.et13);
   float lobes = fxm_mix(0.48f, 1.18f, fxm_smoothstep(-0.42f, 0.48f, lobeArg));
   float texture = fxm_mix(cellular, cellular * lobes, st.em_cinematic);
   float pulse = 0.84f + 0.16f * fxm_sin(st.et37 + px * 27.0f + pz * 21.0f);
-  return fxm_exp(-(radial + vertical)) * texture * pulse;
+  return make_float4(radial, texture, pulse, 0.0f);
+}
+__device__ inline float emitter_vertical(const FxStep& st, float npy) {
+  float oy = npy * st.ey - st.em_cy;
+  return oy * oy / st.em_vdenom;
+}
+
+__global__ __launch_bounds__(256) void k_sim_tables(float* __restrict__ tab, FxStep st, int reference_emitter) {
+  const int nx = st.nx, ny = st.ny, nz = st.nz;
+  const SimTab o(nx, ny, nz);
+  const uint32_t n_xz = (uint32_t)nx * nz, n_xy = (uint32_t)nx * ny, n_yz = (uint32_t)ny * nz;
+  const uint32_t total = n_xz + n_xy + n_yz + (uint32_t)(nx + ny + nz);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < total; i += gridDim.x * 256u) {
+    uint32_t j = i;
+    if (j < n_xz) {  // (x, z): lty and the reference emitter's horizontal factors
+      const float npx = np_of((int)(j % nx), st.gx), npz = np_of((int)(j / nx), st.gz);
+      tab[o.xz + j] = 0.12f * fxm_sin(npx * 29.0f + npz * 23.0f - st.tt41);
+      if (reference_emitter) reinterpret_cast<float4*>(tab + o.xz4)[j] = emitter_xz(st, npx, npz);
+      continue;
+    }
+    j -= n_xz;
+    if (j < n_xy) {  // (x, y): ltz
+      const float npx = np_of((int)(j % nx), st.gx), npy = np_of((int)(j / nx), st.gy);
+      tab[o.xy + j] = fxm_cos(npy * 27.0f - npx * 21.0f + st.tt27) + 0.45f * fxm_cos(npy * 59.0f + st.tt47);
+      continue;
+    }
+    j -= n_xy;
+    if (j < n_yz) {  // (y, z): ltx
+      const float npy = np_of((int)(j % ny), st.gy), npz = np_of((int)(j / ny), st.gz);
+      tab[o.yz + j] = fxm_sin(npy * 31.0f + npz * 19.0f + st.tt31) + 0.45f * fxm_sin(npy * 67.0f - st.tt53);
+      continue;
+    }
+    j -= n_yz;
+    if (j < (uint32_t)nx) {  // x: swirl z-factor
+      tab[o.x + j] = fxm_cos(st.swirl_tx + np_of((int)j, st.gx) * 25.0f);
+      continue;
+    }
+    j -= nx;
+    if (j < (uint32_t)ny) {  // y: height envelope, outflow terms, emitter vertical
+      const float npy = np_of((int)j, st.gy);
+      const float envelope = fxm_smoothstep(0.045f, 0.15f, npy) * (1.0f - fxm_smoothstep(0.80f, 1.0f, npy));
+      const float outflow = fxm_smoothstep(0.84f, 0.995f, npy);
+      reinterpret_cast<float4*>(tab + o.y4)[j] =
+          make_float4(envelope, 1.0f - outflow, fxm_exp(-outflow * st.dt * 15.0f / st.persistence),
+                      outflow * 0.16f / st.persistence);
+      tab[o.yv + j] = emitter_vertical(st, npy);
+      continue;
+    }
+    j -= ny;  // z: swirl x-factor
+    tab[o.z + j] = fxm_sin(st.swirl_tz + np_of((int)j, st.gz) * 22.0f);
+  }
 }
 
 __device__ inline float emitter_seeded(const FxStep& st, float pcx, float pcy, float pcz, uint32_t cell) {
@@ -193,13 +272,15 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
                                                   float4* __restrict__ flowOut,
                                                   typename SpeciesIO<S>::Word* __restrict__ specOut,
                                                   float* __restrict__ reactionOut,
-                                                  const uint32_t* __restrict__ liquid, FxStep st) {
+                                                  const uint32_t* __restrict__ liquid,
+                                                  const float* __restrict__ tab, FxStep st) {
   int x, y, z;
   if (!cell_coords(st, x, y, z)) return;
   const uint32_t id = cidx(st, x, y, z);
   const float dt = st.dt;
   float pcx = (float)x + 0.5f, pcy = (float)y + 0.5f, pcz = (float)z + 0.5f;
-  float npx = pcx / st.gx, npy = pcy / st.gy, npz = pcz / st.gz;
+  const SimTab to(st.nx, st.ny, st.nz);
+  const float4 ty = reinterpret_cast<const float4*>(tab + to.y4)[y];  // envelope, 1 - outflow, decay, mix
 
   const float4 orig = flowIn[id];
   float bxp = pcx - orig.x * dt * st.mx;
@@ -235,12 +316,18 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     sp = mix4(sp, sa, st.mix_species);
   }
 
-  const float source = kSeeded ? emitter_seeded(st, pcx, pcy, pcz, id) : emitter_reference(st, npx, npy, npz);
+  float source;
+  if (kSeeded) {
+    source = emitter_seeded(st, pcx, pcy, pcz, id);
+  } else {
+    const float4 e = reinterpret_cast<const float4*>(tab + to.xz4)[x + st.nx * z];
+    source = fxm_exp(-(e.x + tab[to.yv + y])) * e.y * e.z;
+  }
   sp.x = fxm_max(sp.x, source * st.chem_x * st.fire_power);
   fl.w = fxm_max(fl.w, st.ambient + source * 1180.0f * st.thermal_power);
   {
-    float swx = fxm_sin(st.swirl_tz + npz * 22.0f);
-    float swz = fxm_cos(st.swirl_tx + npx * 25.0f);
+    float swx = tab[to.z + z];
+    float swz = tab[to.x + x];
     float swk = source * dt * 0.11f * st.chem_w * st.power_root;
     fl.x = fl.x + swx * swk;
     fl.y = fl.y + 0.0f * swk;
@@ -315,10 +402,10 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
   }
   {
     float hotPlume = fxm_smoothstep(520.0f, 1650.0f, fl.w);
-    float heightEnvelope = fxm_smoothstep(0.045f, 0.15f, npy) * (1.0f - fxm_smoothstep(0.80f, 1.0f, npy));
-    float ltx = fxm_sin(npy * 31.0f + npz * 19.0f + st.tt31) + 0.45f * fxm_sin(npy * 67.0f - st.tt53);
-    float lty = 0.12f * fxm_sin(npx * 29.0f + npz * 23.0f - st.tt41);
-    float ltz = fxm_cos(npy * 27.0f - npx * 21.0f + st.tt27) + 0.45f * fxm_cos(npy * 59.0f + st.tt47);
+    float heightEnvelope = ty.x;
+    float ltx = tab[to.yz + y + st.ny * z];
+    float lty = tab[to.xz + x + st.nx * z];
+    float ltz = tab[to.xy + x + st.nx * y];
     float turbulenceStrength = st.turbulence_base * hotPlume * heightEnvelope;
     float tk = turbulenceStrength * dt;
     fl.x = fl.x + ltx * tk;
@@ -340,13 +427,12 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     sp.w *= st.decay_vapor;
   }
   {
-    float outflow = fxm_smoothstep(0.84f, 0.995f, npy);
-    sp.x *= 1.0f - outflow;
-    float outflowDecay = fxm_exp(-outflow * dt * 15.0f / st.persistence);
+    sp.x *= ty.y;  // 1 - outflow, outflow = smoothstep(0.84, 0.995, npy)
+    float outflowDecay = ty.z;
     sp.z *= outflowDecay;
     sp.w *= outflowDecay;
-    fl.w = fxm_mix(fl.w, st.ambient, outflow * 0.16f / st.persistence);
-    react *= 1.0f - outflow;
+    fl.w = fxm_mix(fl.w, st.ambient, ty.w);
+    react *= ty.y;
   }
   if (y == st.ny - 1) {
     sp.x = 0.0f; sp.z = 0.0f; sp.w = 0.0f;
@@ -571,13 +657,21 @@ void launch_copy_u128(void* dst, const void* src, uint64_t n16, hipStream_t s) {
 template <int S>
 static void simulate_s(const SimulateArgs& a, hipStream_t s) {
   using W = typename SpeciesIO<S>::Word;
-  if (a.st.seeded_count > 0)
-    k_simulate<S, true><<<cell_grid(a.st), dim3(kBX, kBY), 0, s>>>(a.flow_in, (const W*)a.species_in, a.flow_out,
-                                                                    (W*)a.species_out, a.reaction, a.liquid, a.st);
+  const FxStep& st = a.st;
+  const uint32_t entries = (uint32_t)st.nx * st.nz + (uint32_t)st.nx * st.ny + (uint32_t)st.ny * st.nz +
+                           (uint32_t)(st.nx + st.ny + st.nz);
+  uint32_t blocks = (entries + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_sim_tables<<<blocks, 256, 0, s>>>(a.tables, st, st.seeded_count > 0 ? 0 : 1);
+  if (st.seeded_count > 0)
+    k_simulate<S, true><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(a.flow_in, (const W*)a.species_in, a.flow_out,
+                                                                  (W*)a.species_out, a.reaction, a.liquid, a.tables, st);
   else
-    k_simulate<S, false><<<cell_grid(a.st), dim3(kBX, kBY), 0, s>>>(a.flow_in, (const W*)a.species_in, a.flow_out,
-                                                                     (W*)a.species_out, a.reaction, a.liquid, a.st);
+    k_simulate<S, false><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(a.flow_in, (const W*)a.species_in, a.flow_out,
+                                                                   (W*)a.species_out, a.reaction, a.liquid, a.tables, st);
 }
+
+size_t sim_table_bytes(const int32_t dims[3]) { return (size_t)fx_sim_table_floats(dims) * 4u; }
 
 void launch_simulate(int storage, const SimulateArgs& a, hipStream_t s) {
   switch (storage) {

@@ -40,6 +40,7 @@ struct firex_solver {
   float* reaction = nullptr;
   uint32_t* liquid = nullptr;
   double* scratch = nullptr;
+  float* simtab = nullptr;  // simulate's per-step coordinate tables
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -229,7 +230,7 @@ void fill(firex_solver* s, void* p, uint64_t words) {
 // simulate, the first pass of encodeSimulationStep (FireXEngine.ts:2172-2173).
 void encode_simulate(firex_solver* s, const FxStep& st) {
   SimulateArgs a{flow0(s), s->species[s->species_index], flow1(s), s->species[1 - s->species_index], s->reaction,
-                 s->liquid, st};
+                 s->liquid, s->simtab, st};
   timed(s, FIREX_PASS_SIMULATE, [&] { launch_simulate(s->cfg.species_storage, a, s->stream); });
   s->species_index = 1 - s->species_index;
 }
@@ -488,6 +489,7 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
   if (!rc) rc = alloc(s, (void**)&s->divergence, c * 4, "divergence");
   if (!rc) rc = alloc(s, (void**)&s->reaction, c * 4, "reaction");
   if (!rc) rc = alloc(s, (void**)&s->scratch, kReduceScratchBytes, "reduction scratch");
+  if (!rc) rc = alloc(s, (void**)&s->simtab, sim_table_bytes(s->dims), "simulate tables");
   if (!rc) {
     if (cfg->jacobi_autotune && cfg->plan == FIREX_PLAN_FUSED) {
       // Times candidate plans on the solver's own pressure and divergence

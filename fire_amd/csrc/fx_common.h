@@ -81,6 +81,13 @@ __device__ static inline float fx_keep_f32(float v) {
 }
 #endif
 
+// Floats of simulate's per-step coordinate tables for an nx x ny x nz grid
+// (layout: SimTab in firex_kernels.hip).
+static inline uint64_t fx_sim_table_floats(const int32_t d[3]) {
+  const uint64_t nx = (uint64_t)d[0], ny = (uint64_t)d[1], nz = (uint64_t)d[2];
+  return 5 * nx * nz + 5 * ny + nx + nz + ny * nz + nx * ny;
+}
+
 // Counter-based seeded noise: splitmix64 finaliser on (seed, frame, cell, stream).
 #if defined(__HIP__)
 __host__ __device__
