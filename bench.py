@@ -106,6 +106,20 @@ def pass_profile(solver, steps: int, cells: int, storage: str, f16_pressure: boo
     return out
 
 
+def pack_profile(solver, cells: int, storage: str, reps: int = 10) -> dict:
+    """packFields (the render volumes) timed alone: flow 16 B + species
+    (8 or 16 B) + reaction 4 B read, three f16 texels (24 B) written."""
+    bpc = 16 + (16 if storage == "f32" else 8) + 4 + 24
+    solver.set_pass_timing(True)
+    for _ in range(reps):
+        solver.pack_fields()
+    t = solver.pass_timing()
+    solver.set_pass_timing(False)
+    ms, n = t["pack"]["ms"], t["pack"]["launches"]
+    return {"avg_launch_us": ms / n * 1e3, "algorithmic_bytes_per_cell": bpc,
+            "algorithmic_GBps": round(bpc * cells * n / (ms * 1e-3) / 1e9, 1), "launches": n}
+
+
 def measured_traffic(workload: str, plan: dict):
     """HBM bytes per Jacobi launch for this plan from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
@@ -290,6 +304,7 @@ def main() -> int:
     value = cells * args.steps / elapsed
     print(f"bench: timed {args.steps} steps in {elapsed:.3f} s ({ms_per_step:.3f} ms/step)", file=sys.stderr, flush=True)
     passes = pass_profile(solver, args.profile_steps, cells, wl["storage"])
+    passes["pack_fields"] = pack_profile(solver, cells, wl["storage"])
     t_fuel = solver.reduce("sum_fuel")
     solver.close()
 

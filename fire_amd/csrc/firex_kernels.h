@@ -29,6 +29,11 @@ void launch_fill_u32(void* p, uint64_t words, uint32_t value, hipStream_t s);
 void launch_copy_u128(void* dst, const void* src, uint64_t n16, hipStream_t s);
 void launch_simulate(int storage, const SimulateArgs& a, hipStream_t s);
 void launch_divergence(const float4* flow, float* div, const FxStep& st, hipStream_t s);
+// packFields into out: three consecutive volumes of 4 x f16 per cell
+// (flow, chemistry, media); liquid may be null (zero field).
+void launch_pack_fields(int storage, const float4* flow, const void* species, const float* reaction,
+                        const uint32_t* liquid, const float* div, void* out, const FxStep& st, int divergence_view,
+                        hipStream_t s);
 void launch_jacobi(int f16, const void* pin, void* pout, const float* div, const FxStep& st, hipStream_t s);
 void launch_project(int f16, const float4* fin, const void* p, float4* fout, const FxStep& st, hipStream_t s);
 void launch_reduce_field(int storage, int kind, int component, const void* data, uint32_t cells, double* partial,

@@ -454,6 +454,52 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
   reactionOut[id] = react;
 }
 
+// packFields / storePackedFields, qualityShaders.ts:37-73: the raymarcher's
+// three rgba16float volumes, stored as 4 x f16 per cell in cell order:
+//   flow      (u, v, w, clamp((T - 300) / 2600, 0, 1))
+//   chemistry max(species, 0)
+//   media     (reaction, soot, liquid, display value)
+// With the divergence view (view mode 7) the display value is the
+// divergence field (refreshed from flow[0] just before), 0 on the outer
+// cell shell.
+__device__ inline uint32_t pack_half2(float a, float b) {
+  const uint16_t ha = __builtin_bit_cast(uint16_t, (_Float16)fx_keep_f32(a));
+  const uint16_t hb = __builtin_bit_cast(uint16_t, (_Float16)fx_keep_f32(b));
+  return (uint32_t)ha | ((uint32_t)hb << 16);
+}
+__device__ inline uint2 pack_half4(float a, float b, float c, float d) {
+  return make_uint2(pack_half2(a, b), pack_half2(c, d));
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_pack_fields(const float4* __restrict__ flow,
+                                                     const typename SpeciesIO<S>::Word* __restrict__ species,
+                                                     const float* __restrict__ reaction,
+                                                     const uint32_t* __restrict__ liquid,
+                                                     const float* __restrict__ div, uint2* __restrict__ out,
+                                                     FxStep st, int divergence_view) {
+  int x, y, z;
+  if (!cell_coords(st, x, y, z)) return;
+  const uint32_t id = cidx(st, x, y, z);
+  const uint32_t cells = (uint32_t)st.nx * (uint32_t)st.ny * (uint32_t)st.nz;
+  const float4 f = flow[id];
+  float4 sp = SpeciesIO<S>::load(species, id);
+  sp = make_float4(fxm_max(sp.x, 0.0f), fxm_max(sp.y, 0.0f), fxm_max(sp.z, 0.0f), fxm_max(sp.w, 0.0f));
+  const float react = fxm_clamp(reaction[id], 0.0f, 4.0f);
+  const float richness = fxm_clamp(sp.x / fxm_max(sp.x + sp.y * 0.25f, 0.0001f), 0.0f, 1.0f);
+  const float soot = sp.z * fxm_smoothstep(0.25f, 0.82f, richness);
+  const float liq = (float)(liquid ? liquid[id] : 0u) * (3.4f / 4096.0f);
+  float display = fxm_max(fxm_max(react, soot),
+                          fxm_max(liq, fxm_max(sp.w, fxm_smoothstep(420.0f, 700.0f, f.w))));
+  if (divergence_view) {
+    const bool shell = x == 0 || y == 0 || z == 0 || x + 1 >= st.nx || y + 1 >= st.ny || z + 1 >= st.nz;
+    display = shell ? 0.0f : div[id];
+  }
+  out[id] = pack_half4(f.x, f.y, f.z, fxm_clamp((f.w - 300.0f) / 2600.0f, 0.0f, 1.0f));
+  out[cells + id] = pack_half4(sp.x, sp.y, sp.z, sp.w);
+  out[2u * cells + id] = pack_half4(react, soot, liq, display);
+}
+
 // computeDivergence, shaders.ts:344-364.
 __global__ __launch_bounds__(256) void k_divergence(const float4* __restrict__ flow, float* __restrict__ div, FxStep st) {
   int x, y, z;
@@ -678,6 +724,28 @@ void launch_simulate(int storage, const SimulateArgs& a, hipStream_t s) {
     case FX_STORAGE_F32: simulate_s<FX_STORAGE_F32>(a, s); break;
     case FX_STORAGE_F16: simulate_s<FX_STORAGE_F16>(a, s); break;
     default: simulate_s<FX_STORAGE_BF16>(a, s); break;
+  }
+}
+
+void launch_pack_fields(int storage, const float4* flow, const void* species, const float* reaction,
+                        const uint32_t* liquid, const float* div, void* out, const FxStep& st, int divergence_view,
+                        hipStream_t s) {
+  uint2* o = (uint2*)out;
+  switch (storage) {
+    case FX_STORAGE_F32:
+      k_pack_fields<FX_STORAGE_F32><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(
+          flow, (const float4*)species, reaction, liquid, div, o, st, divergence_view);
+      break;
+    case FX_STORAGE_F16:
+      k_pack_fields<FX_STORAGE_F16><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(
+          flow, (const typename SpeciesIO<FX_STORAGE_F16>::Word*)species, reaction, liquid, div, o, st,
+          divergence_view);
+      break;
+    default:
+      k_pack_fields<FX_STORAGE_BF16><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(
+          flow, (const typename SpeciesIO<FX_STORAGE_BF16>::Word*)species, reaction, liquid, div, o, st,
+          divergence_view);
+      break;
   }
 }
 

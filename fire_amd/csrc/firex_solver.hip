@@ -41,6 +41,7 @@ struct firex_solver {
   uint32_t* liquid = nullptr;
   double* scratch = nullptr;
   float* simtab = nullptr;  // simulate's per-step coordinate tables
+  void* packed = nullptr;   // render volumes of firex_pack_fields (24 B/cell), on first use
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -698,6 +699,12 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
     case FIREX_FIELD_DIVERGENCE: src = s->divergence; break;
     case FIREX_FIELD_REACTION: src = s->reaction; break;
     case FIREX_FIELD_LIQUID: src = s->liquid; kind = 3; break;
+    case FIREX_FIELD_PACKED_FLOW: case FIREX_FIELD_PACKED_CHEMISTRY: case FIREX_FIELD_PACKED_MEDIA:
+      if (!s->packed) return set_error(FIREX_ERR_INVALID, "the render volumes are filled by firex_pack_fields");
+      src = (const uint8_t*)s->packed + (size_t)(field - FIREX_FIELD_PACKED_FLOW) * c * 8;
+      comps = 4;
+      kind = 1;  // 4 x f16
+      break;
     default: return set_error(FIREX_ERR_INVALID, "unknown field");
   }
   const uint64_t n = c * comps;
@@ -780,6 +787,35 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
   FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before write");
   FX_HIP(s, hipMemcpy(dst, staging.data(), sbytes, hipMemcpyHostToDevice), "field upload");
   return FIREX_OK;
+}
+
+int firex_pack_fields(firex_solver* s) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  hipSetDevice(s->cfg.device);
+  if (!s->packed) {
+    const uint64_t bytes = (uint64_t)s->cells * 24;
+    if (s->cfg.memory_budget_bytes && s->bytes + bytes > s->cfg.memory_budget_bytes)
+      return set_error(FIREX_ERR_BUDGET, "the render volumes (" + std::to_string(bytes) +
+                                             " bytes) do not fit the memory budget");
+    rc = alloc(s, &s->packed, bytes, "render volumes");
+    if (rc) return rc;
+    s->bytes += bytes;
+  }
+  float u[FXU_COUNT];
+  write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
+                 s->cfg.cinematic != 0, u);
+  const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
+  // u32(clamp(round(controls.x), 0, 7)) == 7u, qualityShaders.ts:55
+  const float view = fxm_clamp(nearbyintf(u[FXU_VIEW_MODE]), 0.0f, 7.0f);
+  const int divergence_view = (uint32_t)view == 7u ? 1 : 0;
+  if (divergence_view)
+    timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow0(s), s->divergence, st, s->stream); });
+  timed(s, FIREX_PASS_PACK, [&] {
+    launch_pack_fields(s->cfg.species_storage, flow0(s), s->species[s->species_index], s->reaction, s->liquid,
+                       s->divergence, s->packed, st, divergence_view, s->stream);
+  });
+  return check_launch(s, "packFields");
 }
 
 int firex_reduce(firex_solver* s, int32_t which, double* out) {

@@ -3,7 +3,8 @@
 // src/sim/firex/FireXEngine.ts:370).
 
 export type FireXFieldName =
-  | 'flow' | 'species' | 'flowPredicted' | 'pressure' | 'correction' | 'divergence' | 'reaction' | 'liquid';
+  | 'flow' | 'species' | 'flowPredicted' | 'pressure' | 'correction' | 'divergence' | 'reaction' | 'liquid'
+  | 'packedFlow' | 'packedChemistry' | 'packedMedia';
 
 export type FireXReductionName =
   | 'sumFuel' | 'sumOxygen' | 'sumProducts' | 'sumVapor' | 'sumTemperature' | 'sumReaction'
@@ -60,6 +61,8 @@ export declare class FireXHipSolver {
   readField<T extends Float32Array | Float64Array | Uint32Array = Float32Array>(name: FireXFieldName, out?: T): T;
   writeField(name: FireXFieldName, data: Float32Array | Float64Array | Uint32Array): void;
   reduce(name: FireXReductionName): number;
+  /** packFields: fill the render volumes read back as packedFlow / packedChemistry / packedMedia. */
+  packFields(): void;
   getGridInfo(): FireXHipGridInfo;
   dispose(): void;
 }

@@ -19,6 +19,7 @@ const native = require(path.join(__dirname, '..', 'firex_napi.node'));
 
 const FIELD_COMPONENTS = {
   flow: 4, species: 4, flowPredicted: 4, pressure: 1, correction: 1, divergence: 1, reaction: 1, liquid: 1,
+  packedFlow: 4, packedChemistry: 4, packedMedia: 4,
 };
 const REDUCTIONS = {
   sumFuel: 0, sumOxygen: 1, sumProducts: 2, sumVapor: 3, sumTemperature: 4, sumReaction: 5,
@@ -119,6 +120,12 @@ class FireXHipSolver {
 
   writeField(name, data) {
     native.writeField(this.requireHandle(), name, data);
+  }
+
+  /** packFields (qualityShaders.ts:37-73): fill the three render volumes
+   *  (readField('packedFlow' | 'packedChemistry' | 'packedMedia')). */
+  packFields() {
+    native.packFields(this.requireHandle());
   }
 
   reduce(name) {

@@ -330,9 +330,10 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
 }
 
 static int field_id(const char* name) {
-  static const char* names[] = {"flow", "species", "pressure", "correction", "divergence", "reaction", "liquid",
-                                "flowPredicted"};
-  for (int i = 0; i < 8; ++i)
+  static const char* names[] = {"flow",          "species",    "pressure",        "correction",
+                                "divergence",    "reaction",   "liquid",          "flowPredicted",
+                                "packedFlow",    "packedChemistry", "packedMedia"};
+  for (int i = 0; i < (int)(sizeof names / sizeof names[0]); ++i)
     if (strcmp(name, names[i]) == 0) return i;
   return -1;
 }
@@ -380,6 +381,16 @@ static napi_value js_field(napi_env env, napi_callback_info info, int write) {
 
 static napi_value js_read_field(napi_env env, napi_callback_info info) { return js_field(env, info, 0); }
 static napi_value js_write_field(napi_env env, napi_callback_info info) { return js_field(env, info, 1); }
+
+/* packFields(h): the three render volumes (qualityShaders.ts:37-73). */
+static napi_value js_pack_fields(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  FX_CHECK(env, firex_pack_fields(s));
+  return NULL;
+}
 
 static napi_value js_reduce(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -514,6 +525,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readField", NULL, js_read_field, NULL, NULL, NULL, napi_default, NULL},
       {"writeField", NULL, js_write_field, NULL, NULL, NULL, napi_default, NULL},
       {"reduce", NULL, js_reduce, NULL, NULL, NULL, napi_default, NULL},
+      {"packFields", NULL, js_pack_fields, NULL, NULL, NULL, napi_default, NULL},
       {"getInfo", NULL, js_get_info, NULL, NULL, NULL, napi_default, NULL},
       {"estimateMemory", NULL, js_estimate_memory, NULL, NULL, NULL, napi_default, NULL},
       {"splitOfflineDelta", NULL, js_split_offline_delta, NULL, NULL, NULL, napi_default, NULL},

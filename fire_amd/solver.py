@@ -35,8 +35,9 @@ OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_BUDGET, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR
 FIELDS = {
     "flow": 0, "species": 1, "pressure": 2, "correction": 3,
     "divergence": 4, "reaction": 5, "liquid": 6, "flow_predicted": 7,
+    "packed_flow": 8, "packed_chemistry": 9, "packed_media": 10,
 }
-VECTOR_FIELDS = {"flow", "species", "flow_predicted"}
+VECTOR_FIELDS = {"flow", "species", "flow_predicted", "packed_flow", "packed_chemistry", "packed_media"}
 DTYPE_F32, DTYPE_F64, DTYPE_U32 = 0, 1, 2
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 HEAVY = {"auto": 0, "off": 1, "f16": 2, "f32": 3}
@@ -125,7 +126,7 @@ class DeviceLimits(ctypes.Structure):
         "max_texture_dimension_3d", "max_storage_buffer_binding_size", "max_buffer_size")]
 
 
-PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy")
+PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack")
 
 
 class PassTiming(ctypes.Structure):
@@ -173,6 +174,7 @@ _SIGNATURES = {
     "firex_read_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
     "firex_write_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
     "firex_reduce": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "firex_pack_fields": (ctypes.c_int, [_P]),
     "firex_get_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
     "firex_set_pass_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "firex_get_pass_timing": (ctypes.c_int, [_P, ctypes.POINTER(PassTiming)]),
@@ -437,6 +439,11 @@ class FireXHipSolver:
             code = DTYPE_F32
         arr = np.ascontiguousarray(arr.reshape(self._shape(name)))
         _check(self._lib.firex_write_field(self._h, FIELDS[name], arr.ctypes.data, arr.nbytes, code))
+
+    def pack_fields(self) -> None:
+        """packFields: fill the three render volumes (read them as packed_flow,
+        packed_chemistry, packed_media)."""
+        _check(self._lib.firex_pack_fields(self._h))
 
     def reduce(self, name: str) -> float:
         out = ctypes.c_double()

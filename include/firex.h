@@ -51,7 +51,11 @@ enum {
   FIREX_FIELD_DIVERGENCE = 4,      /* f32 divergence buffer (residual RHS after a step)   */
   FIREX_FIELD_REACTION = 5,        /* f32 heat-release diagnostic                         */
   FIREX_FIELD_LIQUID = 6,          /* u32 fixed-point liquid (x 4096), input of simulate  */
-  FIREX_FIELD_FLOW_PREDICTED = 7   /* vec4 f32 flow[1]                                    */
+  FIREX_FIELD_FLOW_PREDICTED = 7,  /* vec4 f32 flow[1]                                    */
+  /* render volumes written by firex_pack_fields (4 x f16 per cell; read as f32/f64) */
+  FIREX_FIELD_PACKED_FLOW = 8,     /* (u, v, w, clamp((T - 300) / 2600, 0, 1))            */
+  FIREX_FIELD_PACKED_CHEMISTRY = 9,/* max(species, 0)                                     */
+  FIREX_FIELD_PACKED_MEDIA = 10    /* (reaction, soot, liquid, display value)             */
 };
 
 enum { FIREX_DTYPE_F32 = 0, FIREX_DTYPE_F64 = 1, FIREX_DTYPE_U32 = 2 };
@@ -241,6 +245,15 @@ int firex_sync(firex_solver* solver);
 int firex_read_field(firex_solver* solver, int32_t field, void* dst, size_t bytes, int32_t dtype);
 /* Overwrite a field from host memory (seeding and tests). */
 int firex_write_field(firex_solver* solver, int32_t field, const void* src, size_t bytes, int32_t dtype);
+/* packFields (src/sim/firex/qualityShaders.ts:37-73) as encodeRender runs it
+ * (FireXEngine.ts:2585-2605): fills the three render volumes (flow,
+ * chemistry, media; rgba16float, 24 B/cell, allocated on the first call) from
+ * the current state.  With viewMode 7 (the divergence view) it first
+ * recomputes the divergence of the final flow into the divergence field, as
+ * the presentation divergence pass does.  Read the volumes with
+ * firex_read_field(FIREX_FIELD_PACKED_*). */
+int firex_pack_fields(firex_solver* solver);
+
 /* FP64 reduction on the device (FIREX_REDUCE_*). */
 int firex_reduce(firex_solver* solver, int32_t which, double* out);
 int firex_get_info(firex_solver* solver, firex_info* info);
@@ -253,6 +266,7 @@ enum {
   FIREX_PASS_PROJECT = 3,
   FIREX_PASS_PROJECT_DIVERGENCE = 4,  /* no longer launched (kept for the layout) */
   FIREX_PASS_FILL_COPY = 5,           /* clears and copyFlow                      */
+  FIREX_PASS_PACK = 6,                /* firex_pack_fields                        */
   FIREX_PASS_COUNT = 8
 };
 typedef struct firex_pass_timing {

@@ -41,6 +41,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 
 #include <omp.h>
 
@@ -170,6 +171,7 @@ bool set_parameter(Params& p, const char* name, double v) {
   else if (n == "domainWidth") p.domainWidth = clampd(finite_or(v, p.domainWidth), 0.5, 4);
   else if (n == "domainHeight") p.domainHeight = clampd(finite_or(v, p.domainHeight), 0.5, 6);
   else if (n == "domainDepth") p.domainDepth = clampd(finite_or(v, p.domainDepth), 0.5, 4);
+  else if (n == "viewMode") p.viewMode = clampd(std::round(finite_or(v, p.viewMode)), 0, 7);  // FireXEngine.ts:994-995
   else return false;
   return true;
 }
@@ -531,6 +533,44 @@ template <typename T> struct State {
   }
 
   V4<T> store_species(const V4<T>& s) const;
+  T texel(T v) const;  // an rgba16float texel component: f16 rounding (FP32 oracle)
+
+  // packFields / storePackedFields, qualityShaders.ts:37-73, with the
+  // presentation divergence refresh that precedes it in view mode 7
+  // (FireXEngine.ts:2590-2597): the raymarcher's three rgba16float volumes
+  //   flow      (u, v, w, clamp((T - 300) / 2600, 0, 1))
+  //   chemistry max(species, 0)
+  //   media     (reaction, soot, liquid, display value)
+  // from flow[0], species[current], reaction and liquid.
+  void pack_fields(std::vector<V4<T>>& pf, std::vector<V4<T>>& pc, std::vector<V4<T>>& pm) {
+    // controls.x, the view mode: the current parameter, as the next
+    // uniform write (every frame) would hold it
+    const T viewMode = T((float)params.viewMode);
+    const bool divergenceView = wclamp(T(std::nearbyint(viewMode)), T(0), T(7)) == T(7);
+    if (divergenceView) compute_divergence(flow[0]);
+    pf.resize(n);
+    pc.resize(n);
+    pm.resize(n);
+    const std::vector<V4<T>>& sp = species[speciesIndex];
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const size_t id = idx(x, y, z);
+          const V4<T> f = flow[0][id];
+          const V4<T> s{wmax(sp[id].x, T(0)), wmax(sp[id].y, T(0)), wmax(sp[id].z, T(0)), wmax(sp[id].w, T(0))};
+          const T react = wclamp(reaction[id], T(0), T(4));
+          const T richness = wclamp(s.x / wmax(s.x + s.y * T(0.25), T(0.0001)), T(0), T(1));
+          const T soot = s.z * wsmooth(T(0.25), T(0.82), richness);
+          const T liq = T((float)liquid[id]) * T(3.4f / 4096.0f);
+          const bool shell = x == 0 || y == 0 || z == 0 || x + 1 >= nx || y + 1 >= ny || z + 1 >= nz;
+          T display = wmax(wmax(react, soot), wmax(liq, wmax(s.w, wsmooth(T(420), T(700), f.w))));
+          if (divergenceView) display = shell ? T(0) : divergence[id];
+          pf[id] = V4<T>{texel(f.x), texel(f.y), texel(f.z), texel(wclamp((f.w - T(300)) / T(2600), T(0), T(1)))};
+          pc[id] = V4<T>{texel(s.x), texel(s.y), texel(s.z), texel(s.w)};
+          pm[id] = V4<T>{texel(react), texel(soot), texel(liq), texel(display)};
+        }
+  }
 
   // computeDivergence, shaders.ts:344-364.
   void compute_divergence(const std::vector<V4<T>>& f) {
@@ -691,6 +731,8 @@ template <> V4<float> State<float>::store_species(const V4<float>& s) const {
   return s;
 }
 template <> V4<double> State<double>::store_species(const V4<double>& s) const { return s; }
+template <> float State<float>::texel(float v) const { return round_to_f16(v); }
+template <> double State<double>::texel(double v) const { return v; }
 
 template <> float State<float>::jacobi_divide(float num, float denom) const {
   return num * (1.0f / denom);
@@ -954,6 +996,25 @@ void orc_stencil_weights(void* p, double dt, double* out3) {
   out3[0] = h->d.mx * h->d.mx;
   out3[1] = h->d.my * h->d.my;
   out3[2] = h->d.mz * h->d.mz;
+}
+
+// packFields: three vec4 volumes (flow, chemistry, media), 4 n values each.
+int orc_pack_fields(void* p, double* out_flow, double* out_chem, double* out_media) {
+  Handle* h = static_cast<Handle*>(p);
+  auto run = [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.flow[0][0].x)>::type;
+    std::vector<V4<T>> pf, pc, pm;
+    s.pack_fields(pf, pc, pm);
+    double* outs[3] = {out_flow, out_chem, out_media};
+    const std::vector<V4<T>>* ins[3] = {&pf, &pc, &pm};
+    for (int k = 0; k < 3; ++k)
+      for (size_t i = 0; i < s.n; ++i) {
+        const V4<T>& v = (*ins[k])[i];
+        outs[k][4 * i] = v.x; outs[k][4 * i + 1] = v.y; outs[k][4 * i + 2] = v.z; outs[k][4 * i + 3] = v.w;
+      }
+    return 0;
+  };
+  return h->precision == 32 ? run(h->f) : run(h->d);
 }
 
 // Threads of the OpenMP loops (the CPU baseline times 1 thread and all cores).

@@ -72,6 +72,7 @@ def load() -> ctypes.CDLL:
         "orc_f32_exp2": (f, [f]),
         "orc_f32_eval": (None, [i, fp, fp, fp, ctypes.c_int64]),
         "orc_stencil_weights": (None, [vp, d, dp]),
+        "orc_pack_fields": (i, [vp, dp, dp, dp]),
         "orc_set_threads": (None, [i]),
         "orc_max_threads": (i, []),
     }
@@ -174,6 +175,16 @@ class Oracle:
         if rc:
             raise ValueError(f"unknown field {field}")
         return out
+
+    def pack_fields(self):
+        """packFields (qualityShaders.ts:37-73): (flow, chemistry, media)
+        volumes of shape (nz, ny, nx, 4); the FP32 oracle rounds each
+        component to f16 as the rgba16float texture store does."""
+        nx, ny, nz = self.dims
+        outs = [np.empty((nz, ny, nx, 4), dtype=np.float64) for _ in range(3)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.lib.orc_pack_fields(self.h, *(o.ctypes.data_as(dp) for o in outs))
+        return tuple(outs)
 
     def write(self, field: int, values) -> None:
         arr = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(self._shape(field)))

@@ -66,6 +66,10 @@ def test_uhd_mixed_workload_bitwise():
     em = seeded_emitters(dims, 3, radius_range=(6.0, 14.0), seed=2026)
     g, o = _run_pair(dims, "f16", "off", em, 0.0, 2)
     _compare(g, o, "UHD mixed", {"flow", "species", "pressure", "correction", "divergence", "reaction"})
+    g.pack_fields()
+    for name, ref in zip(("packed_flow", "packed_chemistry", "packed_media"), o.pack_fields()):
+        bad = int(np.count_nonzero(_bits(g.read_field(name)) != _bits(ref.astype(np.float32))))
+        assert bad == 0, f"UHD {name} differs in {bad} values"
     g.close()
     o.close()
 

@@ -289,3 +289,29 @@ def test_more_than_1024_jacobi_launches_bitwise():
     assert_fields_equal(g, o, "1100 sweeps", names=["flow", "species", "divergence", "reaction"])
     np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
     g.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("view", [0, 7])
+def test_pack_fields_bitwise(storage, view):
+    """packFields (qualityShaders.ts:37-73) after a few burning steps with a
+    liquid field: the three render volumes, f16 texels, bit for bit; view 7
+    is the divergence view (refreshed divergence, zero boundary shell)."""
+    dims = (20, 28, 16)
+    em = seeded_emitters(dims, 2, radius_range=(3.0, 5.0), seed=11)
+    g, o = make_pair(dims, iters=(8, 8), plan="fused", storage=storage, emitters=(em, 0.1, 0.0, 11))
+    liquid = np.random.default_rng(2).integers(0, 6000, size=(dims[2], dims[1], dims[0])).astype(np.uint32)
+    g.write_field("liquid", liquid)
+    o.write(orc.FIELD_LIQUID, liquid)
+    o.step(DT, 4)
+    g.step(DT, 4)
+    g.set_parameter("viewMode", view)
+    o.set_parameter("viewMode", view)
+    g.pack_fields()
+    pf, pc, pm = o.pack_fields()
+    for name, ref in (("packed_flow", pf), ("packed_chemistry", pc), ("packed_media", pm)):
+        got = g.read_field(name)
+        assert np.array_equal(_bits(got), _bits(ref.astype(np.float32))), \
+            f"{name} ({storage}, view {view}) differs in {int((_bits(got) != _bits(ref.astype(np.float32))).sum())} values"
+    assert pm[..., 1].max() > 0 or pm[..., 0].max() > 0  # the plume burns
+    g.close()
