@@ -104,7 +104,7 @@ bool params_get(const Params& p, const char* name, double* value) {
 
 // ---------------------------------------------------------------- uniforms
 void write_uniforms(const Params& p, const int32_t dims[3], double dt, double simulation_time, double frame_index,
-                    double time_step, bool cinematic, float u[FXU_COUNT]) {
+                    double time_step, bool cinematic, float u[FXU_COUNT], int32_t particle_count) {
   memset(u, 0, sizeof(float) * FXU_COUNT);
   double cells = (double)dims[0] * (double)dims[1] * (double)dims[2];
   u[FXU_GRID_X] = (float)dims[0];
@@ -126,6 +126,7 @@ void write_uniforms(const Params& p, const int32_t dims[3], double dt, double si
   u[FXU_CAMERA_YAW] = (float)p.cameraYaw;
   u[FXU_CAMERA_PITCH] = (float)p.cameraPitch;
   u[FXU_CAMERA_DISTANCE] = (float)p.cameraDistance;
+  u[FXU_PARTICLES] = (float)particle_count;
   u[FXU_VIEW_MODE] = (float)p.viewMode;
   u[FXU_AIM_HEIGHT] = (float)p.aimHeight;
   u[FXU_NOZZLE] = (float)p.nozzleType;
@@ -248,6 +249,94 @@ FxStep make_step(const float u[FXU_COUNT], const int32_t dims[3], const EmitterC
   return st;
 }
 
+FxDroplets make_droplets(const float u[FXU_COUNT], const int32_t dims[3]) {
+  FxDroplets d;
+  memset(&d, 0, sizeof d);
+  d.count = (uint32_t)u[FXU_PARTICLES];
+  d.nx = dims[0];
+  d.ny = dims[1];
+  d.nz = dims[2];
+  d.dt = u[FXU_DT];
+  d.frame = u[FXU_FRAME];
+  d.gx = u[FXU_GRID_X];
+  d.gy = u[FXU_GRID_Y];
+  d.gz = u[FXU_GRID_Z];
+  d.ex = fxm_max(u[FXU_DOMAIN_X], 0.001f);
+  d.ey = fxm_max(u[FXU_DOMAIN_Y], 0.001f);
+  d.ez = fxm_max(u[FXU_DOMAIN_Z], 0.001f);
+  d.water_flow = u[FXU_WATER_FLOW];
+  d.spray_angle = u[FXU_SPRAY_ANGLE];
+  d.water_enabled = u[FXU_WATER_ENABLED];
+  d.aim = u[FXU_AIM_HEIGHT];
+  d.nozzle = u[FXU_NOZZLE];
+  return d;
+}
+
+// Math.hypot as JS engines evaluate it: the magnitudes scaled by the largest,
+// a Kahan-compensated sum of their squares, square root, rescaled.
+static double js_hypot(const double* v, int n) {
+  double largest = 0;
+  bool nan = false;
+  for (int i = 0; i < n; ++i) {
+    const double a = std::fabs(v[i]);
+    if (std::isinf(a)) return INFINITY;
+    if (std::isnan(a)) nan = true;
+    else largest = std::max(largest, a);
+  }
+  if (nan) return NAN;
+  if (largest == 0) return 0;
+  double sum = 0, compensation = 0;
+  for (int i = 0; i < n; ++i) {
+    const double r = std::fabs(v[i]) / largest;
+    const double summand = r * r - compensation;
+    const double next = sum + summand;
+    compensation = (next - sum) - summand;
+    sum = next;
+  }
+  return std::sqrt(sum) * largest;
+}
+
+// writeParticleInitialState, FireXEngine.ts:1976-2034: float64 arithmetic,
+// stored as f32 (the reference fills a Float32Array).
+void particle_initial_state(const Params& p, int32_t count, bool active, float* out) {
+  const double w = p.domainWidth, h = p.domainHeight, depth = p.domainDepth;
+  const double aim[2] = {(0.52 - 0.035) * w, (p.aimHeight - 0.5) * h};
+  const double aim_length = js_hypot(aim, 2);
+  const double bx = aim[0] / aim_length, by = aim[1] / aim_length;
+  const double tx = by, ty = -bx;  // tangent in the xy plane
+  auto hash = [](double v) {
+    const double x = std::sin(v * 12.9898 + 1.317) * 43758.5453;
+    return x - std::floor(x);
+  };
+  const bool spray = p.nozzleType >= 0.5;
+  const double spread = spray ? 0.72 : 0.1;
+  const double variation = spray ? 0.24 : 0.05;
+  const double origin_radius = spray ? 0.006 : 0.0028;
+  for (int32_t i = 0; i < count; ++i) {
+    const double phase = (double)i / (double)count;
+    const double angle = hash(i * 3.771 + 4.91) * M_PI * 2;
+    const double radial = std::sqrt(hash(i * 3.771 + 0.17));
+    const double cone = std::tan((p.sprayAngle * M_PI) / 180) * radial * spread;
+    const double disk_x = tx * std::cos(angle), disk_y = ty * std::cos(angle), disk_z = -std::sin(angle);
+    double dir[3] = {bx + disk_x * cone, by + disk_y * cone, disk_z * cone};
+    const double length = js_hypot(dir, 3);
+    dir[0] /= length;
+    dir[1] /= length;
+    dir[2] /= length;
+    const double speed = (0.82 + p.waterFlow * 0.82) * (1 - variation * 0.5 + hash(i * 3.771 + 9.37) * variation);
+    const double age = phase * 0.72;
+    float* o = out + (size_t)i * 8;
+    o[0] = (float)(0.035 + (disk_x * radial * origin_radius + dir[0] * speed * age) / w);
+    o[1] = (float)std::max(0.012, 0.5 + (disk_y * radial * origin_radius + dir[1] * speed * age - 0.29 * age * age) / h);
+    o[2] = (float)(0.5 + (disk_z * radial * origin_radius + dir[2] * speed * age) / depth);
+    o[3] = (float)(active ? age : -phase * 0.72);
+    o[4] = (float)(active ? dir[0] * speed : 0);
+    o[5] = (float)(active ? dir[1] * speed - 0.58 * age : 0);
+    o[6] = (float)(active ? dir[2] * speed : 0);
+    o[7] = (float)(active ? 0.52 + p.waterFlow * 0.88 : 0);
+  }
+}
+
 // ------------------------------------------------------------------- clock
 int32_t FixedStepClock::advance(double delta_seconds) {
   if (!std::isfinite(delta_seconds)) return 0;
@@ -278,6 +367,8 @@ uint64_t device_bytes(const int32_t dims[3], int32_t species_storage, bool f16_p
   return bytes + (uint64_t)kReduceScratchBytes + fx_sim_table_floats(dims) * 4;
 }
 
+uint64_t particle_device_bytes(int32_t count) { return count > 0 ? (uint64_t)count * (2 * 32 + 16) : 0; }
+
 }  // namespace fx
 
 // =========================================================== C ABI (host only)
@@ -291,20 +382,21 @@ struct Tier {
   int32_t pressure, correction;
   double time_step;
   int32_t cinematic;
+  int32_t particles;
 };
 
 // FIREX_QUALITY (FireXEngine.ts:187-238) and FIREX_OFFLINE_QUALITY (:240-277).
 // The uniform's cinematic flag is set for the cinematic live tier and for
 // every offline tier (usesCinematicPresentation, FireXEngine.ts:2819-2821).
 const Tier kTiers[] = {
-    {"low", {32, 48, 32}, 14, 14, 1.0 / 30.0, 0},
-    {"balanced", {40, 64, 40}, 24, 24, 1.0 / 30.0, 0},
-    {"high", {64, 96, 64}, 36, 36, 1.0 / 30.0, 0},
-    {"maximum", {80, 120, 80}, 48, 48, 1.0 / 30.0, 0},
-    {"cinematic", {80, 120, 80}, 48, 48, 1.0 / 30.0, 1},
-    {"hd", {128, 192, 128}, 64, 64, 1.0 / 60.0, 1},
-    {"qhd", {160, 240, 160}, 80, 80, 1.0 / 60.0, 1},
-    {"uhd", {192, 288, 192}, 96, 96, 1.0 / 60.0, 1},
+    {"low", {32, 48, 32}, 14, 14, 1.0 / 30.0, 0, 384},
+    {"balanced", {40, 64, 40}, 24, 24, 1.0 / 30.0, 0, 640},
+    {"high", {64, 96, 64}, 36, 36, 1.0 / 30.0, 0, 1024},
+    {"maximum", {80, 120, 80}, 48, 48, 1.0 / 30.0, 0, 1536},
+    {"cinematic", {80, 120, 80}, 48, 48, 1.0 / 30.0, 1, 1536},
+    {"hd", {128, 192, 128}, 64, 64, 1.0 / 60.0, 1, 2048},
+    {"qhd", {160, 240, 160}, 80, 80, 1.0 / 60.0, 1, 3072},
+    {"uhd", {192, 288, 192}, 96, 96, 1.0 / 60.0, 1, 4096},
 };
 
 const int64_t kGridBytesPerCell = 116;       // gridConfiguration.ts:106
@@ -334,6 +426,7 @@ void firex_config_init(firex_config* c) {
   c->species_storage = FIREX_STORAGE_F32;
   c->heavy_mode = FIREX_HEAVY_AUTO;
   c->plan = FIREX_PLAN_REFERENCE;
+  c->particle_count = 640;
 }
 
 int firex_config_from_tier(firex_config* c, const char* tier) {
@@ -346,6 +439,7 @@ int firex_config_from_tier(firex_config* c, const char* tier) {
       c->time_step = t.time_step;
       c->maximum_substeps = 1;
       c->cinematic = t.cinematic;
+      c->particle_count = t.particles;
       return FIREX_OK;
     }
   }

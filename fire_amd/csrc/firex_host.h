@@ -35,7 +35,15 @@ double* params_field(Params& p, const char* name);  // numeric keys only
 
 // writeUniforms, FireXEngine.ts:2043-2113.
 void write_uniforms(const Params& p, const int32_t dims[3], double dt, double simulation_time, double frame_index,
-                    double time_step, bool cinematic, float out[FXU_COUNT]);
+                    double time_step, bool cinematic, float out[FXU_COUNT], int32_t particle_count = 0);
+
+// waterSimulationActive, FireXEngine.ts:2537-2539.
+inline bool water_active(const Params& p) { return p.waterEnabled && p.waterFlow > 0.001; }
+// writeParticleInitialState, FireXEngine.ts:1976-2034: count records of 8
+// floats (positionAge, velocityMass) into out.
+void particle_initial_state(const Params& p, int32_t count, bool active, float* out);
+// The droplet passes' uniform slots.
+FxDroplets make_droplets(const float u[FXU_COUNT], const int32_t dims[3]);
 
 struct EmitterConfig {
   int32_t count = 0;
@@ -61,6 +69,8 @@ struct FixedStepClock {
 // Bytes this library allocates on the device for one plume.
 uint64_t device_bytes(const int32_t dims[3], int32_t species_storage, bool f16_pressure, bool compact_f32,
                       bool with_liquid);
+// Particle buffers (two of 32 B records, one of 16 B density records).
+uint64_t particle_device_bytes(int32_t count);
 bool heavy_grid(const int32_t dims[3]);
 constexpr uint64_t kHeavyCellThreshold = 64ull * 1024 * 1024;  // gridConfiguration.ts:105
 // FP64 reduction scratch: 1024 block partials plus 8 results.

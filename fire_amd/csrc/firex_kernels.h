@@ -42,6 +42,13 @@ void launch_reduce_residual(int f16, const void* p, const float* div, const FxSt
                             hipStream_t s);
 void launch_reduce_flow_divergence(const float4* flow, const FxStep& st, double* partial, double* out, hipStream_t s);
 
+// SPH droplet passes (one wave per particle for the neighbour sums).
+void launch_sph_density(const float4* particles, float4* density, const FxDroplets& d, hipStream_t s);
+void launch_sph_integrate(const float4* particles_in, float4* particles_out, const float4* density,
+                          const float4* flow, const FxDroplets& d, hipStream_t s);
+void launch_liquid_deposit(const float4* current, const float4* previous, uint32_t* liquid, const FxDroplets& d,
+                           hipStream_t s);
+
 constexpr uint32_t kReducePartials = 1024;
 
 }  // namespace fx

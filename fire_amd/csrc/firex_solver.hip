@@ -42,6 +42,16 @@ struct firex_solver {
   double* scratch = nullptr;
   float* simtab = nullptr;  // simulate's per-step coordinate tables
   void* packed = nullptr;   // render volumes of firex_pack_fields (24 B/cell), on first use
+  // SPH droplets: two particle buffers (2 x float4 per particle), density
+  // records, a pinned staging buffer for the host-written initial state and
+  // the event that marks its last upload.
+  float4* particles[2] = {nullptr, nullptr};
+  float4* particle_density = nullptr;
+  float* particle_host = nullptr;
+  hipEvent_t particle_upload = nullptr;
+  int particle_index = 0;
+  bool liquid_may_contain_water = false;  // FireXEngine.ts:399
+  bool particle_field_active = false;     // FireXEngine.ts:400
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -102,6 +112,10 @@ int check_launch(firex_solver* s, const char* what) {
 void free_all(firex_solver* s) {
   for (void* p : s->allocations) hipFree(p);
   s->allocations.clear();
+  if (s->particle_host) hipHostFree(s->particle_host);
+  s->particle_host = nullptr;
+  if (s->particle_upload) hipEventDestroy(s->particle_upload);
+  s->particle_upload = nullptr;
 }
 
 int alloc(firex_solver* s, void** p, size_t bytes, const char* what) {
@@ -116,8 +130,42 @@ int alloc(firex_solver* s, void** p, size_t bytes, const char* what) {
   return FIREX_OK;
 }
 
+// writeParticleInitialState, FireXEngine.ts:1976-2034: the host computes the
+// state (float64, stored as f32) and uploads it to both particle buffers.
+int write_particle_state(firex_solver* s, bool active) {
+  const int32_t n = s->cfg.particle_count;
+  if (n > 0) {
+    // the staging buffer is rewritten only after its last upload finished
+    FX_HIP(s, hipEventSynchronize(s->particle_upload), "particle state upload");
+    particle_initial_state(s->params, n, active, s->particle_host);
+    const size_t bytes = (size_t)n * 32;
+    for (int k = 0; k < 2; ++k)
+      FX_HIP(s, hipMemcpyAsync(s->particles[k], s->particle_host, bytes, hipMemcpyHostToDevice, s->stream),
+             "particle state upload");
+    FX_HIP(s, hipEventRecord(s->particle_upload, s->stream), "particle state upload");
+  }
+  s->particle_index = 0;
+  s->particle_field_active = active;
+  return FIREX_OK;
+}
+
+// The liquid field is allocated on first use (droplets or a write).
+int ensure_liquid(firex_solver* s) {
+  if (s->liquid) return FIREX_OK;
+  const uint64_t bytes = (uint64_t)s->cells * 4;
+  if (s->cfg.memory_budget_bytes && s->bytes + bytes > s->cfg.memory_budget_bytes)
+    return set_error(FIREX_ERR_BUDGET, "the liquid field (" + std::to_string(bytes) +
+                                           " bytes) does not fit the memory budget");
+  int rc = alloc(s, (void**)&s->liquid, bytes, "liquid");
+  if (rc) return rc;
+  s->bytes += bytes;
+  launch_fill_u32(s->liquid, s->cells, 0, s->stream);
+  return FIREX_OK;
+}
+
 // initializeFields, FireXEngine.ts:1933-1974: clear every buffer, then
-// flow[0] = (0, 0, 0, 300), species[0] = (0, oxygenRate, 0, 0).
+// flow[0] = (0, 0, 0, 300), species[0] = (0, oxygenRate, 0, 0), and write the
+// particle state for the current water setting.
 int initialize_fields(firex_solver* s) {
   const uint64_t c = s->cells;
   size_t pw = s->f16p ? 2 : 4;
@@ -130,6 +178,10 @@ int initialize_fields(firex_solver* s) {
   launch_fill_u32(s->divergence, c, 0, s->stream);
   launch_fill_u32(s->reaction, c, 0, s->stream);
   if (s->liquid) launch_fill_u32(s->liquid, c, 0, s->stream);
+  if (s->particle_density) launch_fill_u32(s->particle_density, (uint64_t)s->cfg.particle_count * 4, 0, s->stream);
+  int prc = write_particle_state(s, water_active(s->params));
+  if (prc) return prc;
+  s->liquid_may_contain_water = false;
   s->flow_a = 0;
   s->corr_index = 0;
   launch_initialize(s->cfg.species_storage, s->flow[0], s->species[0], s->cells, 300.0f, (float)s->params.oxygenRate,
@@ -349,9 +401,45 @@ bool launch_rest_graph(firex_solver* s, const FxStep& st) {
   return hipGraphLaunch(g.exec, s->stream) == hipSuccess;
 }
 
+// The water branch at the head of encodeSimulationStep
+// (FireXEngine.ts:2158-2171): SPH density, integration into the other
+// particle buffer and deposition into the cleared liquid field while water is
+// on; once it is off, one clear of the liquid field and an inactive particle
+// state.
+int droplet_passes(firex_solver* s, const float u[FXU_COUNT]) {
+  if (water_active(s->params)) {
+    int rc = s->particle_field_active ? FIREX_OK : write_particle_state(s, true);
+    if (!rc) rc = ensure_liquid(s);
+    if (rc) return rc;
+    fill(s, s->liquid, s->cells);
+    const FxDroplets d = make_droplets(u, s->dims);
+    const int pi = s->particle_index;
+    timed(
+        s, FIREX_PASS_DROPLETS,
+        [&] {
+          launch_sph_density(s->particles[pi], s->particle_density, d, s->stream);
+          launch_sph_integrate(s->particles[pi], s->particles[1 - pi], s->particle_density, flow0(s), d, s->stream);
+          launch_liquid_deposit(s->particles[1 - pi], s->particles[pi], s->liquid, d, s->stream);
+        },
+        3);
+    s->particle_index = 1 - pi;
+    s->liquid_may_contain_water = true;
+    s->has_liquid = true;
+  } else if (s->liquid_may_contain_water || s->particle_field_active) {
+    if (s->liquid) fill(s, s->liquid, s->cells);
+    int rc = write_particle_state(s, false);
+    if (rc) return rc;
+    s->liquid_may_contain_water = false;
+    s->has_liquid = false;
+  }
+  return FIREX_OK;
+}
+
 // One solver step with the given uniform block (encodeSimulationStep,
 // FireXEngine.ts:2158-2275).
 int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
+  int drc = droplet_passes(s, u);
+  if (drc) return drc;
   const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
   encode_simulate(s, st);
   if (s->cfg.capture_graphs && !s->timing) {
@@ -376,7 +464,7 @@ int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
 int step_with_dt(firex_solver* s, double dt) {
   float u[FXU_COUNT];
   write_uniforms(s->params, s->dims, dt, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
-                 s->cfg.cinematic != 0, u);
+                 s->cfg.cinematic != 0, u, s->cfg.particle_count);
   int rc = encode_step(s, u);
   if (rc) return rc;
   s->simulation_time += dt;
@@ -417,12 +505,15 @@ int solver_plan(const firex_config* cfg, bool* heavy_out, bool* f16p_out, bool* 
   if (cfg->species_storage < 0 || cfg->species_storage > 2) return set_error(FIREX_ERR_INVALID, "unknown species storage");
   if (cfg->heavy_mode < 0 || cfg->heavy_mode > 3) return set_error(FIREX_ERR_INVALID, "unknown heavy mode");
   if (cfg->plan < 0 || cfg->plan > 1) return set_error(FIREX_ERR_INVALID, "unknown plan");
+  if (cfg->particle_count < 0 || cfg->particle_count > (1 << 20))
+    return set_error(FIREX_ERR_INVALID, "particle count must be in [0, 2^20]");
 
   bool heavy = cfg->heavy_mode == FIREX_HEAVY_AUTO ? heavy_grid(cfg->dims)
                                                      : (cfg->heavy_mode == FIREX_HEAVY_F16 || cfg->heavy_mode == FIREX_HEAVY_F32);
   bool f16p = heavy && cfg->heavy_mode != FIREX_HEAVY_F32;
   bool compact = heavy && !f16p;
-  uint64_t need = device_bytes(cfg->dims, cfg->species_storage, f16p, compact, false);
+  uint64_t need = device_bytes(cfg->dims, cfg->species_storage, f16p, compact, false) +
+                  particle_device_bytes(cfg->particle_count);
   if (cfg->memory_budget_bytes && need > cfg->memory_budget_bytes)
     return set_error(FIREX_ERR_BUDGET, "Transactional rebuild peak requires " + std::to_string(need) +
                                            " bytes, above the selected " + std::to_string(cfg->memory_budget_bytes) +
@@ -491,6 +582,21 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
   if (!rc) rc = alloc(s, (void**)&s->reaction, c * 4, "reaction");
   if (!rc) rc = alloc(s, (void**)&s->scratch, kReduceScratchBytes, "reduction scratch");
   if (!rc) rc = alloc(s, (void**)&s->simtab, sim_table_bytes(s->dims), "simulate tables");
+  if (!rc && cfg->particle_count > 0) {
+    const size_t n = (size_t)cfg->particle_count;
+    rc = alloc(s, (void**)&s->particles[0], n * 32, "particles");
+    if (!rc) rc = alloc(s, (void**)&s->particles[1], n * 32, "particles");
+    if (!rc) rc = alloc(s, (void**)&s->particle_density, n * 16, "particle density");
+    if (!rc && hipHostMalloc((void**)&s->particle_host, n * 32, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      s->particle_host = nullptr;
+      rc = set_error(FIREX_ERR_OUT_OF_MEMORY, "pinned particle staging buffer");
+    }
+    if (!rc) {
+      e = hipEventCreateWithFlags(&s->particle_upload, hipEventDisableTiming);
+      if (e != hipSuccess) rc = hip_fail(s, e, "particle upload event");
+    }
+  }
   if (!rc) {
     if (cfg->jacobi_autotune && cfg->plan == FIREX_PLAN_FUSED) {
       // Times candidate plans on the solver's own pressure and divergence
@@ -648,7 +754,7 @@ int firex_frame(firex_solver* s, double delta, int32_t* steps_taken) {
   // One uniform write covers every step of the frame (FireXEngine.ts:1089-1097).
   float u[FXU_COUNT];
   write_uniforms(s->params, s->dims, s->cfg.time_step, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
-                 s->cfg.cinematic != 0, u);
+                 s->cfg.cinematic != 0, u, s->cfg.particle_count);
   for (int32_t i = 0; i < n; ++i) {
     rc = encode_step(s, u);
     if (rc) return rc;
@@ -671,11 +777,21 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
   if (rc) return rc;
   if (!dst) return set_error(FIREX_ERR_INVALID, "dst is null");
   hipSetDevice(s->cfg.device);
-  const uint64_t c = s->cells;
+  uint64_t c = s->cells;  // records: cells, or particles for the per-particle fields
   const void* src = nullptr;
   int comps = 1;
   int kind = 0;  // 0 f32, 1 f16, 2 bf16, 3 u32
   switch (field) {
+    case FIREX_FIELD_PARTICLES:
+      src = s->particles[s->particle_index];
+      c = (uint64_t)s->cfg.particle_count;
+      comps = 8;
+      break;
+    case FIREX_FIELD_PARTICLE_DENSITY:
+      src = s->particle_density;
+      c = (uint64_t)s->cfg.particle_count;
+      comps = 4;
+      break;
     case FIREX_FIELD_FLOW: src = flow0(s); comps = 4; break;
     case FIREX_FIELD_FLOW_PREDICTED:
       // After a step the reference's flow[1] equals flow[0] (copyFlow); the
@@ -715,6 +831,7 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
     return set_error(FIREX_ERR_INVALID, "dtype does not match the field");
   if (bytes != n * elem) return set_error(FIREX_ERR_INVALID, "byte count does not match the field size");
   FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before read");
+  if (n == 0) return FIREX_OK;
   if (kind == 3 && !src) {
     memset(dst, 0, bytes);  // no liquid field was ever written: it is zero
     return FIREX_OK;
@@ -745,10 +862,17 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
   if (rc) return rc;
   if (!srcp) return set_error(FIREX_ERR_INVALID, "src is null");
   hipSetDevice(s->cfg.device);
-  const uint64_t c = s->cells;
+  uint64_t c = s->cells;
   void* dst = nullptr;
   int comps = 1, kind = 0;
   switch (field) {
+    case FIREX_FIELD_PARTICLES:
+      // both buffers, as writeParticleInitialState writes them; the written
+      // state is the live one (the next water step advances it)
+      dst = s->particles[0];
+      c = (uint64_t)s->cfg.particle_count;
+      comps = 8;
+      break;
     case FIREX_FIELD_FLOW: dst = flow0(s); comps = 4; break;
     case FIREX_FIELD_FLOW_PREDICTED: dst = flow1(s); comps = 4; break;
     case FIREX_FIELD_SPECIES: dst = s->species[s->species_index]; comps = 4; kind = s->cfg.species_storage; break;
@@ -757,11 +881,8 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
     case FIREX_FIELD_REACTION: dst = s->reaction; break;
     case FIREX_FIELD_LIQUID:
       kind = 3;
-      if (!s->liquid) {
-        rc = alloc(s, (void**)&s->liquid, c * 4, "liquid");
-        if (rc) return rc;
-        s->bytes += c * 4;
-      }
+      rc = ensure_liquid(s);
+      if (rc) return rc;
       dst = s->liquid;
       s->has_liquid = true;
       break;
@@ -785,7 +906,13 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
     else ((uint32_t*)staging.data())[i] = dtype == FIREX_DTYPE_U32 ? ((const uint32_t*)srcp)[i] : (uint32_t)v;
   }
   FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before write");
+  if (n == 0) return FIREX_OK;
   FX_HIP(s, hipMemcpy(dst, staging.data(), sbytes, hipMemcpyHostToDevice), "field upload");
+  if (field == FIREX_FIELD_PARTICLES) {
+    FX_HIP(s, hipMemcpy(s->particles[1], staging.data(), sbytes, hipMemcpyHostToDevice), "field upload");
+    s->particle_index = 0;
+    s->particle_field_active = true;
+  }
   return FIREX_OK;
 }
 
@@ -804,7 +931,7 @@ int firex_pack_fields(firex_solver* s) {
   }
   float u[FXU_COUNT];
   write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
-                 s->cfg.cinematic != 0, u);
+                 s->cfg.cinematic != 0, u, s->cfg.particle_count);
   const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
   // u32(clamp(round(controls.x), 0, 7)) == 7u, qualityShaders.ts:55
   const float view = fxm_clamp(nearbyintf(u[FXU_VIEW_MODE]), 0.0f, 7.0f);
@@ -825,7 +952,7 @@ int firex_reduce(firex_solver* s, int32_t which, double* out) {
   hipSetDevice(s->cfg.device);
   float u[FXU_COUNT];
   write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
-                 s->cfg.cinematic != 0, u);
+                 s->cfg.cinematic != 0, u, s->cfg.particle_count);
   const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
   double* partial = s->scratch;
   double* result = s->scratch + kReducePartials;
@@ -902,6 +1029,7 @@ int firex_get_info(firex_solver* s, firex_info* info) {
   info->jacobi_chunk_z = s->jacobi.chunk_z;
   info->jacobi_prefetch = s->jacobi.prefetch;
   info->jacobi_tuned_us_per_sweep = s->jacobi_tuned_us;
+  info->particle_count = s->cfg.particle_count;
   return FIREX_OK;
 }
 

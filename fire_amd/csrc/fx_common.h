@@ -69,6 +69,19 @@ struct FxStep {
   float wx, wy, wz, jacobi_inv;        // metric^2, 1 / (2 (wx + wy + wz))
 };
 
+// Per-step constants of the droplet passes (uniform slots of
+// FIREX_PARTICLE_WGSL, shaders.ts:514-530); every derived value is computed
+// per thread, in WGSL order.
+struct FxDroplets {
+  uint32_t count;              // u32(params.render.w)
+  int32_t nx, ny, nz;
+  float dt, frame;             // params.time.x, params.time.z
+  float gx, gy, gz;            // params.grid.xyz
+  float ex, ey, ez;            // domainExtent() = max(domain, 0.001)
+  float water_flow, spray_angle, water_enabled;  // params.water.xyz
+  float aim, nozzle;           // params.controls.yz
+};
+
 #if defined(__HIP__)
 // Keeps an f32 value as computed.  Without it the compiler may fold
 // "f32 multiply, then convert to f16" into one mixed-precision FMA that

@@ -4,7 +4,9 @@
 
 export type FireXFieldName =
   | 'flow' | 'species' | 'flowPredicted' | 'pressure' | 'correction' | 'divergence' | 'reaction' | 'liquid'
-  | 'packedFlow' | 'packedChemistry' | 'packedMedia';
+  | 'packedFlow' | 'packedChemistry' | 'packedMedia'
+  /** per SPH droplet: 8 (positionAge, velocityMass) and 4 (density, near density, pressures) values */
+  | 'particles' | 'particleDensity';
 
 export type FireXReductionName =
   | 'sumFuel' | 'sumOxygen' | 'sumProducts' | 'sumVapor' | 'sumTemperature' | 'sumReaction'
@@ -28,6 +30,8 @@ export interface FireXHipSolverOptions {
   jacobiAutotune?: number;
   /** 1: replay everything after simulate as a captured hipGraph. */
   captureGraphs?: number;
+  /** SPH droplets of the resource set (384-4096 by tier, FireXEngine.ts:187-277). */
+  particleCount?: number;
   scene?: Readonly<Record<string, number | boolean>>;
 }
 

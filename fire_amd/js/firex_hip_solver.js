@@ -21,6 +21,8 @@ const FIELD_COMPONENTS = {
   flow: 4, species: 4, flowPredicted: 4, pressure: 1, correction: 1, divergence: 1, reaction: 1, liquid: 1,
   packedFlow: 4, packedChemistry: 4, packedMedia: 4,
 };
+// per-particle fields (records per SPH droplet, not per cell)
+const PARTICLE_COMPONENTS = { particles: 8, particleDensity: 4 };
 const REDUCTIONS = {
   sumFuel: 0, sumOxygen: 1, sumProducts: 2, sumVapor: 3, sumTemperature: 4, sumReaction: 5,
   residualL2: 6, flowDivergenceL2: 7,
@@ -38,6 +40,7 @@ class FireXHipSolver {
    *   dims, pressureIterations, correctionIterations, timeStep,
    *   maximumSubsteps, speciesStorage ('f32' | 'f16' | 'bf16'),
    *   heavyMode, plan ('reference' | 'fused'), device, memoryBudgetBytes,
+   *   particleCount (SPH droplets; the tier's count by default),
    *   scene (a { key: value } map applied with setParameter).
    */
   constructor(options = {}) {
@@ -111,9 +114,10 @@ class FireXHipSolver {
   /** Copies a field to a new (or the given) Float32Array / Float64Array / Uint32Array. */
   readField(name, out) {
     const info = native.getInfo(this.requireHandle());
-    const components = FIELD_COMPONENTS[name];
+    const perParticle = PARTICLE_COMPONENTS[name];
+    const components = perParticle || FIELD_COMPONENTS[name];
     if (!components) throw new RangeError(`Unknown Fire-X field: ${name}`);
-    const length = info.cellCount * components;
+    const length = (perParticle ? info.particleCount : info.cellCount) * components;
     const target = out || (name === 'liquid' ? new Uint32Array(length) : new Float32Array(length));
     return native.readField(this.handle, name, target);
   }

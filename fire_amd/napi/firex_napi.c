@@ -124,7 +124,7 @@ static int str_prop(napi_env env, napi_value obj, const char* key, char* buf, si
 /* create({tier, dims:[x,y,z], pressureIterations, correctionIterations,
  *         timeStep, maximumSubsteps, speciesStorage, heavyMode, plan,
  *         cinematic, device, memoryBudgetBytes, jacobiSweepsPerLaunch,
- *         captureGraphs, jacobiAutotune}) */
+ *         captureGraphs, jacobiAutotune, particleCount}) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -160,6 +160,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.jacobi_sweeps_per_launch = (int32_t)num_or(env, argv[0], "jacobiSweepsPerLaunch", 0);
     cfg.capture_graphs = (int32_t)num_or(env, argv[0], "captureGraphs", 0);
     cfg.jacobi_autotune = (int32_t)num_or(env, argv[0], "jacobiAutotune", 0);
+    cfg.particle_count = (int32_t)num_or(env, argv[0], "particleCount", cfg.particle_count);
     if (str_prop(env, argv[0], "speciesStorage", buf, sizeof buf)) cfg.species_storage = lookup(buf, storage_names, 3);
     if (str_prop(env, argv[0], "heavyMode", buf, sizeof buf)) cfg.heavy_mode = lookup(buf, heavy_names, 4);
     if (str_prop(env, argv[0], "plan", buf, sizeof buf)) cfg.plan = lookup(buf, plan_names, 2);
@@ -332,7 +333,8 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
 static int field_id(const char* name) {
   static const char* names[] = {"flow",          "species",    "pressure",        "correction",
                                 "divergence",    "reaction",   "liquid",          "flowPredicted",
-                                "packedFlow",    "packedChemistry", "packedMedia"};
+                                "packedFlow",    "packedChemistry", "packedMedia",
+                                "particles",     "particleDensity"};
   for (int i = 0; i < (int)(sizeof names / sizeof names[0]); ++i)
     if (strcmp(name, names[i]) == 0) return i;
   return -1;
@@ -442,6 +444,7 @@ static napi_value js_get_info(napi_env env, napi_callback_info info) {
   set_num(env, o, "deviceBytes", (double)fi.device_bytes);
   set_num(env, o, "jacobiSweepsPerLaunch", fi.jacobi_sweeps_per_launch);
   set_num(env, o, "jacobiChunkZ", fi.jacobi_chunk_z);
+  set_num(env, o, "particleCount", fi.particle_count);
   return o;
 }
 

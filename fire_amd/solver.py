@@ -36,8 +36,11 @@ FIELDS = {
     "flow": 0, "species": 1, "pressure": 2, "correction": 3,
     "divergence": 4, "reaction": 5, "liquid": 6, "flow_predicted": 7,
     "packed_flow": 8, "packed_chemistry": 9, "packed_media": 10,
+    "particles": 11, "particle_density": 12,
 }
 VECTOR_FIELDS = {"flow", "species", "flow_predicted", "packed_flow", "packed_chemistry", "packed_media"}
+# per-particle fields: components per particle record
+PARTICLE_FIELDS = {"particles": 8, "particle_density": 4}
 DTYPE_F32, DTYPE_F64, DTYPE_U32 = 0, 1, 2
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 HEAVY = {"auto": 0, "off": 1, "f16": 2, "f32": 3}
@@ -86,7 +89,8 @@ class Config(ctypes.Structure):
         ("jacobi_chunk_z", ctypes.c_int32),
         ("jacobi_kernel", ctypes.c_int32),
         ("jacobi_autotune", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("particle_count", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -112,7 +116,7 @@ class Info(ctypes.Structure):
         ("jacobi_chunk_z", ctypes.c_int32),
         ("jacobi_prefetch", ctypes.c_int32),
         ("jacobi_tuned_us_per_sweep", ctypes.c_float),
-        ("reserved", ctypes.c_int32),
+        ("particle_count", ctypes.c_int32),
     ]
 
 
@@ -126,7 +130,7 @@ class DeviceLimits(ctypes.Structure):
         "max_texture_dimension_3d", "max_storage_buffer_binding_size", "max_buffer_size")]
 
 
-PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack")
+PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack", "droplets")
 
 
 class PassTiming(ctypes.Structure):
@@ -323,7 +327,8 @@ class FireXHipSolver:
                  species_storage: str = "f32", heavy_mode: str = "auto", plan: str = "reference",
                  cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
                  jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0,
-                 jacobi_kernel: str = "auto", jacobi_autotune: bool = False):
+                 jacobi_kernel: str = "auto", jacobi_autotune: bool = False,
+                 particle_count: Optional[int] = None):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -349,6 +354,8 @@ class FireXHipSolver:
         cfg.jacobi_chunk_z = int(jacobi_chunk_z)
         cfg.jacobi_kernel = {"auto": 0, "shift": 1, "ring": 2}[jacobi_kernel]
         cfg.jacobi_autotune = int(bool(jacobi_autotune))
+        if particle_count is not None:
+            cfg.particle_count = int(particle_count)
         self.config = cfg
         self.dims = tuple(int(v) for v in cfg.dims)
         self.cells = self.dims[0] * self.dims[1] * self.dims[2]
@@ -417,6 +424,8 @@ class FireXHipSolver:
 
     # fields -----------------------------------------------------------------
     def _shape(self, name: str):
+        if name in PARTICLE_FIELDS:
+            return (int(self.config.particle_count), PARTICLE_FIELDS[name])
         nx, ny, nz = self.dims
         return (nz, ny, nx, 4) if name in VECTOR_FIELDS else (nz, ny, nx)
 
@@ -476,6 +485,7 @@ class FireXHipSolver:
                        "sweeps_per_launch": inf.jacobi_sweeps_per_launch, "chunk_z": inf.jacobi_chunk_z,
                        "prefetch": inf.jacobi_prefetch,
                        "tuned_us_per_sweep": round(float(inf.jacobi_tuned_us_per_sweep), 3)},
+            "particle_count": inf.particle_count,
         }
 
 

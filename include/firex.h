@@ -4,7 +4,8 @@
  * The library is a drop-in for the compute half of the reference engine
  * (class FireXEngine, src/sim/firex/FireXEngine.ts:370): it owns every device
  * buffer of one plume, advances it with the Fire-X fixed step and hands fields
- * back to the caller.  Rendering, droplets and video export are not part of it.
+ * back to the caller, including the SPH droplet passes that feed its liquid
+ * field.  Rendering and video export are not part of it.
  *
  * Conventions
  *   - Every function returning int returns FIREX_OK (0) or a FIREX_ERR_* code;
@@ -55,7 +56,11 @@ enum {
   /* render volumes written by firex_pack_fields (4 x f16 per cell; read as f32/f64) */
   FIREX_FIELD_PACKED_FLOW = 8,     /* (u, v, w, clamp((T - 300) / 2600, 0, 1))            */
   FIREX_FIELD_PACKED_CHEMISTRY = 9,/* max(species, 0)                                     */
-  FIREX_FIELD_PACKED_MEDIA = 10    /* (reaction, soot, liquid, display value)             */
+  FIREX_FIELD_PACKED_MEDIA = 10,   /* (reaction, soot, liquid, display value)             */
+  /* per-particle fields: particle_count records (not cells) */
+  FIREX_FIELD_PARTICLES = 11,      /* 8 f32: positionAge (domain fractions, age s),
+                                      velocityMass (m/s, mass); the current buffer       */
+  FIREX_FIELD_PARTICLE_DENSITY = 12 /* 4 f32: density, near density, pressure, near p.   */
 };
 
 enum { FIREX_DTYPE_F32 = 0, FIREX_DTYPE_F64 = 1, FIREX_DTYPE_U32 = 2 };
@@ -99,7 +104,9 @@ typedef struct firex_config {
   int32_t jacobi_autotune;       /* FUSED plan: 1 = time candidate Jacobi plans on the
                                     device at creation and keep the fastest; fields
                                     above left at 0 are searched (cached per shape)  */
-  int32_t reserved[3];
+  int32_t particle_count;        /* SPH droplets of the resource set (384-4096 by
+                                    tier, FireXEngine.ts:187-277); 0 = none         */
+  int32_t reserved[2];
 } firex_config;
 
 /* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */
@@ -126,7 +133,7 @@ typedef struct firex_info {
   int32_t jacobi_chunk_z;
   int32_t jacobi_prefetch;
   float jacobi_tuned_us_per_sweep; /* autotuner's time per sweep; 0 = untuned */
-  int32_t reserved;
+  int32_t particle_count;       /* SPH droplets (firex_config.particle_count) */
 } firex_info;
 
 typedef struct firex_memory_estimate {
@@ -241,9 +248,12 @@ int firex_sync(firex_solver* solver);
 
 /* ---- fields -------------------------------------------------------------- */
 /* Copy a field to host memory (waits for enqueued steps).  bytes must equal
- * cells * components * sizeof(dtype). */
+ * cells * components * sizeof(dtype), or particle_count * components *
+ * sizeof(dtype) for the per-particle fields. */
 int firex_read_field(firex_solver* solver, int32_t field, void* dst, size_t bytes, int32_t dtype);
-/* Overwrite a field from host memory (seeding and tests). */
+/* Overwrite a field from host memory (seeding and tests).  Writing
+ * FIREX_FIELD_PARTICLES fills both particle buffers, as
+ * writeParticleInitialState does (FireXEngine.ts:2030-2031). */
 int firex_write_field(firex_solver* solver, int32_t field, const void* src, size_t bytes, int32_t dtype);
 /* packFields (src/sim/firex/qualityShaders.ts:37-73) as encodeRender runs it
  * (FireXEngine.ts:2585-2605): fills the three render volumes (flow,
@@ -267,6 +277,7 @@ enum {
   FIREX_PASS_PROJECT_DIVERGENCE = 4,  /* no longer launched (kept for the layout) */
   FIREX_PASS_FILL_COPY = 5,           /* clears and copyFlow                      */
   FIREX_PASS_PACK = 6,                /* firex_pack_fields                        */
+  FIREX_PASS_DROPLETS = 7,            /* SPH density, integration, deposition     */
   FIREX_PASS_COUNT = 8
 };
 typedef struct firex_pass_timing {

@@ -22,6 +22,9 @@
 //   * field reset (initializeFields host side):  FireXEngine.ts:1933-1974
 //   * pass order / ping-pong / warm start:       FireXEngine.ts:2115-2276
 //   * parameter defaults and clamps:             FireXEngine.ts:422-451, 937-1032
+//   * SPH droplets (density, integration, liquid deposition) and the host
+//     particle state:                            shaders.ts:514-825,
+//                                                FireXEngine.ts:1976-2034, 2158-2171
 //
 // Two precisions are instantiated.  State<double> is the FP64 judge: every
 // operation in double, libm transcendentals, true division.  State<float>
@@ -168,6 +171,7 @@ bool set_parameter(Params& p, const char* name, double v) {
   else if (n == "waterFlow") p.waterFlow = finite_or(v, p.waterFlow);
   else if (n == "sprayAngle") p.sprayAngle = finite_or(v, p.sprayAngle);
   else if (n == "aimHeight") p.aimHeight = clampd(finite_or(v, p.aimHeight), 0.08, 0.82);
+  else if (n == "nozzleType") p.nozzleType = finite_or(v, p.nozzleType) >= 0.5 ? 1 : 0;  // FireXEngine.ts:1000-1001
   else if (n == "domainWidth") p.domainWidth = clampd(finite_or(v, p.domainWidth), 0.5, 4);
   else if (n == "domainHeight") p.domainHeight = clampd(finite_or(v, p.domainHeight), 0.5, 6);
   else if (n == "domainDepth") p.domainDepth = clampd(finite_or(v, p.domainDepth), 0.5, 4);
@@ -244,6 +248,7 @@ template <typename T> struct State {
     u[12] = (float)params.waterFlow; u[13] = (float)(params.sprayAngle * M_PI / 180.0);
     u[14] = params.waterEnabled ? 1.0f : 0.0f; u[15] = 300.0f;
     u[16] = (float)params.cameraYaw; u[17] = (float)params.cameraPitch; u[18] = (float)params.cameraDistance;
+    u[23] = (float)particleCount;
     u[24] = (float)params.viewMode; u[25] = (float)params.aimHeight; u[26] = (float)params.nozzleType;
     u[27] = (float)params.exposure;
     u[28] = (float)params.burnerSize; u[29] = (float)params.burnerDepth;
@@ -344,6 +349,9 @@ template <typename T> struct State {
       flow[0][i] = V4<T>{0, 0, 0, T(300)};
       species[0][i] = s0;
     }
+    particleDensity.assign((size_t)particleCount * 4, T(0));
+    write_particle_initial_state(water_active());
+    liquidMayContainWater = false;
   }
 
   void simulate(int sIn) {
@@ -532,6 +540,368 @@ template <typename T> struct State {
     }
   }
 
+  // ------------------------------------------------------------- droplets
+  // The SPH droplet passes of FIREX_PARTICLE_WGSL (shaders.ts:514-825) and
+  // the host particle state (FireXEngine.ts:1976-2034).  Particle records are
+  // 8 values (positionAge: position in domain fractions, age; velocityMass:
+  // velocity in m/s, mass); density records 4 (density, near density,
+  // pressure, near pressure).
+  int particleCount = 0;
+  std::vector<T> particles[2];
+  std::vector<T> particleDensity;
+  int particleIndex = 0;
+  bool liquidMayContainWater = false, particleFieldActive = false;
+
+  // waterSimulationActive, FireXEngine.ts:2537-2539.
+  bool water_active() const { return params.waterEnabled && params.waterFlow > 0.001; }
+
+  // Math.hypot as the JS engine evaluates it: scale by the largest magnitude,
+  // Kahan-compensated sum of squares, sqrt, rescale.
+  static double js_hypot(const double* v, int n) {
+    double mx = 0;
+    for (int i = 0; i < n; ++i) {
+      double a = std::fabs(v[i]);
+      if (std::isinf(a)) return INFINITY;
+      if (std::isnan(a)) mx = NAN;
+      else if (a > mx) mx = a;
+    }
+    if (std::isnan(mx)) return NAN;
+    if (mx == 0) return 0;
+    double sum = 0, comp = 0;
+    for (int i = 0; i < n; ++i) {
+      double r = std::fabs(v[i]) / mx;
+      double summand = r * r - comp;
+      double prelim = sum + summand;
+      comp = (prelim - sum) - summand;
+      sum = prelim;
+    }
+    return std::sqrt(sum) * mx;
+  }
+
+  // writeParticleInitialState, FireXEngine.ts:1976-2034: float64 arithmetic,
+  // stored through a Float32Array.
+  void write_particle_initial_state(bool active) {
+    const int N = particleCount;
+    std::vector<float> init((size_t)N * 8, 0.0f);
+    const double W = params.domainWidth, H = params.domainHeight, D = params.domainDepth;
+    const double aim[2] = {(0.52 - 0.035) * W, (params.aimHeight - 0.5) * H};
+    const double aimLength = js_hypot(aim, 2);
+    const double base[3] = {aim[0] / aimLength, aim[1] / aimLength, 0};
+    const double tangent[2] = {base[1], -base[0]};
+    auto hash = [](double v) {
+      double h = std::sin(v * 12.9898 + 1.317) * 43758.5453;
+      return h - std::floor(h);
+    };
+    const bool spray = params.nozzleType >= 0.5;
+    for (int i = 0; i < N; ++i) {
+      const double phase = (double)i / (double)N;
+      const double angle = hash(i * 3.771 + 4.91) * M_PI * 2;
+      const double radial = std::sqrt(hash(i * 3.771 + 0.17));
+      const double spread = spray ? 0.72 : 0.1;
+      const double cone = std::tan((params.sprayAngle * M_PI) / 180) * radial * spread;
+      const double diskX = tangent[0] * std::cos(angle);
+      const double diskY = tangent[1] * std::cos(angle);
+      const double diskZ = -std::sin(angle);
+      double d[3] = {base[0] + diskX * cone, base[1] + diskY * cone, diskZ * cone};
+      const double len = js_hypot(d, 3);
+      d[0] /= len;
+      d[1] /= len;
+      d[2] /= len;
+      const double variation = spray ? 0.24 : 0.05;
+      const double speed = (0.82 + params.waterFlow * 0.82) * (1 - variation * 0.5 + hash(i * 3.771 + 9.37) * variation);
+      const double age = phase * 0.72;
+      const double originRadius = spray ? 0.006 : 0.0028;
+      float* o = &init[(size_t)i * 8];
+      o[0] = (float)(0.035 + (diskX * radial * originRadius + d[0] * speed * age) / W);
+      o[1] = (float)std::max(0.012, 0.5 + (diskY * radial * originRadius + d[1] * speed * age - 0.29 * age * age) / H);
+      o[2] = (float)(0.5 + (diskZ * radial * originRadius + d[2] * speed * age) / D);
+      o[3] = (float)(active ? age : -phase * 0.72);
+      o[4] = (float)(active ? d[0] * speed : 0);
+      o[5] = (float)(active ? d[1] * speed - 0.58 * age : 0);
+      o[6] = (float)(active ? d[2] * speed : 0);
+      o[7] = (float)(active ? 0.52 + params.waterFlow * 0.88 : 0);
+    }
+    for (int k = 0; k < 2; ++k) {
+      particles[k].resize(init.size());
+      for (size_t j = 0; j < init.size(); ++j) particles[k][j] = T(init[j]);
+    }
+    particleIndex = 0;
+    particleFieldActive = active;
+  }
+
+  // Resource rebuild with a new particle count (installResourceSet,
+  // FireXEngine.ts:1917-1931, particle part).
+  void set_particle_count(int count) {
+    particleCount = count;
+    particleDensity.assign((size_t)count * 4, T(0));
+    liquidMayContainWater = false;
+    particleFieldActive = false;
+    write_particle_initial_state(water_active());
+  }
+
+  // hash11, shaders.ts:550-552.
+  T hash11(T v) const {
+    T h = Math<T>::sin(v * T(12.9898) + T(1.317)) * T(43758.5453);
+    return h - Math<T>::floor(h);
+  }
+  static T len3(T x, T y, T z) { return Math<T>::sqrt(x * x + y * y + z * z); }
+  bool spray_nozzle() const { return T(u[26]) >= T(0.5); }  // params.controls.z
+
+  // spawnParticle, shaders.ts:579-601.
+  void spawn(uint32_t id, T cycle, T* o) const {
+    const T seed = T((float)id) * T(3.771) + cycle * T(19.173);
+    const T radialRandom = Math<T>::sqrt(hash11(seed + T(0.17)));
+    const T angularRandom = hash11(seed + T(4.91)) * T(6.2831853);
+    const T speedRandom = hash11(seed + T(9.37));
+    // nozzleDirection(), shaders.ts:575-578
+    T dx = (T(0.52) - T(0.035)) * ex;
+    T dy = (wclamp(T(u[25]), T(0.08), T(0.82)) - T(0.50)) * ey;
+    T dz = (T(0.5) - T(0.5)) * ez;
+    T l = len3(dx, dy, dz);
+    dx = dx / l; dy = dy / l; dz = dz / l;
+    // tangent = normalize(cross(direction, (0, 0, 1)))
+    T tx = dy * T(1) - dz * T(0), ty = dz * T(0) - dx * T(1), tz = dx * T(0) - dy * T(0);
+    l = len3(tx, ty, tz);
+    tx = tx / l; ty = ty / l; tz = tz / l;
+    // bitangent = normalize(cross(direction, tangent))
+    T bx = dy * tz - dz * ty, by = dz * tx - dx * tz, bz = dx * ty - dy * tx;
+    l = len3(bx, by, bz);
+    bx = bx / l; by = by / l; bz = bz / l;
+    const T ca = Math<T>::cos(angularRandom), sa = Math<T>::sin(angularRandom);
+    const T kx = tx * ca + bx * sa, ky = ty * ca + by * sa, kz = tz * ca + bz * sa;
+    const bool spray = spray_nozzle();
+    const T nozzleSpread = spray ? T(0.72) : T(0.10);
+    const T ang = wclamp(T(u[13]), T(0.005), T(0.78));
+    const T cone = Math<T>::sin(ang) / Math<T>::cos(ang) * radialRandom * nozzleSpread;
+    T lx = dx + kx * cone, ly = dy + ky * cone, lz = dz + kz * cone;
+    l = len3(lx, ly, lz);
+    lx = lx / l; ly = ly / l; lz = lz / l;
+    const T originRadius = spray ? T(0.006) : T(0.0028);
+    const T speedVariation = spray ? T(0.24) : T(0.05);
+    const T speed = (T(0.82) + T(u[12]) * T(0.82)) *
+                    wmix(T(1) - speedVariation * T(0.5), T(1) + speedVariation * T(0.5), speedRandom);
+    o[0] = T(0.035) + kx * radialRandom * originRadius / ex;
+    o[1] = T(0.50) + ky * radialRandom * originRadius / ey;
+    o[2] = T(0.5) + kz * radialRandom * originRadius / ez;
+    o[3] = T(0);
+    o[4] = lx * speed; o[5] = ly * speed; o[6] = lz * speed;
+    o[7] = T(0.52) + T(u[12]) * T(0.88);
+  }
+
+  // computeParticleDensity, shaders.ts:603-637.
+  void sph_density(const std::vector<T>& pin) {
+    const int N = particleCount;
+    const T radius = spray_nozzle() ? T(0.060) : T(0.038);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const T* p = &pin[(size_t)i * 8];
+      T* out = &particleDensity[(size_t)i * 4];
+      if (p[7] <= T(0.0001)) {
+        out[0] = out[1] = out[2] = out[3] = T(0);
+        continue;
+      }
+      T density = T(0), nearDensity = T(0);
+      for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const T* q = &pin[(size_t)j * 8];
+        if (q[7] <= T(0.0001)) continue;
+        const T distance = len3((p[0] - q[0]) * ex, (p[1] - q[1]) * ey, (p[2] - q[2]) * ez);
+        if (distance > T(0.00001) && distance < radius) {
+          const T w = T(1) - distance / radius;
+          density = density + w * w;
+          nearDensity = nearDensity + w * w * w;
+        }
+      }
+      const T restDensity = T(2.7) + T(u[12]) * T(0.9);
+      out[0] = density;
+      out[1] = nearDensity;
+      out[2] = wmax(density - restDensity, T(0)) * T(0.055);
+      out[3] = nearDensity * T(0.065);
+    }
+  }
+
+  // integrateParticles, shaders.ts:639-735.
+  void sph_integrate(const std::vector<T>& pin, std::vector<T>& pout) {
+    const int N = particleCount;
+    const T dt = T(u[4]);
+    const bool spray = spray_nozzle();
+    const T radius = spray ? T(0.060) : T(0.038);
+    const bool respawn = T(u[14]) >= T(0.5) && T(u[12]) > T(0.001);
+#pragma omp parallel for schedule(static)
+    for (int id = 0; id < N; ++id) {
+      const T* p = &pin[(size_t)id * 8];
+      T* o = &pout[(size_t)id * 8];
+      if (dt <= T(0)) {
+        for (int k = 0; k < 8; ++k) o[k] = p[k];
+        continue;
+      }
+      if (p[7] <= T(0.0001)) {
+        const T wait = p[3] + dt;
+        if (respawn && wait >= T(0)) {
+          spawn((uint32_t)id, wait + T(u[6]), o);
+        } else {
+          o[0] = T(0.035); o[1] = T(0.50); o[2] = T(0.5);
+          o[3] = wait > T(0) ? T(-0.72) : wait;
+          o[4] = o[5] = o[6] = o[7] = T(0);
+        }
+        continue;
+      }
+      const T lifetime = T(0.88) + hash11(T((float)id) * T(2.31) + T(7.0)) * T(0.52);
+      const T nextAge = p[3] + dt;
+      const bool outside = p[0] < T(-0.03) || p[1] < T(-0.03) || p[2] < T(-0.03) ||
+                           p[0] > T(1.03) || p[1] > T(1.03) || p[2] > T(1.03);
+      if (nextAge >= lifetime || outside) {
+        if (respawn) {
+          spawn((uint32_t)id, nextAge + T(u[6]), o);
+        } else {
+          o[0] = T(0.035); o[1] = T(0.50); o[2] = T(0.5);
+          o[3] = -hash11(T((float)id) * T(8.17)) * T(0.72);
+          o[4] = o[5] = o[6] = o[7] = T(0);
+        }
+        continue;
+      }
+      T vx = p[4], vy = p[5], vz = p[6];
+      const T* pd = &particleDensity[(size_t)id * 4];
+      T pax = T(0), pay = T(0), paz = T(0), vax = T(0), vay = T(0), vaz = T(0);
+      for (int j = 0; j < N; ++j) {
+        if (j == id) continue;
+        const T* q = &pin[(size_t)j * 8];
+        if (q[7] <= T(0.0001)) continue;
+        const T sx = (p[0] - q[0]) * ex, sy = (p[1] - q[1]) * ey, sz = (p[2] - q[2]) * ez;
+        const T distance = len3(sx, sy, sz);
+        if (distance > T(0.00001) && distance < radius) {
+          const T w = T(1) - distance / radius;
+          const T dx = sx / distance, dy = sy / distance, dz = sz / distance;
+          const T* nd = &particleDensity[(size_t)j * 4];
+          const T pressure = (pd[2] + nd[2]) * w;
+          const T nearPressure = (pd[3] + nd[3]) * w * w;
+          const T push = pressure + nearPressure;
+          pax = pax + dx * push; pay = pay + dy * push; paz = paz + dz * push;
+          const T visc = w * T(0.018);
+          vax = vax + (q[4] - vx) * visc; vay = vay + (q[5] - vy) * visc; vaz = vaz + (q[6] - vz) * visc;
+        }
+      }
+      // sampleFlow, shaders.ts:566-569: flow[0] at the cell holding the position
+      const int cx = (int)(wclamp(p[0], T(0), T(0.99999)) * gx);
+      const int cy = (int)(wclamp(p[1], T(0), T(0.99999)) * gy);
+      const int cz = (int)(wclamp(p[2], T(0), T(0.99999)) * gz);
+      const V4<T> gas = flow[0][idx(cx, cy, cz)];
+      const T heat = wsmooth(T(380), T(1500), gas.w);
+      const T drag = T(0.30) + heat * T(0.18);
+      T ax = T(0) + (gas.x - vx) * drag, ay = T(-0.58) + (gas.y - vy) * drag, az = T(0) + (gas.z - vz) * drag;
+      const T pressureDispersion = spray ? T(1) : T(0.055);
+      const T viscousCohesion = spray ? T(1) : T(2.2);
+      ax = ax + (pax * pressureDispersion + vax * viscousCohesion);
+      ay = ay + (pay * pressureDispersion + vay * viscousCohesion);
+      az = az + (paz * pressureDispersion + vaz * viscousCohesion);
+      vx = vx + ax * dt; vy = vy + ay * dt; vz = vz + az * dt;
+      T nx_ = p[0] + vx * dt / ex, ny_ = p[1] + vy * dt / ey, nz_ = p[2] + vz * dt / ez;
+      if (ny_ < T(0.012)) {
+        ny_ = T(0.012);
+        vy = std::fabs(vy) * T(0.18);
+        vx = vx * T(0.72);
+        vz = vz * T(0.72);
+      }
+      if (nx_ < T(0.008) || nx_ > T(0.992)) {
+        nx_ = wclamp(nx_, T(0.008), T(0.992));
+        vx = vx * T(-0.24);
+      }
+      if (nz_ < T(0.008) || nz_ > T(0.992)) {
+        nz_ = wclamp(nz_, T(0.008), T(0.992));
+        vz = vz * T(-0.24);
+      }
+      const T remainingMass = p[7] * Math<T>::exp(-heat * dt * T(0.34));
+      o[0] = nx_; o[1] = ny_; o[2] = nz_; o[3] = nextAge;
+      o[4] = wclamp(vx, T(-2), T(2)); o[5] = wclamp(vy, T(-2), T(2)); o[6] = wclamp(vz, T(-2), T(2));
+      o[7] = remainingMass;
+    }
+  }
+
+  // depositCorner, shaders.ts:737-740 (a NaN amount deposits nothing).
+  void deposit_corner(int x, int y, int z, T weight, T mass) {
+    const T v = wclamp(weight * mass * T(4096), T(0), T(65535));
+    liquid[idx(x, y, z)] += (v == v) ? (uint32_t)v : 0u;
+  }
+
+  // depositAtPosition, shaders.ts:742-791.
+  void deposit_at(T px, T py, T pz, T mass) {
+    const T vx = px * gx - T(0.5), vy = py * gy - T(0.5), vz = pz * gz - T(0.5);
+    if (!spray_nozzle()) {
+      const T flx = Math<T>::floor(vx), fly = Math<T>::floor(vy), flz = Math<T>::floor(vz);
+      const int bx = (int)flx, by = (int)fly, bz = (int)flz;
+      const T fx = vx - flx, fy = vy - fly, fz = vz - flz;
+      const T ix = T(1) - fx, iy = T(1) - fy, iz = T(1) - fz;
+      const T m = mass * T(0.20);
+      deposit_corner(bx, by, bz, ix * iy * iz, m);
+      deposit_corner(bx + 1, by, bz, fx * iy * iz, m);
+      deposit_corner(bx, by + 1, bz, ix * fy * iz, m);
+      deposit_corner(bx + 1, by + 1, bz, fx * fy * iz, m);
+      deposit_corner(bx, by, bz + 1, ix * iy * fz, m);
+      deposit_corner(bx + 1, by, bz + 1, fx * iy * fz, m);
+      deposit_corner(bx, by + 1, bz + 1, ix * fy * fz, m);
+      deposit_corner(bx + 1, by + 1, bz + 1, fx * fy * fz, m);
+      return;
+    }
+    const int cx = (int)std::nearbyint(vx), cy = (int)std::nearbyint(vy), cz = (int)std::nearbyint(vz);
+    T kernel[27];
+    T total = T(0);
+    int k = 0;
+    for (int oz = -1; oz <= 1; ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox, ++k) {
+          const T distance = len3(vx - T((float)(cx + ox)), vy - T((float)(cy + oy)), vz - T((float)(cz + oz)));
+          const T w = wmax(T(0), T(1) - distance * T(0.52));
+          kernel[k] = w;
+          total = total + w * w;
+        }
+    k = 0;
+    for (int oz = -1; oz <= 1; ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox, ++k)
+          deposit_corner(cx + ox, cy + oy, cz + oz, kernel[k] * kernel[k] / wmax(total, T(0.0001)), mass);
+  }
+
+  // depositParticles, shaders.ts:793-825.
+  void liquid_deposit(const std::vector<T>& current, const std::vector<T>& previous) {
+    for (int id = 0; id < particleCount; ++id) {
+      const T* p = &current[(size_t)id * 8];
+      const T mass = p[7];
+      if (mass <= T(0.0001) || p[0] < T(0) || p[1] < T(0) || p[2] < T(0) || p[0] >= T(1) || p[1] >= T(1) ||
+          p[2] >= T(1))
+        continue;
+      const T* q = &previous[(size_t)id * 8];
+      const T dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+      const bool canSweep = q[7] > T(0.0001) && p[3] >= q[3] && len3(dx * ex, dy * ey, dz * ez) < T(0.20);
+      if (!canSweep) {
+        deposit_at(p[0], p[1], p[2], mass);
+        continue;
+      }
+      for (int s = 0; s < 3; ++s) {
+        const T f = T(s) / T(2);
+        deposit_at(q[0] + dx * f, q[1] + dy * f, q[2] + dz * f, mass / T(3));
+      }
+    }
+  }
+
+  // The water branch at the head of encodeSimulationStep (FireXEngine.ts:2158-2171).
+  void droplet_passes() {
+    if (water_active()) {
+      if (!particleFieldActive) write_particle_initial_state(true);
+      std::fill(liquid.begin(), liquid.end(), 0u);
+      sph_density(particles[particleIndex]);
+      sph_integrate(particles[particleIndex], particles[1 - particleIndex]);
+      liquid_deposit(particles[1 - particleIndex], particles[particleIndex]);
+      particleIndex = 1 - particleIndex;
+      liquidMayContainWater = true;
+      log += "clear_liquid;sph_density;sph_integrate;liquid_deposit;";
+    } else if (liquidMayContainWater || particleFieldActive) {
+      std::fill(liquid.begin(), liquid.end(), 0u);
+      write_particle_initial_state(false);
+      liquidMayContainWater = false;
+      log += "clear_liquid;particle_reset;";
+    }
+  }
+
   V4<T> store_species(const V4<T>& s) const;
   T texel(T v) const;  // an rgba16float texel component: f16 rounding (FP32 oracle)
 
@@ -644,11 +1014,11 @@ template <typename T> struct State {
         }
   }
 
-  // encodeSimulationStep, FireXEngine.ts:2115-2276 (water passes skipped:
-  // the liquid field is an input here).
+  // encodeSimulationStep, FireXEngine.ts:2115-2276.
   void step(double deltaTime) {
     write_uniforms(deltaTime);
     log.clear();
+    droplet_passes();
     log += "simulate;";
     simulate(speciesIndex);
     speciesIndex = 1 - speciesIndex;
@@ -749,7 +1119,8 @@ struct Handle {
 
 enum Field {
   F_FLOW = 0, F_SPECIES = 1, F_PRESSURE = 2, F_CORRECTION = 3, F_DIVERGENCE = 4,
-  F_REACTION = 5, F_LIQUID = 6, F_FLOW_PREDICTED = 7
+  F_REACTION = 5, F_LIQUID = 6, F_FLOW_PREDICTED = 7,
+  F_PARTICLES = 11, F_PARTICLE_DENSITY = 12  // per particle: 8 and 4 values
 };
 
 template <typename T> size_t field_components(int field) {
@@ -781,6 +1152,14 @@ template <typename T> int read_field(State<T>& s, int field, double* out) {
     case F_DIVERGENCE: for (size_t i = 0; i < n; ++i) out[i] = s.divergence[i]; return 0;
     case F_REACTION: for (size_t i = 0; i < n; ++i) out[i] = s.reaction[i]; return 0;
     case F_LIQUID: for (size_t i = 0; i < n; ++i) out[i] = (double)s.liquid[i]; return 0;
+    case F_PARTICLES: {
+      const auto& q = s.particles[s.particleIndex];
+      for (size_t i = 0; i < q.size(); ++i) out[i] = q[i];
+      return 0;
+    }
+    case F_PARTICLE_DENSITY:
+      for (size_t i = 0; i < s.particleDensity.size(); ++i) out[i] = s.particleDensity[i];
+      return 0;
   }
   return -1;
 }
@@ -802,6 +1181,19 @@ template <typename T> int write_field(State<T>& s, int field, const double* in) 
     case F_DIVERGENCE: for (size_t i = 0; i < n; ++i) s.divergence[i] = T(in[i]); return 0;
     case F_REACTION: for (size_t i = 0; i < n; ++i) s.reaction[i] = T(in[i]); return 0;
     case F_LIQUID: for (size_t i = 0; i < n; ++i) s.liquid[i] = (uint32_t)in[i]; return 0;
+    case F_PARTICLES: {
+      // both particle buffers, as writeParticleInitialState writes them; the
+      // written state is the live one (the next water step advances it)
+      for (int k = 0; k < 2; ++k)
+        for (size_t i = 0; i < s.particles[k].size(); ++i) s.particles[k][i] = T(in[i]);
+      s.particleIndex = 0;
+      s.particleFieldActive = true;
+      return 0;
+    }
+    case F_PARTICLE_DENSITY: {
+      for (size_t i = 0; i < s.particleDensity.size(); ++i) s.particleDensity[i] = T(in[i]);
+      return 0;
+    }
   }
   return -1;
 }
@@ -884,7 +1276,56 @@ int orc_set_emitters(void* p, int count, const float* data /* count*5: cx cy cz 
 void orc_reset(void* p) {
   Handle* h = static_cast<Handle*>(p);
   ORC_DISPATCH(h, (s.simulationTime = 0, s.frameIndex = 0, s.speciesIndex = 0, s.pressureIndex = 0,
-                   s.initialize_fields(), 0));
+                   s.particleIndex = 0, s.initialize_fields(), 0));
+}
+
+// Particle count of the resource set (FireXEngine.ts:187-277 per tier);
+// rebuilds the particle state as a resource rebuild does.
+int orc_set_particle_count(void* p, int count) {
+  if (count < 0) return 1;
+  Handle* h = static_cast<Handle*>(p);
+  h->f.set_particle_count(count);
+  h->d.set_particle_count(count);
+  return 0;
+}
+int orc_particle_count(void* p) { return static_cast<Handle*>(p)->f.particleCount; }
+int orc_particle_index(void* p) { Handle* h = static_cast<Handle*>(p); return h->precision == 32 ? h->f.particleIndex : h->d.particleIndex; }
+
+// One droplet pass on the current particle buffers with uniforms for dt:
+// 0 computeParticleDensity, 1 integrateParticles (current -> other buffer,
+// then flip), 2 depositParticles (current as the new state, other as the
+// previous one; adds to the liquid field).
+int orc_kernel_droplets(void* p, double dt, int which) {
+  Handle* h = static_cast<Handle*>(p);
+  return ORC_DISPATCH(h, (s.write_uniforms(dt), [&] {
+    const int pi = s.particleIndex;
+    if (which == 0) s.sph_density(s.particles[pi]);
+    else if (which == 1) { s.sph_integrate(s.particles[pi], s.particles[1 - pi]); s.particleIndex = 1 - pi; }
+    else if (which == 2) s.liquid_deposit(s.particles[pi], s.particles[1 - pi]);
+    else return 1;
+    return 0;
+  }()));
+}
+
+// Sets the two particle buffers separately (the live one first), for
+// single-pass tests of the deposition.
+void orc_write_particle_buffers(void* p, const double* current, const double* previous) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, ([&] {
+    using T = typename std::remove_reference<decltype(s.particles[0][0])>::type;
+    for (size_t i = 0; i < s.particles[0].size(); ++i) {
+      s.particles[0][i] = T(current[i]);
+      s.particles[1][i] = T(previous[i]);
+    }
+    s.particleIndex = 0;
+    s.particleFieldActive = true;
+  }(), 0));
+}
+
+// writeParticleInitialState(active) on the current parameters.
+void orc_write_particle_initial_state(void* p, int active) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.write_particle_initial_state(active != 0), 0));
 }
 
 void orc_step(void* p, double dt, int count) {

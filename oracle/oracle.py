@@ -18,6 +18,10 @@ LIB_PATH = os.path.join(_HERE, "liboracle_firex.so")
 FIELD_FLOW, FIELD_SPECIES, FIELD_PRESSURE, FIELD_CORRECTION = 0, 1, 2, 3
 FIELD_DIVERGENCE, FIELD_REACTION, FIELD_LIQUID, FIELD_FLOW_PREDICTED = 4, 5, 6, 7
 VECTOR_FIELDS = (FIELD_FLOW, FIELD_SPECIES, FIELD_FLOW_PREDICTED)
+# per-particle fields: (count, 8) positionAge + velocityMass; (count, 4) density
+FIELD_PARTICLES, FIELD_PARTICLE_DENSITY = 11, 12
+PARTICLE_FIELDS = {FIELD_PARTICLES: 8, FIELD_PARTICLE_DENSITY: 4}
+DROPLET_DENSITY, DROPLET_INTEGRATE, DROPLET_DEPOSIT = 0, 1, 2
 
 SPECIES_F32, SPECIES_F16, SPECIES_BF16 = 0, 1, 2
 HEAVY_AUTO, HEAVY_OFF, HEAVY_F16, HEAVY_F32 = 0, 1, 2, 3
@@ -73,6 +77,12 @@ def load() -> ctypes.CDLL:
         "orc_f32_eval": (None, [i, fp, fp, fp, ctypes.c_int64]),
         "orc_stencil_weights": (None, [vp, d, dp]),
         "orc_pack_fields": (i, [vp, dp, dp, dp]),
+        "orc_set_particle_count": (i, [vp, i]),
+        "orc_particle_count": (i, [vp]),
+        "orc_particle_index": (i, [vp]),
+        "orc_kernel_droplets": (i, [vp, d, i]),
+        "orc_write_particle_initial_state": (None, [vp, i]),
+        "orc_write_particle_buffers": (None, [vp, dp, dp]),
         "orc_set_threads": (None, [i]),
         "orc_max_threads": (i, []),
     }
@@ -164,7 +174,41 @@ class Oracle:
     def step(self, dt: float, count: int = 1) -> None:
         self.lib.orc_step(self.h, float(dt), int(count))
 
+    def set_particle_count(self, count: int) -> None:
+        """Particle count of the resource set (384-4096 by tier); rebuilds
+        the particle state (FireXEngine.ts:1917-1931, 1976-2034)."""
+        if self.lib.orc_set_particle_count(self.h, int(count)):
+            raise ValueError("particle count must be non-negative")
+
+    @property
+    def particle_count(self) -> int:
+        return self.lib.orc_particle_count(self.h)
+
+    @property
+    def particle_index(self) -> int:
+        return self.lib.orc_particle_index(self.h)
+
+    def kernel_droplets(self, dt: float, which: int) -> None:
+        """One droplet pass (DROPLET_DENSITY / _INTEGRATE / _DEPOSIT) on the
+        current particle buffers with the uniforms of a dt step."""
+        if self.lib.orc_kernel_droplets(self.h, float(dt), int(which)):
+            raise ValueError(f"unknown droplet pass {which}")
+
+    def write_particle_buffers(self, current, previous) -> None:
+        """The live particle buffer and the other one (the previous state
+        the deposition sweeps from), each (count, 8)."""
+        shape = (self.particle_count, 8)
+        c = np.ascontiguousarray(np.asarray(current, dtype=np.float64).reshape(shape))
+        q = np.ascontiguousarray(np.asarray(previous, dtype=np.float64).reshape(shape))
+        dp = ctypes.POINTER(ctypes.c_double)
+        self.lib.orc_write_particle_buffers(self.h, c.ctypes.data_as(dp), q.ctypes.data_as(dp))
+
+    def write_particle_initial_state(self, active: bool) -> None:
+        self.lib.orc_write_particle_initial_state(self.h, int(bool(active)))
+
     def _shape(self, field):
+        if field in PARTICLE_FIELDS:
+            return (self.particle_count, PARTICLE_FIELDS[field])
         nx, ny, nz = self.dims
         return (nz, ny, nx, 4) if field in VECTOR_FIELDS else (nz, ny, nx)
 
