@@ -120,6 +120,32 @@ def pack_profile(solver, cells: int, storage: str, reps: int = 10) -> dict:
             "algorithmic_GBps": round(bpc * cells * n / (ms * 1e-3) / 1e9, 1), "launches": n}
 
 
+def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict) -> dict:
+    """The same workload with the hose on (waterFlow 0.8, spray nozzle,
+    waterEnabled): the uhd tier's 4096 SPH droplets advance every step
+    (density and integration each test all N^2 ordered pairs; deposition
+    scatters into the liquid field that simulate reads).  The Jacobi plan is
+    the one the dry run used."""
+    s = make_solver(wl, "fused", plan["sweeps_per_launch"], chunk=plan["chunk_z"])
+    for k, v in (("waterFlow", 0.8), ("nozzleType", 1), ("sprayAngle", 24.0)):
+        s.set_parameter(k, v)
+    s.set_parameter("waterEnabled", True)
+    n = s.info["particle_count"]
+    el = time_steps(s, steps, warmup)
+    s.set_pass_timing(True)
+    s.step(DT, 10)
+    t = s.pass_timing()
+    s.set_pass_timing(False)
+    liquid = float(s.read_field("liquid", dtype="uint32").sum()) / 4096.0
+    s.close()
+    d = t["droplets"]
+    us = d["ms"] / t["steps"] * 1e3
+    return {"particles": n, "nozzle": "spray", "steps": steps, "ms_per_step_with_water": el / steps * 1e3,
+            "droplet_passes_us_per_step": round(us, 2), "launches_per_step": d["launches"] / t["steps"],
+            "pair_tests_per_s": 2.0 * n * n / (us * 1e-6) if us > 0 else None,
+            "deposited_liquid_mass": round(liquid, 3)}
+
+
 def measured_traffic(workload: str, plan: dict):
     """HBM bytes per Jacobi launch for this plan from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
@@ -253,6 +279,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-steps", type=int, default=5)
     ap.add_argument("--large-steps", type=int, default=10,
                     help="steps of the 508^3 plume for its per-pass HBM GB/s (0 = skip)")
+    ap.add_argument("--droplet-steps", type=int, default=60,
+                    help="steps of the same workload with the hose on (SPH droplets; 0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16, help="serving-stream sample size (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
@@ -349,6 +377,12 @@ def main() -> int:
                  "passes": {k: dict(v, frac_of_peak=round(v["algorithmic_GBps"] / PEAK_HBM_GBS, 4))
                             if isinstance(v, dict) else v for k, v in lpasses.items()}}
 
+    droplets = None
+    if args.droplet_steps > 0:
+        print(f"bench: hose on, {args.droplet_steps} steps", file=sys.stderr, flush=True)
+        droplets = droplet_profile(wl, args.droplet_steps, 10, jacobi_plan)
+        droplets["step_time_over_dry"] = round(droplets["ms_per_step_with_water"] / ms_per_step, 4)
+
     serving = None
     if args.serve_requests > 0:
         print(f"bench: serving stream ({args.serve_requests} requests at {args.serve_rate}/s)", file=sys.stderr,
@@ -384,6 +418,7 @@ def main() -> int:
         "multiple_over_plain_f32": (value / plain["cell_updates_per_s"]) if plain else None,
         "serving": serving,
         "plume_508": large,
+        "droplets": droplets,
         "check": {"sum_fuel_after_run": t_fuel},
     }
     print(json.dumps(line), flush=True)

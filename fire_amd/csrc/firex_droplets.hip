@@ -14,7 +14,8 @@
 //
 // The neighbour sums of the density and integration passes are sequential
 // folds over particle index in the reference, and f32 addition does not
-// associate.  Each particle therefore gets one wave: lane l evaluates
+// associate.  Each particle therefore gets one wave (16 per block, which
+// stages the neighbours' positions and masses in LDS): lane l evaluates
 // neighbour base + l of every 64-wide chunk (distance test and the pair
 // terms), and the wave then adds the contributing lanes' terms into its
 // accumulators one by one in lane order (ballot, then readlane per set bit).
@@ -33,7 +34,9 @@
 
 namespace {
 
-constexpr int kWavesPerBlock = 4;  // particles per density / integration block
+constexpr int kParticlesPerBlock = 16;  // waves (one per particle) of a density / integration block
+constexpr int kTile = 4096;             // neighbours staged in LDS at a time (64 KB)
+constexpr int kTileIntegrate = 2048;    // the same for the integration pass (72 KB)
 
 __device__ inline float hash11(float v) {
   const float h = fxm_sin(v * 12.9898f + 1.317f) * 43758.5453f;
@@ -97,161 +100,214 @@ __device__ void spawn(uint32_t id, float cycle, const FxDroplets& d, float4& pa,
   vm = make_float4(lx * speed, ly * speed, lz * speed, 0.52f + d.water_flow * 0.88f);
 }
 
-// computeParticleDensity: one wave per particle.
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_sph_density(const float4* __restrict__ pin,
-                                                                      float4* __restrict__ dens, FxDroplets d) {
-  const int lane = (int)(threadIdx.x & 63u);
-  const uint32_t i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (i >= d.count) return;  // whole waves
-  const float4 p = pin[2 * i];
-  const float mass = pin[2 * i + 1].w;
-  if (mass <= 0.0001f) {
-    if (lane == 0) dens[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
-  }
-  const float radius = d.nozzle >= 0.5f ? 0.060f : 0.038f;
-  const float guard = (radius * 1.001f) * (radius * 1.001f);
-  float density = 0.0f, nearDensity = 0.0f;
-  for (uint32_t base = 0; base < d.count; base += 64) {
-    const uint32_t j = base + (uint32_t)lane;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    bool near = false;
-    if (j < d.count && j != i && !(pin[2 * j + 1].w <= 0.0001f)) {
-      const float4 q = pin[2 * j];
-      sx = (p.x - q.x) * d.ex;
-      sy = (p.y - q.y) * d.ey;
-      sz = (p.z - q.z) * d.ez;
-      near = sx * sx + sy * sy + sz * sz < guard;
-    }
-    if (__ballot(near) == 0) continue;
-    float w2 = 0.0f, w3 = 0.0f;
-    bool hit = false;
-    if (near) {
-      const float distance = len3(sx, sy, sz);
-      if (distance > 0.00001f && distance < radius) {
-        const float w = 1.0f - distance / radius;
-        w2 = w * w;
-        w3 = w * w * w;
-        hit = true;
-      }
-    }
-    for (uint64_t m = __ballot(hit); m; m &= m - 1) {
-      const int l = __builtin_ctzll(m);
-      density = density + lane_value(w2, l);
-      nearDensity = nearDensity + lane_value(w3, l);
-    }
-  }
-  if (lane == 0) {
-    const float restDensity = 2.7f + d.water_flow * 0.9f;
-    dens[i] = make_float4(density, nearDensity, fxm_max(density - restDensity, 0.0f) * 0.055f,
-                          nearDensity * 0.065f);
+// Neighbour positions and masses, staged in LDS tile by tile: every wave of
+// a block reads each tile, so the block fetches a neighbour once instead of
+// once per particle.
+struct NeighbourTile {
+  float4 pm[kTile];  // (x, y, z, mass)
+};
+
+__device__ inline void load_tile(NeighbourTile& t, const float4* __restrict__ pin, uint32_t t0, uint32_t tn) {
+  for (uint32_t k = threadIdx.x; k < tn; k += blockDim.x) {
+    float4 a = pin[2 * (t0 + k)];
+    a.w = pin[2 * (t0 + k) + 1].w;
+    t.pm[k] = a;
   }
 }
 
-// integrateParticles: one wave per particle.
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_sph_integrate(const float4* __restrict__ pin,
-                                                                        float4* __restrict__ pout,
-                                                                        const float4* __restrict__ dens,
-                                                                        const float4* __restrict__ flow,
-                                                                        FxDroplets d) {
+// computeParticleDensity: one wave per particle, kParticlesPerBlock per block.
+__global__ __launch_bounds__(64 * kParticlesPerBlock) void k_sph_density(const float4* __restrict__ pin,
+                                                                          float4* __restrict__ dens, FxDroplets d) {
+  __shared__ NeighbourTile tile;
   const int lane = (int)(threadIdx.x & 63u);
-  const uint32_t id = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (id >= d.count) return;
-  const float4 pa = pin[2 * id];
-  const float4 vm = pin[2 * id + 1];
+  const uint32_t i = blockIdx.x * kParticlesPerBlock + (threadIdx.x >> 6);
+  const bool mine = i < d.count;  // whole waves; every wave joins the tile barriers
+  const float4 p = mine ? pin[2 * i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float mass = mine ? pin[2 * i + 1].w : 0.0f;
+  const bool live = mine && !(mass <= 0.0001f);
+  const float radius = d.nozzle >= 0.5f ? 0.060f : 0.038f;
+  const float guard = (radius * 1.001f) * (radius * 1.001f);
+  float density = 0.0f, nearDensity = 0.0f;
+  for (uint32_t t0 = 0; t0 < d.count; t0 += kTile) {
+    const uint32_t tn = min((uint32_t)kTile, d.count - t0);
+    __syncthreads();
+    load_tile(tile, pin, t0, tn);
+    __syncthreads();
+    if (!live) continue;
+    for (uint32_t base = 0; base < tn; base += 64) {
+      const uint32_t k = base + (uint32_t)lane;
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+      bool near = false;
+      if (k < tn && t0 + k != i) {
+        const float4 q = tile.pm[k];
+        if (!(q.w <= 0.0001f)) {
+          sx = (p.x - q.x) * d.ex;
+          sy = (p.y - q.y) * d.ey;
+          sz = (p.z - q.z) * d.ez;
+          near = sx * sx + sy * sy + sz * sz < guard;
+        }
+      }
+      if (__ballot(near) == 0) continue;
+      float w2 = 0.0f, w3 = 0.0f;
+      bool hit = false;
+      if (near) {
+        const float distance = len3(sx, sy, sz);
+        if (distance > 0.00001f && distance < radius) {
+          const float w = 1.0f - distance / radius;
+          w2 = w * w;
+          w3 = w * w * w;
+          hit = true;
+        }
+      }
+      for (uint64_t m = __ballot(hit); m; m &= m - 1) {
+        const int l = __builtin_ctzll(m);
+        density = density + lane_value(w2, l);
+        nearDensity = nearDensity + lane_value(w3, l);
+      }
+    }
+  }
+  if (!mine || lane != 0) return;
+  if (!live) {
+    dens[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const float restDensity = 2.7f + d.water_flow * 0.9f;
+  dens[i] = make_float4(density, nearDensity, fxm_max(density - restDensity, 0.0f) * 0.055f, nearDensity * 0.065f);
+}
+
+// The integration pass also needs each contributing neighbour's velocity and
+// pressures: staged with its position.
+struct IntegrateTile {
+  float4 pm[kTileIntegrate];  // (x, y, z, mass)
+  float4 vp[kTileIntegrate];  // (vx, vy, vz, pressure)
+  float np[kTileIntegrate];   // near pressure
+};
+
+__device__ inline void load_integrate_tile(IntegrateTile& t, const float4* __restrict__ pin,
+                                           const float4* __restrict__ dens, uint32_t t0, uint32_t tn) {
+  for (uint32_t k = threadIdx.x; k < tn; k += blockDim.x) {
+    float4 a = pin[2 * (t0 + k)];
+    float4 v = pin[2 * (t0 + k) + 1];
+    const float4 q = dens[t0 + k];
+    a.w = v.w;
+    v.w = q.z;
+    t.pm[k] = a;
+    t.vp[k] = v;
+    t.np[k] = q.w;
+  }
+}
+
+// integrateParticles: one wave per particle, kParticlesPerBlock per block.
+__global__ __launch_bounds__(64 * kParticlesPerBlock) void k_sph_integrate(const float4* __restrict__ pin,
+                                                                            float4* __restrict__ pout,
+                                                                            const float4* __restrict__ dens,
+                                                                            const float4* __restrict__ flow,
+                                                                            FxDroplets d) {
+  __shared__ IntegrateTile tile;
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t id = blockIdx.x * kParticlesPerBlock + (threadIdx.x >> 6);
+  const bool mine = id < d.count;
+  const float4 pa = mine ? pin[2 * id] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 vm = mine ? pin[2 * id + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float dt = d.dt;
-  if (dt <= 0.0f) {
-    if (lane == 0) {
-      pout[2 * id] = pa;
-      pout[2 * id + 1] = vm;
-    }
-    return;
-  }
   const bool respawn = d.water_enabled >= 0.5f && d.water_flow > 0.001f;
-  if (vm.w <= 0.0001f) {
-    const float wait = pa.w + dt;
-    float4 oa, ov;
-    if (respawn && wait >= 0.0f) {
-      spawn(id, wait + d.frame, d, oa, ov);
-    } else {
-      oa = make_float4(0.035f, 0.50f, 0.5f, wait > 0.0f ? -0.72f : wait);
-      ov = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    if (lane == 0) {
-      pout[2 * id] = oa;
-      pout[2 * id + 1] = ov;
-    }
-    return;
+  // what this particle does: 0 copy (dt <= 0), 1 waiting (inactive),
+  // 2 expired or outside, 3 integrate
+  int mode = 3;
+  float lifetime = 0.0f, nextAge = 0.0f;
+  if (dt <= 0.0f) {
+    mode = 0;
+  } else if (vm.w <= 0.0001f) {
+    mode = 1;
+  } else {
+    lifetime = 0.88f + hash11((float)id * 2.31f + 7.0f) * 0.52f;
+    nextAge = pa.w + dt;
+    const bool outside = pa.x < -0.03f || pa.y < -0.03f || pa.z < -0.03f || pa.x > 1.03f || pa.y > 1.03f ||
+                         pa.z > 1.03f;
+    if (nextAge >= lifetime || outside) mode = 2;
   }
-  const float lifetime = 0.88f + hash11((float)id * 2.31f + 7.0f) * 0.52f;
-  const float nextAge = pa.w + dt;
-  const bool outside = pa.x < -0.03f || pa.y < -0.03f || pa.z < -0.03f || pa.x > 1.03f || pa.y > 1.03f ||
-                       pa.z > 1.03f;
-  if (nextAge >= lifetime || outside) {
-    float4 oa, ov;
-    if (respawn) {
-      spawn(id, nextAge + d.frame, d, oa, ov);
-    } else {
-      oa = make_float4(0.035f, 0.50f, 0.5f, -hash11((float)id * 8.17f) * 0.72f);
-      ov = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    if (lane == 0) {
-      pout[2 * id] = oa;
-      pout[2 * id + 1] = ov;
-    }
-    return;
-  }
+  const bool live = mine && mode == 3;
 
   const bool spray = d.nozzle >= 0.5f;
   const float radius = spray ? 0.060f : 0.038f;
   const float guard = (radius * 1.001f) * (radius * 1.001f);
-  const float4 pd = dens[id];
+  const float4 pd = live ? dens[id] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float pax = 0.0f, pay = 0.0f, paz = 0.0f, vax = 0.0f, vay = 0.0f, vaz = 0.0f;
-  for (uint32_t base = 0; base < d.count; base += 64) {
-    const uint32_t j = base + (uint32_t)lane;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    bool near = false;
-    if (j < d.count && j != id && !(pin[2 * j + 1].w <= 0.0001f)) {
-      const float4 q = pin[2 * j];
-      sx = (pa.x - q.x) * d.ex;
-      sy = (pa.y - q.y) * d.ey;
-      sz = (pa.z - q.z) * d.ez;
-      near = sx * sx + sy * sy + sz * sz < guard;
-    }
-    if (__ballot(near) == 0) continue;
-    float tpx = 0.0f, tpy = 0.0f, tpz = 0.0f, tvx = 0.0f, tvy = 0.0f, tvz = 0.0f;
-    bool hit = false;
-    if (near) {
-      const float distance = len3(sx, sy, sz);
-      if (distance > 0.00001f && distance < radius) {
-        const float w = 1.0f - distance / radius;
-        const float dx = sx / distance, dy = sy / distance, dz = sz / distance;
-        const float4 nd = dens[j];
-        const float4 nv = pin[2 * j + 1];
-        const float pressure = (pd.z + nd.z) * w;
-        const float nearPressure = (pd.w + nd.w) * w * w;
-        const float push = pressure + nearPressure;
-        tpx = dx * push;
-        tpy = dy * push;
-        tpz = dz * push;
-        const float visc = w * 0.018f;
-        tvx = (nv.x - vm.x) * visc;
-        tvy = (nv.y - vm.y) * visc;
-        tvz = (nv.z - vm.z) * visc;
-        hit = true;
+  for (uint32_t t0 = 0; t0 < d.count; t0 += kTileIntegrate) {
+    const uint32_t tn = min((uint32_t)kTileIntegrate, d.count - t0);
+    __syncthreads();
+    load_integrate_tile(tile, pin, dens, t0, tn);
+    __syncthreads();
+    if (!live) continue;
+    for (uint32_t base = 0; base < tn; base += 64) {
+      const uint32_t k = base + (uint32_t)lane;
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+      bool near = false;
+      if (k < tn && t0 + k != id) {
+        const float4 q = tile.pm[k];
+        if (!(q.w <= 0.0001f)) {
+          sx = (pa.x - q.x) * d.ex;
+          sy = (pa.y - q.y) * d.ey;
+          sz = (pa.z - q.z) * d.ez;
+          near = sx * sx + sy * sy + sz * sz < guard;
+        }
+      }
+      if (__ballot(near) == 0) continue;
+      float tpx = 0.0f, tpy = 0.0f, tpz = 0.0f, tvx = 0.0f, tvy = 0.0f, tvz = 0.0f;
+      bool hit = false;
+      if (near) {
+        const float distance = len3(sx, sy, sz);
+        if (distance > 0.00001f && distance < radius) {
+          const float w = 1.0f - distance / radius;
+          const float dx = sx / distance, dy = sy / distance, dz = sz / distance;
+          const float4 nv = tile.vp[k];
+          const float pressure = (pd.z + nv.w) * w;
+          const float nearPressure = (pd.w + tile.np[k]) * w * w;
+          const float push = pressure + nearPressure;
+          tpx = dx * push;
+          tpy = dy * push;
+          tpz = dz * push;
+          const float visc = w * 0.018f;
+          tvx = (nv.x - vm.x) * visc;
+          tvy = (nv.y - vm.y) * visc;
+          tvz = (nv.z - vm.z) * visc;
+          hit = true;
+        }
+      }
+      for (uint64_t m = __ballot(hit); m; m &= m - 1) {
+        const int l = __builtin_ctzll(m);
+        pax = pax + lane_value(tpx, l);
+        pay = pay + lane_value(tpy, l);
+        paz = paz + lane_value(tpz, l);
+        vax = vax + lane_value(tvx, l);
+        vay = vay + lane_value(tvy, l);
+        vaz = vaz + lane_value(tvz, l);
       }
     }
-    for (uint64_t m = __ballot(hit); m; m &= m - 1) {
-      const int l = __builtin_ctzll(m);
-      pax = pax + lane_value(tpx, l);
-      pay = pay + lane_value(tpy, l);
-      paz = paz + lane_value(tpz, l);
-      vax = vax + lane_value(tvx, l);
-      vay = vay + lane_value(tvy, l);
-      vaz = vaz + lane_value(tvz, l);
-    }
   }
-  if (lane != 0) return;
+  if (!mine || lane != 0) return;
+
+  if (mode == 0) {
+    pout[2 * id] = pa;
+    pout[2 * id + 1] = vm;
+    return;
+  }
+  if (mode == 1 || mode == 2) {
+    float4 oa, ov;
+    const float cycle = mode == 1 ? pa.w + dt : nextAge;  // wait, or the age at expiry
+    if (respawn && (mode == 2 || cycle >= 0.0f)) {
+      spawn(id, cycle + d.frame, d, oa, ov);
+    } else if (mode == 1) {
+      oa = make_float4(0.035f, 0.50f, 0.5f, cycle > 0.0f ? -0.72f : cycle);
+      ov = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      oa = make_float4(0.035f, 0.50f, 0.5f, -hash11((float)id * 8.17f) * 0.72f);
+      ov = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    pout[2 * id] = oa;
+    pout[2 * id + 1] = ov;
+    return;
+  }
 
   // sampleFlow, shaders.ts:566-569
   const int cx = (int)(fxm_clamp(pa.x, 0.0f, 0.99999f) * d.gx);
@@ -338,12 +394,15 @@ __device__ void deposit_at(uint32_t* liquid, float px, float py, float pz, float
   }
 }
 
-// depositParticles: one thread per particle.
+// depositParticles: one thread per particle and sweep sample (a droplet
+// that is not swept deposits from sample 0 only), so the scattered atomics
+// of neighbouring samples are issued by different waves.
 __global__ __launch_bounds__(256) void k_liquid_deposit(const float4* __restrict__ current,
                                                         const float4* __restrict__ previous,
                                                         uint32_t* __restrict__ liquid, FxDroplets d) {
-  const uint32_t id = blockIdx.x * 256 + threadIdx.x;
-  if (id >= d.count) return;
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 3 * d.count) return;
+  const uint32_t id = t / 3, sample = t - 3 * id;
   const float4 p = current[2 * id];
   const float mass = current[2 * id + 1].w;
   if (mass <= 0.0001f || p.x < 0.0f || p.y < 0.0f || p.z < 0.0f || p.x >= 1.0f || p.y >= 1.0f || p.z >= 1.0f) return;
@@ -352,13 +411,11 @@ __global__ __launch_bounds__(256) void k_liquid_deposit(const float4* __restrict
   const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
   const bool canSweep = qm > 0.0001f && p.w >= q.w && len3(dx * d.ex, dy * d.ey, dz * d.ez) < 0.20f;
   if (!canSweep) {
-    deposit_at(liquid, p.x, p.y, p.z, mass, d);
+    if (sample == 0) deposit_at(liquid, p.x, p.y, p.z, mass, d);
     return;
   }
-  for (int s = 0; s < 3; ++s) {
-    const float f = (float)s / 2.0f;
-    deposit_at(liquid, q.x + dx * f, q.y + dy * f, q.z + dz * f, mass / 3.0f, d);
-  }
+  const float f = (float)sample / 2.0f;
+  deposit_at(liquid, q.x + dx * f, q.y + dy * f, q.z + dz * f, mass / 3.0f, d);
 }
 
 }  // namespace
@@ -367,21 +424,21 @@ namespace fx {
 
 void launch_sph_density(const float4* particles, float4* density, const FxDroplets& d, hipStream_t s) {
   if (d.count == 0) return;
-  const uint32_t blocks = (d.count + kWavesPerBlock - 1) / kWavesPerBlock;
-  k_sph_density<<<blocks, 64 * kWavesPerBlock, 0, s>>>(particles, density, d);
+  const uint32_t blocks = (d.count + kParticlesPerBlock - 1) / kParticlesPerBlock;
+  k_sph_density<<<blocks, 64 * kParticlesPerBlock, 0, s>>>(particles, density, d);
 }
 
 void launch_sph_integrate(const float4* particles_in, float4* particles_out, const float4* density,
                           const float4* flow, const FxDroplets& d, hipStream_t s) {
   if (d.count == 0) return;
-  const uint32_t blocks = (d.count + kWavesPerBlock - 1) / kWavesPerBlock;
-  k_sph_integrate<<<blocks, 64 * kWavesPerBlock, 0, s>>>(particles_in, particles_out, density, flow, d);
+  const uint32_t blocks = (d.count + kParticlesPerBlock - 1) / kParticlesPerBlock;
+  k_sph_integrate<<<blocks, 64 * kParticlesPerBlock, 0, s>>>(particles_in, particles_out, density, flow, d);
 }
 
 void launch_liquid_deposit(const float4* current, const float4* previous, uint32_t* liquid, const FxDroplets& d,
                            hipStream_t s) {
   if (d.count == 0) return;
-  k_liquid_deposit<<<(d.count + 255) / 256, 256, 0, s>>>(current, previous, liquid, d);
+  k_liquid_deposit<<<(3 * d.count + 255) / 256, 256, 0, s>>>(current, previous, liquid, d);
 }
 
 }  // namespace fx
