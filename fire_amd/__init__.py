@@ -11,6 +11,7 @@ from .solver import (  # noqa: F401
     FireXHipSolver,
     FireXLibraryError,
     abi_version,
+    checksum_bytes,
     config_from_tier,
     device_count,
     estimate_memory,

@@ -446,6 +446,21 @@ int firex_config_from_tier(firex_config* c, const char* tier) {
   return set_error(FIREX_ERR_INVALID, std::string("unknown Fire-X tier: ") + tier);
 }
 
+int firex_checksum_bytes(const void* data, size_t bytes, uint64_t* out) {
+  if (!out || (!data && bytes)) return set_error(FIREX_ERR_INVALID, "data and out must be non-null");
+  const uint8_t* b = (const uint8_t*)data;
+  const uint64_t n = (bytes + 3) / 4;
+  uint64_t h = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t w = 0;
+    const size_t left = bytes - (size_t)i * 4;
+    memcpy(&w, b + i * 4, left < 4 ? left : 4);  // little-endian host
+    h += fx_checksum_term(i, w);
+  }
+  *out = h;
+  return FIREX_OK;
+}
+
 int firex_estimate_memory(const int32_t dims[3], int64_t particle_count, firex_memory_estimate* out) {
   if (!dims || !out) return set_error(FIREX_ERR_INVALID, "dims and out must be non-null");
   int64_t cells = (int64_t)dims[0] * (int64_t)dims[1] * (int64_t)dims[2];

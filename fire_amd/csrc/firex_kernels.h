@@ -49,6 +49,10 @@ void launch_sph_integrate(const float4* particles_in, float4* particles_out, con
 void launch_liquid_deposit(const float4* current, const float4* previous, uint32_t* liquid, const FxDroplets& d,
                            hipStream_t s);
 
+// Field checksum (fx_checksum_term summed over the words of `bytes` stored
+// bytes; data may be null for an all-zero field) into *out (device memory).
+void launch_checksum(const void* data, uint64_t bytes, uint64_t* out, hipStream_t s);
+
 constexpr uint32_t kReducePartials = 1024;
 
 }  // namespace fx

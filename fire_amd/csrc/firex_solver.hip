@@ -477,6 +477,67 @@ int correction_buffer(firex_solver* s, void** p) {
   return FIREX_OK;
 }
 
+// Where a readable field lives and how it is stored.
+struct FieldView {
+  const void* ptr = nullptr;  // null: a liquid field never written (all zero)
+  uint64_t records = 0;       // cells, or particles for the per-particle fields
+  int comps = 1;
+  int kind = 0;               // FIREX_KIND_*
+};
+
+uint64_t field_stored_bytes(const FieldView& v) {
+  return v.records * (uint64_t)v.comps * (v.kind == FIREX_KIND_F16 || v.kind == FIREX_KIND_BF16 ? 2 : 4);
+}
+
+int field_view(firex_solver* s, int32_t field, FieldView* v) {
+  *v = FieldView();
+  v->records = s->cells;
+  switch (field) {
+    case FIREX_FIELD_FLOW: v->ptr = flow0(s); v->comps = 4; break;
+    case FIREX_FIELD_FLOW_PREDICTED:
+      // After a step the reference's flow[1] equals flow[0] (copyFlow); the
+      // fused plan swaps instead of copying, so both read the same buffer.
+      v->ptr = s->cfg.plan == FIREX_PLAN_FUSED && !(s->heavy && s->compact) && s->frame_index > 0 ? flow0(s)
+                                                                                                  : flow1(s);
+      v->comps = 4;
+      break;
+    case FIREX_FIELD_SPECIES:
+      v->ptr = s->species[s->species_index];
+      v->comps = 4;
+      v->kind = s->cfg.species_storage;
+      break;
+    case FIREX_FIELD_PRESSURE: v->ptr = s->pressure[s->pressure_index]; v->kind = s->f16p ? 1 : 0; break;
+    case FIREX_FIELD_CORRECTION: {
+      void* p;
+      correction_buffer(s, &p);
+      v->ptr = p;
+      v->kind = s->f16p ? 1 : 0;
+      break;
+    }
+    case FIREX_FIELD_DIVERGENCE: v->ptr = s->divergence; break;
+    case FIREX_FIELD_REACTION: v->ptr = s->reaction; break;
+    case FIREX_FIELD_LIQUID: v->ptr = s->liquid; v->kind = FIREX_KIND_U32; break;
+    case FIREX_FIELD_PACKED_FLOW: case FIREX_FIELD_PACKED_CHEMISTRY: case FIREX_FIELD_PACKED_MEDIA:
+      if (!s->packed) return set_error(FIREX_ERR_INVALID, "the render volumes are filled by firex_pack_fields");
+      v->ptr = (const uint8_t*)s->packed + (size_t)(field - FIREX_FIELD_PACKED_FLOW) * s->cells * 8;
+      v->comps = 4;
+      v->kind = FIREX_KIND_F16;
+      break;
+    case FIREX_FIELD_PARTICLES:
+      v->ptr = s->particles[s->particle_index];
+      v->records = (uint64_t)s->cfg.particle_count;
+      v->comps = 8;
+      break;
+    case FIREX_FIELD_PARTICLE_DENSITY:
+      v->ptr = s->particle_density;
+      v->records = (uint64_t)s->cfg.particle_count;
+      v->comps = 4;
+      break;
+    default: return set_error(FIREX_ERR_INVALID, "unknown field");
+  }
+  return FIREX_OK;
+}
+
 }  // namespace
 
 extern "C" int firex_device_count(void) {
@@ -777,53 +838,24 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
   if (rc) return rc;
   if (!dst) return set_error(FIREX_ERR_INVALID, "dst is null");
   hipSetDevice(s->cfg.device);
-  uint64_t c = s->cells;  // records: cells, or particles for the per-particle fields
-  const void* src = nullptr;
-  int comps = 1;
-  int kind = 0;  // 0 f32, 1 f16, 2 bf16, 3 u32
-  switch (field) {
-    case FIREX_FIELD_PARTICLES:
-      src = s->particles[s->particle_index];
-      c = (uint64_t)s->cfg.particle_count;
-      comps = 8;
-      break;
-    case FIREX_FIELD_PARTICLE_DENSITY:
-      src = s->particle_density;
-      c = (uint64_t)s->cfg.particle_count;
-      comps = 4;
-      break;
-    case FIREX_FIELD_FLOW: src = flow0(s); comps = 4; break;
-    case FIREX_FIELD_FLOW_PREDICTED:
-      // After a step the reference's flow[1] equals flow[0] (copyFlow); the
-      // fused plan swaps instead of copying, so both read the same buffer.
-      src = s->cfg.plan == FIREX_PLAN_FUSED && !(s->heavy && s->compact) && s->frame_index > 0 ? flow0(s) : flow1(s);
-      comps = 4;
-      break;
-    case FIREX_FIELD_SPECIES:
-      src = s->species[s->species_index];
-      comps = 4;
-      kind = s->cfg.species_storage;
-      break;
-    case FIREX_FIELD_PRESSURE: src = s->pressure[s->pressure_index]; kind = s->f16p ? 1 : 0; break;
-    case FIREX_FIELD_CORRECTION: {
-      void* p;
-      correction_buffer(s, &p);
-      src = p;
-      kind = s->f16p ? 1 : 0;
-      break;
+  FieldView v;
+  rc = field_view(s, field, &v);
+  if (rc) return rc;
+  const void* src = v.ptr;
+  const int kind = v.kind;
+  const uint64_t n = v.records * v.comps;
+  const size_t sbytes = (size_t)field_stored_bytes(v);
+  if (dtype == FIREX_DTYPE_RAW) {
+    if (bytes != sbytes) return set_error(FIREX_ERR_INVALID, "byte count does not match the field's stored size");
+    FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before read");
+    if (sbytes == 0) return FIREX_OK;
+    if (!src) {
+      memset(dst, 0, bytes);
+      return FIREX_OK;
     }
-    case FIREX_FIELD_DIVERGENCE: src = s->divergence; break;
-    case FIREX_FIELD_REACTION: src = s->reaction; break;
-    case FIREX_FIELD_LIQUID: src = s->liquid; kind = 3; break;
-    case FIREX_FIELD_PACKED_FLOW: case FIREX_FIELD_PACKED_CHEMISTRY: case FIREX_FIELD_PACKED_MEDIA:
-      if (!s->packed) return set_error(FIREX_ERR_INVALID, "the render volumes are filled by firex_pack_fields");
-      src = (const uint8_t*)s->packed + (size_t)(field - FIREX_FIELD_PACKED_FLOW) * c * 8;
-      comps = 4;
-      kind = 1;  // 4 x f16
-      break;
-    default: return set_error(FIREX_ERR_INVALID, "unknown field");
+    FX_HIP(s, hipMemcpy(dst, src, sbytes, hipMemcpyDeviceToHost), "field readback");
+    return FIREX_OK;
   }
-  const uint64_t n = c * comps;
   size_t elem = dtype == FIREX_DTYPE_F64 ? 8 : 4;
   if (dtype != FIREX_DTYPE_F32 && dtype != FIREX_DTYPE_F64 && dtype != FIREX_DTYPE_U32)
     return set_error(FIREX_ERR_INVALID, "unknown dtype");
@@ -832,11 +864,10 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
   if (bytes != n * elem) return set_error(FIREX_ERR_INVALID, "byte count does not match the field size");
   FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before read");
   if (n == 0) return FIREX_OK;
-  if (kind == 3 && !src) {
+  if (!src) {
     memset(dst, 0, bytes);  // no liquid field was ever written: it is zero
     return FIREX_OK;
   }
-  const size_t sbytes = n * (kind == 1 || kind == 2 ? 2 : 4);
   std::vector<uint8_t> staging;
   void* host = dst;
   if (!(kind == 0 && dtype == FIREX_DTYPE_F32) && !(kind == 3 && dtype == FIREX_DTYPE_U32)) {
@@ -846,14 +877,47 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
   FX_HIP(s, hipMemcpy(host, src, sbytes, hipMemcpyDeviceToHost), "field readback");
   if (host == dst) return FIREX_OK;
   for (uint64_t i = 0; i < n; ++i) {
-    double v;
-    if (kind == 0) v = ((const float*)host)[i];
-    else if (kind == 1) v = (double)(float)((const _Float16*)host)[i];
-    else if (kind == 2) v = fx_bf16_bits_to_float(((const uint16_t*)host)[i]);
-    else v = (double)((const uint32_t*)host)[i];
-    if (dtype == FIREX_DTYPE_F64) ((double*)dst)[i] = v;
-    else ((float*)dst)[i] = (float)v;
+    double x;
+    if (kind == 0) x = ((const float*)host)[i];
+    else if (kind == 1) x = (double)(float)((const _Float16*)host)[i];
+    else if (kind == 2) x = fx_bf16_bits_to_float(((const uint16_t*)host)[i]);
+    else x = (double)((const uint32_t*)host)[i];
+    if (dtype == FIREX_DTYPE_F64) ((double*)dst)[i] = x;
+    else ((float*)dst)[i] = (float)x;
   }
+  return FIREX_OK;
+}
+
+int firex_get_field_layout(firex_solver* s, int32_t field, firex_field_layout* out) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
+  FieldView v;
+  rc = field_view(s, field, &v);
+  if (rc) return rc;
+  out->kind = v.kind;
+  out->components = v.comps;
+  out->records = (int64_t)v.records;
+  out->stored_bytes = (int64_t)field_stored_bytes(v);
+  return FIREX_OK;
+}
+
+int firex_field_checksum(firex_solver* s, int32_t field, uint64_t* out) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
+  hipSetDevice(s->cfg.device);
+  FieldView v;
+  rc = field_view(s, field, &v);
+  if (rc) return rc;
+  uint64_t* result = (uint64_t*)(s->scratch + kReducePartials);
+  launch_checksum(v.ptr, field_stored_bytes(v), result, s->stream);
+  rc = check_launch(s, "field checksum");
+  if (rc) return rc;
+  uint64_t h = 0;
+  FX_HIP(s, hipMemcpyAsync(&h, result, sizeof h, hipMemcpyDeviceToHost, s->stream), "checksum readback");
+  FX_HIP(s, hipStreamSynchronize(s->stream), "field checksum");
+  *out = h;
   return FIREX_OK;
 }
 

@@ -112,6 +112,17 @@ static inline uint64_t fx_mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+// Field checksum (an addition, for auditable captures): the stored bytes
+// read as little-endian u32 words w_i (a final partial word zero-padded),
+// checksum = sum over i of fx_mix64((i << 32) | w_i), mod 2^64.  Integer
+// addition commutes, so any summation order gives the same value; a change
+// of any word changes its term (fx_mix64 is a bijection).  Fields hold at
+// most 2^32 words.
+#if defined(__HIP__)
+__host__ __device__
+#endif
+static inline uint64_t fx_checksum_term(uint64_t i, uint32_t w) { return fx_mix64((i << 32) | (uint64_t)w); }
+
 #if defined(__HIP__)
 __host__ __device__
 #endif

@@ -47,6 +47,14 @@ export interface FireXHipGridInfo {
   custom: boolean;
 }
 
+export interface FireXFieldLayout {
+  kind: 'f32' | 'f16' | 'bf16' | 'u32';
+  kindCode: number;
+  components: number;
+  records: number;
+  storedBytes: number;
+}
+
 export declare class FireXHipSolver {
   readonly backend: string;
   constructor(options?: FireXHipSolverOptions);
@@ -67,6 +75,12 @@ export declare class FireXHipSolver {
   reduce(name: FireXReductionName): number;
   /** packFields: fill the render volumes read back as packedFlow / packedChemistry / packedMedia. */
   packFields(): void;
+  /** Stored format of a field on the device. */
+  fieldLayout(name: FireXFieldName): FireXFieldLayout;
+  /** The field's stored bytes, verbatim. */
+  readFieldRaw(name: FireXFieldName, out?: Uint8Array): Uint8Array;
+  /** 64-bit checksum of the field's stored bytes (computed on the device), 16 hex digits. */
+  fieldChecksum(name: FireXFieldName): string;
   getGridInfo(): FireXHipGridInfo;
   dispose(): void;
 }

@@ -126,6 +126,22 @@ class FireXHipSolver {
     native.writeField(this.requireHandle(), name, data);
   }
 
+  /** Stored format of a field: {kind: 'f32'|'f16'|'bf16'|'u32', components, records, storedBytes}. */
+  fieldLayout(name) {
+    return native.fieldLayout(this.requireHandle(), name);
+  }
+
+  /** The field's stored bytes, verbatim (a new Uint8Array, or the given one). */
+  readFieldRaw(name, out) {
+    const layout = native.fieldLayout(this.requireHandle(), name);
+    return native.readFieldRaw(this.handle, name, out || new Uint8Array(layout.storedBytes));
+  }
+
+  /** 64-bit checksum of the field's stored bytes, computed on the device, as 16 hex digits. */
+  fieldChecksum(name) {
+    return native.fieldChecksum(this.requireHandle(), name);
+  }
+
   /** packFields (qualityShaders.ts:37-73): fill the three render volumes
    *  (readField('packedFlow' | 'packedChemistry' | 'packedMedia')). */
   packFields() {

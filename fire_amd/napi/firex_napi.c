@@ -382,6 +382,132 @@ static napi_value js_field(napi_env env, napi_callback_info info, int write) {
 }
 
 static napi_value js_read_field(napi_env env, napi_callback_info info) { return js_field(env, info, 0); }
+
+static void set_num(napi_env env, napi_value obj, const char* key, double v);
+static napi_value dims_array(napi_env env, const int32_t* d);
+
+static napi_value hex64(napi_env env, uint64_t v) {
+  char buf[24];
+  snprintf(buf, sizeof buf, "%016llx", (unsigned long long)v);
+  napi_value out;
+  napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &out);
+  return out;
+}
+
+/* resolveOfflineDims(tier, override: [x, y, z] | null) -> [x, y, z]
+ * (resolveFireXOfflineDimensions, FireXEngine.ts:279-290) */
+static napi_value js_resolve_offline_dims(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  char tier[16];
+  if (!str_arg(env, argv[0], tier, sizeof tier)) {
+    napi_throw_type_error(env, NULL, "tier must be a string");
+    return NULL;
+  }
+  napi_valuetype t;
+  napi_typeof(env, argv[1], &t);
+  int32_t over[3], out[3];
+  const int32_t* op = NULL;
+  if (t == napi_object) {
+    for (uint32_t i = 0; i < 3; ++i) {
+      napi_value e;
+      double d = 0;
+      if (napi_get_element(env, argv[1], i, &e) != napi_ok || napi_get_value_double(env, e, &d) != napi_ok) {
+        napi_throw_type_error(env, NULL, "override must be an array of three numbers");
+        return NULL;
+      }
+      over[i] = (int32_t)d;
+    }
+    op = over;
+  }
+  FX_CHECK(env, firex_resolve_offline_dims(tier, op, out));
+  return dims_array(env, out);
+}
+
+/* fieldLayout(h, name) -> {kind, components, records, storedBytes} */
+static napi_value js_field_layout(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  char name[32];
+  if (!str_arg(env, argv[1], name, sizeof name) || field_id(name) < 0) {
+    napi_throw_range_error(env, NULL, "unknown Fire-X field");
+    return NULL;
+  }
+  firex_field_layout l;
+  FX_CHECK(env, firex_get_field_layout(s, field_id(name), &l));
+  static const char* kinds[] = {"f32", "f16", "bf16", "u32"};
+  napi_value o, k;
+  napi_create_object(env, &o);
+  napi_create_string_utf8(env, kinds[l.kind & 3], NAPI_AUTO_LENGTH, &k);
+  napi_set_named_property(env, o, "kind", k);
+  set_num(env, o, "kindCode", l.kind);
+  set_num(env, o, "components", l.components);
+  set_num(env, o, "records", (double)l.records);
+  set_num(env, o, "storedBytes", (double)l.stored_bytes);
+  return o;
+}
+
+/* readFieldRaw(h, name, out: Uint8Array of storedBytes) -> out */
+static napi_value js_read_field_raw(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  char name[32];
+  if (!str_arg(env, argv[1], name, sizeof name) || field_id(name) < 0) {
+    napi_throw_range_error(env, NULL, "unknown Fire-X field");
+    return NULL;
+  }
+  napi_typedarray_type type;
+  size_t length;
+  void* data;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[2], &is_ta);
+  if (!is_ta || napi_get_typedarray_info(env, argv[2], &type, &length, &data, NULL, NULL) != napi_ok ||
+      type != napi_uint8_array) {
+    napi_throw_type_error(env, NULL, "expected a Uint8Array");
+    return NULL;
+  }
+  FX_CHECK(env, firex_read_field(s, field_id(name), data, length, FIREX_DTYPE_RAW));
+  return argv[2];
+}
+
+/* fieldChecksum(h, name) -> 16 hex digits (firex_field_checksum) */
+static napi_value js_field_checksum(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  char name[32];
+  if (!str_arg(env, argv[1], name, sizeof name) || field_id(name) < 0) {
+    napi_throw_range_error(env, NULL, "unknown Fire-X field");
+    return NULL;
+  }
+  uint64_t h = 0;
+  FX_CHECK(env, firex_field_checksum(s, field_id(name), &h));
+  return hex64(env, h);
+}
+
+/* checksumBytes(data: Uint8Array) -> 16 hex digits (firex_checksum_bytes) */
+static napi_value js_checksum_bytes(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  napi_typedarray_type type;
+  size_t length;
+  void* data;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[0], &is_ta);
+  if (!is_ta || napi_get_typedarray_info(env, argv[0], &type, &length, &data, NULL, NULL) != napi_ok ||
+      type != napi_uint8_array) {
+    napi_throw_type_error(env, NULL, "expected a Uint8Array");
+    return NULL;
+  }
+  uint64_t h = 0;
+  FX_CHECK(env, firex_checksum_bytes(data, length, &h));
+  return hex64(env, h);
+}
 static napi_value js_write_field(napi_env env, napi_callback_info info) { return js_field(env, info, 1); }
 
 /* packFields(h): the three render volumes (qualityShaders.ts:37-73). */
@@ -529,6 +655,11 @@ static napi_value init(napi_env env, napi_value exports) {
       {"writeField", NULL, js_write_field, NULL, NULL, NULL, napi_default, NULL},
       {"reduce", NULL, js_reduce, NULL, NULL, NULL, napi_default, NULL},
       {"packFields", NULL, js_pack_fields, NULL, NULL, NULL, napi_default, NULL},
+      {"fieldLayout", NULL, js_field_layout, NULL, NULL, NULL, napi_default, NULL},
+      {"resolveOfflineDims", NULL, js_resolve_offline_dims, NULL, NULL, NULL, napi_default, NULL},
+      {"readFieldRaw", NULL, js_read_field_raw, NULL, NULL, NULL, napi_default, NULL},
+      {"fieldChecksum", NULL, js_field_checksum, NULL, NULL, NULL, napi_default, NULL},
+      {"checksumBytes", NULL, js_checksum_bytes, NULL, NULL, NULL, napi_default, NULL},
       {"getInfo", NULL, js_get_info, NULL, NULL, NULL, napi_default, NULL},
       {"estimateMemory", NULL, js_estimate_memory, NULL, NULL, NULL, napi_default, NULL},
       {"splitOfflineDelta", NULL, js_split_offline_delta, NULL, NULL, NULL, napi_default, NULL},

@@ -41,7 +41,8 @@ FIELDS = {
 VECTOR_FIELDS = {"flow", "species", "flow_predicted", "packed_flow", "packed_chemistry", "packed_media"}
 # per-particle fields: components per particle record
 PARTICLE_FIELDS = {"particles": 8, "particle_density": 4}
-DTYPE_F32, DTYPE_F64, DTYPE_U32 = 0, 1, 2
+DTYPE_F32, DTYPE_F64, DTYPE_U32, DTYPE_RAW = 0, 1, 2, 3
+KINDS = {0: "f32", 1: "f16", 2: "bf16", 3: "u32"}  # stored formats (firex_field_layout.kind)
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 HEAVY = {"auto": 0, "off": 1, "f16": 2, "f32": 3}
 PLANS = {"reference": 0, "fused": 1}
@@ -120,6 +121,11 @@ class Info(ctypes.Structure):
     ]
 
 
+class FieldLayout(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("components", ctypes.c_int32), ("records", ctypes.c_int64),
+                ("stored_bytes", ctypes.c_int64)]
+
+
 class MemoryEstimate(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "cell_count", "grid_bytes", "particle_bytes", "fixed_bytes", "total_bytes", "largest_storage_buffer_bytes")]
@@ -179,6 +185,9 @@ _SIGNATURES = {
     "firex_write_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
     "firex_reduce": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "firex_pack_fields": (ctypes.c_int, [_P]),
+    "firex_get_field_layout": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(FieldLayout)]),
+    "firex_field_checksum": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]),
+    "firex_checksum_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
     "firex_get_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
     "firex_set_pass_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "firex_get_pass_timing": (ctypes.c_int, [_P, ctypes.POINTER(PassTiming)]),
@@ -449,6 +458,29 @@ class FireXHipSolver:
         arr = np.ascontiguousarray(arr.reshape(self._shape(name)))
         _check(self._lib.firex_write_field(self._h, FIELDS[name], arr.ctypes.data, arr.nbytes, code))
 
+    def field_layout(self, name: str) -> dict:
+        """Stored format of a field: kind ('f32', 'f16', 'bf16', 'u32'),
+        components, records (cells or particles), stored_bytes."""
+        out = FieldLayout()
+        _check(self._lib.firex_get_field_layout(self._h, FIELDS[name], ctypes.byref(out)))
+        return {"kind": KINDS[out.kind], "components": out.components, "records": out.records,
+                "stored_bytes": out.stored_bytes}
+
+    def read_field_raw(self, name: str) -> bytes:
+        """The field's stored bytes, verbatim (f16 / bf16 species and
+        pressure as 2-byte values)."""
+        n = self.field_layout(name)["stored_bytes"]
+        buf = ctypes.create_string_buffer(max(1, n))
+        _check(self._lib.firex_read_field(self._h, FIELDS[name], buf, n, DTYPE_RAW))
+        return buf.raw[:n]
+
+    def field_checksum(self, name: str) -> int:
+        """64-bit checksum of the field's stored bytes, computed on the device
+        (definition: firex_field_checksum in include/firex.h)."""
+        out = ctypes.c_uint64()
+        _check(self._lib.firex_field_checksum(self._h, FIELDS[name], ctypes.byref(out)))
+        return out.value
+
     def pack_fields(self) -> None:
         """packFields: fill the three render volumes (read them as packed_flow,
         packed_chemistry, packed_media)."""
@@ -487,6 +519,15 @@ class FireXHipSolver:
                        "tuned_us_per_sweep": round(float(inf.jacobi_tuned_us_per_sweep), 3)},
             "particle_count": inf.particle_count,
         }
+
+
+def checksum_bytes(data) -> int:
+    """The field checksum of host bytes (the same definition as the device's)."""
+    mv = memoryview(data).cast("B")
+    buf = (ctypes.c_char * max(1, mv.nbytes)).from_buffer_copy(mv.tobytes() or b"\0")
+    out = ctypes.c_uint64()
+    _check(load_library().firex_checksum_bytes(buf, mv.nbytes, ctypes.byref(out)))
+    return out.value
 
 
 def seeded_emitters(dims: Sequence[int], count: int, radius_range=(6.0, 14.0), seed: int = 0,

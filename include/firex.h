@@ -63,7 +63,20 @@ enum {
   FIREX_FIELD_PARTICLE_DENSITY = 12 /* 4 f32: density, near density, pressure, near p.   */
 };
 
-enum { FIREX_DTYPE_F32 = 0, FIREX_DTYPE_F64 = 1, FIREX_DTYPE_U32 = 2 };
+/* Host dtypes of firex_read_field / firex_write_field.  FIREX_DTYPE_RAW
+ * (read only) copies the field's stored bytes verbatim (firex_get_field_layout
+ * gives their format and size). */
+enum { FIREX_DTYPE_F32 = 0, FIREX_DTYPE_F64 = 1, FIREX_DTYPE_U32 = 2, FIREX_DTYPE_RAW = 3 };
+
+/* Storage formats of a field on the device. */
+enum { FIREX_KIND_F32 = 0, FIREX_KIND_F16 = 1, FIREX_KIND_BF16 = 2, FIREX_KIND_U32 = 3 };
+
+typedef struct firex_field_layout {
+  int32_t kind;          /* FIREX_KIND_* of each component                    */
+  int32_t components;    /* per record                                         */
+  int64_t records;       /* cells, or particles for the per-particle fields    */
+  int64_t stored_bytes;  /* records * components * element size               */
+} firex_field_layout;
 
 /* Storage of the advected species field. Velocity, temperature, pressure and
  * all arithmetic stay f32. */
@@ -255,6 +268,20 @@ int firex_read_field(firex_solver* solver, int32_t field, void* dst, size_t byte
  * FIREX_FIELD_PARTICLES fills both particle buffers, as
  * writeParticleInitialState does (FireXEngine.ts:2030-2031). */
 int firex_write_field(firex_solver* solver, int32_t field, const void* src, size_t bytes, int32_t dtype);
+/* Stored format and size of a field (for FIREX_DTYPE_RAW reads and dumps). */
+int firex_get_field_layout(firex_solver* solver, int32_t field, firex_field_layout* out);
+/* 64-bit checksum of a field's stored bytes, computed on the device (an
+ * addition for auditable captures; the capture manifest of record-fire.mjs,
+ * scripts/record-fire.mjs:425-440, has no per-field record).  Definition:
+ * the bytes read as little-endian u32 words w_i (a final partial word
+ * zero-padded), checksum = sum over i of splitmix64((i << 32) | w_i) mod
+ * 2^64, where splitmix64(z) adds 0x9e3779b97f4a7c15 and applies the
+ * finaliser (z ^ z >> 30) * 0xbf58476d1ce4e5b9, (z ^ z >> 27) *
+ * 0x94d049bb133111eb, z ^ z >> 31.  Order-independent, so deterministic. */
+int firex_field_checksum(firex_solver* solver, int32_t field, uint64_t* out);
+/* The same checksum of host bytes (verifying dumped fields; no device). */
+int firex_checksum_bytes(const void* data, size_t bytes, uint64_t* out);
+
 /* packFields (src/sim/firex/qualityShaders.ts:37-73) as encodeRender runs it
  * (FireXEngine.ts:2585-2605): fills the three render volumes (flow,
  * chemistry, media; rgba16float, 24 B/cell, allocated on the first call) from
