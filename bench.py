@@ -146,6 +146,24 @@ def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict) -> dict:
             "deposited_liquid_mass": round(liquid, 3)}
 
 
+def audit_profile(solver, reps: int = 20) -> dict:
+    """The capture audit: device checksums of every solver field (wall time
+    per call, launch + 8-byte readback + sync included) and of the flow field
+    alone as an HBM stream (16 B/cell read once)."""
+    names = ("flow", "species", "pressure", "correction", "divergence", "reaction")
+    out = {}
+    for name in names:
+        solver.field_checksum(name)  # warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            solver.field_checksum(name)
+        us = (time.perf_counter() - t0) / reps * 1e6
+        nbytes = solver.field_layout(name)["stored_bytes"]
+        out[name] = {"us_per_call": round(us, 2), "bytes": nbytes, "GBps_wall": round(nbytes / (us * 1e-6) / 1e9, 1)}
+    out["all_fields_us"] = round(sum(v["us_per_call"] for v in out.values()), 2)
+    return out
+
+
 def measured_traffic(workload: str, plan: dict):
     """HBM bytes per Jacobi launch for this plan from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel), or
@@ -333,6 +351,7 @@ def main() -> int:
     print(f"bench: timed {args.steps} steps in {elapsed:.3f} s ({ms_per_step:.3f} ms/step)", file=sys.stderr, flush=True)
     passes = pass_profile(solver, args.profile_steps, cells, wl["storage"])
     passes["pack_fields"] = pack_profile(solver, cells, wl["storage"])
+    audit = audit_profile(solver)
     t_fuel = solver.reduce("sum_fuel")
     solver.close()
 
@@ -419,6 +438,7 @@ def main() -> int:
         "serving": serving,
         "plume_508": large,
         "droplets": droplets,
+        "capture_audit": audit,
         "check": {"sum_fuel_after_run": t_fuel},
     }
     print(json.dumps(line), flush=True)

@@ -50,8 +50,9 @@ void launch_liquid_deposit(const float4* current, const float4* previous, uint32
                            hipStream_t s);
 
 // Field checksum (fx_checksum_term summed over the words of `bytes` stored
-// bytes; data may be null for an all-zero field) into *out (device memory).
-void launch_checksum(const void* data, uint64_t bytes, uint64_t* out, hipStream_t s);
+// bytes; data may be null for an all-zero field) into *out (device memory),
+// through up to 1024 block partials in `partial`.
+void launch_checksum(const void* data, uint64_t bytes, uint64_t* partial, uint64_t* out, hipStream_t s);
 
 constexpr uint32_t kReducePartials = 1024;
 

@@ -911,7 +911,7 @@ int firex_field_checksum(firex_solver* s, int32_t field, uint64_t* out) {
   rc = field_view(s, field, &v);
   if (rc) return rc;
   uint64_t* result = (uint64_t*)(s->scratch + kReducePartials);
-  launch_checksum(v.ptr, field_stored_bytes(v), result, s->stream);
+  launch_checksum(v.ptr, field_stored_bytes(v), (uint64_t*)s->scratch, result, s->stream);
   rc = check_launch(s, "field checksum");
   if (rc) return rc;
   uint64_t h = 0;
