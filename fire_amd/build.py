@@ -19,8 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "libfirex_hip.so")
 ARCH = os.environ.get("FIREX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["firex_kernels.hip", "firex_jacobi.hip", "firex_droplets.hip", "firex_audit.hip", "firex_solver.hip",
-           "firex_server.hip", "firex_host.cpp"]
+SOURCES = ["firex_kernels.hip", "firex_jacobi.hip", "firex_droplets.hip", "firex_audit.hip", "firex_render.hip",
+           "firex_solver.hip", "firex_server.hip", "firex_host.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

@@ -104,7 +104,8 @@ bool params_get(const Params& p, const char* name, double* value) {
 
 // ---------------------------------------------------------------- uniforms
 void write_uniforms(const Params& p, const int32_t dims[3], double dt, double simulation_time, double frame_index,
-                    double time_step, bool cinematic, float u[FXU_COUNT], int32_t particle_count) {
+                    double time_step, bool cinematic, float u[FXU_COUNT], int32_t particle_count, int32_t canvas_width,
+                    int32_t canvas_height, int32_t ray_steps) {
   memset(u, 0, sizeof(float) * FXU_COUNT);
   double cells = (double)dims[0] * (double)dims[1] * (double)dims[2];
   u[FXU_GRID_X] = (float)dims[0];
@@ -126,6 +127,10 @@ void write_uniforms(const Params& p, const int32_t dims[3], double dt, double si
   u[FXU_CAMERA_YAW] = (float)p.cameraYaw;
   u[FXU_CAMERA_PITCH] = (float)p.cameraPitch;
   u[FXU_CAMERA_DISTANCE] = (float)p.cameraDistance;
+  u[FXU_ASPECT] = (float)((double)canvas_width / (double)std::max(canvas_height, 1));
+  u[FXU_CANVAS_W] = (float)canvas_width;
+  u[FXU_CANVAS_H] = (float)canvas_height;
+  u[FXU_RAY_STEPS] = (float)std::min(384, ray_steps);
   u[FXU_PARTICLES] = (float)particle_count;
   u[FXU_VIEW_MODE] = (float)p.viewMode;
   u[FXU_AIM_HEIGHT] = (float)p.aimHeight;
@@ -337,6 +342,80 @@ void particle_initial_state(const Params& p, int32_t count, bool active, float* 
   }
 }
 
+// createPeriodicDetailVolume, FireXEngine.ts:311-355: value noise of
+// frequencies 4, 8 and 14 over a period of size - 1 texels in r, g, b (alpha
+// 255).  The hash is JS 32-bit integer arithmetic (Math.imul, <<, >>>), the
+// interpolation float64, Math.round = floor(x + 0.5).
+void periodic_detail_volume(int size, uint8_t* out) {
+  const int period = size - 1;
+  auto hash = [](int x, int y, int z, uint32_t seed) -> double {
+    uint32_t v = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u) ^ seed;
+    v ^= v << 13;
+    v ^= v >> 17;
+    v ^= v << 5;
+    return (double)((v >> 24) & 255u);
+  };
+  auto fade = [](double v) { return v * v * (3 - 2 * v); };
+  auto mixd = [](double a, double b, double t) { return a + (b - a) * t; };
+  auto noise = [&](int x, int y, int z, int frequency, uint32_t seed) -> uint8_t {
+    const double px = ((double)x / period) * frequency;
+    const double py = ((double)y / period) * frequency;
+    const double pz = ((double)z / period) * frequency;
+    const int x0 = (int)std::floor(px) % frequency, y0 = (int)std::floor(py) % frequency;
+    const int z0 = (int)std::floor(pz) % frequency;
+    const int x1 = (x0 + 1) % frequency, y1 = (y0 + 1) % frequency, z1 = (z0 + 1) % frequency;
+    const double tx = fade(px - std::floor(px)), ty = fade(py - std::floor(py)), tz = fade(pz - std::floor(pz));
+    const double lower = mixd(mixd(hash(x0, y0, z0, seed), hash(x1, y0, z0, seed), tx),
+                              mixd(hash(x0, y1, z0, seed), hash(x1, y1, z0, seed), tx), ty);
+    const double upper = mixd(mixd(hash(x0, y0, z1, seed), hash(x1, y0, z1, seed), tx),
+                              mixd(hash(x0, y1, z1, seed), hash(x1, y1, z1, seed), tx), ty);
+    return (uint8_t)std::floor(mixd(lower, upper, tz) + 0.5);
+  };
+  for (int z = 0; z < size; ++z)
+    for (int y = 0; y < size; ++y)
+      for (int x = 0; x < size; ++x) {
+        uint8_t* o = out + ((size_t)x + (size_t)size * ((size_t)y + (size_t)size * z)) * 4;
+        o[0] = noise(x, y, z, 4, 0x51f15e5du);
+        o[1] = noise(x, y, z, 8, 0x6d2b79f5u);
+        o[2] = noise(x, y, z, 14, 0x1b873593u);
+        o[3] = 255;
+      }
+}
+
+FxRender make_render(const float u[FXU_COUNT], const int32_t dims[3], int32_t width, int32_t height) {
+  FxRender r;
+  memset(&r, 0, sizeof r);
+  r.width = width;
+  r.height = height;
+  const float steps = u[FXU_RAY_STEPS];
+  const uint32_t s = steps > 0.0f ? (uint32_t)steps : 0u;
+  r.steps = (int32_t)(s < 384u ? s : 384u);
+  r.nx = dims[0];
+  r.ny = dims[1];
+  r.nz = dims[2];
+  r.gx = u[FXU_GRID_X];
+  r.gy = u[FXU_GRID_Y];
+  r.gz = u[FXU_GRID_Z];
+  r.ex = fxm_max(u[FXU_DOMAIN_X], 0.001f);
+  r.ey = fxm_max(u[FXU_DOMAIN_Y], 0.001f);
+  r.ez = fxm_max(u[FXU_DOMAIN_Z], 0.001f);
+  r.yaw = u[FXU_CAMERA_YAW];
+  r.pitch = u[FXU_CAMERA_PITCH];
+  r.cam_distance = u[FXU_CAMERA_DISTANCE];
+  r.aspect = u[FXU_ASPECT];
+  r.render_z = steps;
+  r.view_mode = u[FXU_VIEW_MODE];
+  r.exposure = u[FXU_EXPOSURE];
+  r.front_definition = u[FXU_FRONT_DEFINITION];
+  r.cinematic = u[FXU_CINEMATIC];
+  r.detail_target = u[FXU_OPTICAL_DETAIL];
+  r.time = u[FXU_SIM_TIME];
+  r.chem_y = u[FXU_OXYGEN_RATE];
+  r.chem_z = u[FXU_HEAT_EFFICIENCY];
+  r.zoom = u[FXU_VIEW_ZOOM];
+  return r;
+}
+
 // ------------------------------------------------------------------- clock
 int32_t FixedStepClock::advance(double delta_seconds) {
   if (!std::isfinite(delta_seconds)) return 0;
@@ -383,20 +462,21 @@ struct Tier {
   double time_step;
   int32_t cinematic;
   int32_t particles;
+  int32_t ray_steps;
 };
 
 // FIREX_QUALITY (FireXEngine.ts:187-238) and FIREX_OFFLINE_QUALITY (:240-277).
 // The uniform's cinematic flag is set for the cinematic live tier and for
 // every offline tier (usesCinematicPresentation, FireXEngine.ts:2819-2821).
 const Tier kTiers[] = {
-    {"low", {32, 48, 32}, 14, 14, 1.0 / 30.0, 0, 384},
-    {"balanced", {40, 64, 40}, 24, 24, 1.0 / 30.0, 0, 640},
-    {"high", {64, 96, 64}, 36, 36, 1.0 / 30.0, 0, 1024},
-    {"maximum", {80, 120, 80}, 48, 48, 1.0 / 30.0, 0, 1536},
-    {"cinematic", {80, 120, 80}, 48, 48, 1.0 / 30.0, 1, 1536},
-    {"hd", {128, 192, 128}, 64, 64, 1.0 / 60.0, 1, 2048},
-    {"qhd", {160, 240, 160}, 80, 80, 1.0 / 60.0, 1, 3072},
-    {"uhd", {192, 288, 192}, 96, 96, 1.0 / 60.0, 1, 4096},
+    {"low", {32, 48, 32}, 14, 14, 1.0 / 30.0, 0, 384, 80},
+    {"balanced", {40, 64, 40}, 24, 24, 1.0 / 30.0, 0, 640, 112},
+    {"high", {64, 96, 64}, 36, 36, 1.0 / 30.0, 0, 1024, 144},
+    {"maximum", {80, 120, 80}, 48, 48, 1.0 / 30.0, 0, 1536, 192},
+    {"cinematic", {80, 120, 80}, 48, 48, 1.0 / 30.0, 1, 1536, 192},
+    {"hd", {128, 192, 128}, 64, 64, 1.0 / 60.0, 1, 2048, 256},
+    {"qhd", {160, 240, 160}, 80, 80, 1.0 / 60.0, 1, 3072, 320},
+    {"uhd", {192, 288, 192}, 96, 96, 1.0 / 60.0, 1, 4096, 384},
 };
 
 const int64_t kGridBytesPerCell = 116;       // gridConfiguration.ts:106
@@ -427,6 +507,7 @@ void firex_config_init(firex_config* c) {
   c->heavy_mode = FIREX_HEAVY_AUTO;
   c->plan = FIREX_PLAN_REFERENCE;
   c->particle_count = 640;
+  c->ray_steps = 112;
 }
 
 int firex_config_from_tier(firex_config* c, const char* tier) {
@@ -440,6 +521,7 @@ int firex_config_from_tier(firex_config* c, const char* tier) {
       c->maximum_substeps = 1;
       c->cinematic = t.cinematic;
       c->particle_count = t.particles;
+      c->ray_steps = t.ray_steps;
       return FIREX_OK;
     }
   }

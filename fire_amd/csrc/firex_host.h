@@ -35,7 +35,13 @@ double* params_field(Params& p, const char* name);  // numeric keys only
 
 // writeUniforms, FireXEngine.ts:2043-2113.
 void write_uniforms(const Params& p, const int32_t dims[3], double dt, double simulation_time, double frame_index,
-                    double time_step, bool cinematic, float out[FXU_COUNT], int32_t particle_count = 0);
+                    double time_step, bool cinematic, float out[FXU_COUNT], int32_t particle_count = 0,
+                    int32_t canvas_width = 0, int32_t canvas_height = 0, int32_t ray_steps = 0);
+
+// createPeriodicDetailVolume, FireXEngine.ts:311-355: size^3 rgba8 texels.
+void periodic_detail_volume(int size, uint8_t* out);
+// The ray marcher's uniform slots and output size.
+FxRender make_render(const float u[FXU_COUNT], const int32_t dims[3], int32_t width, int32_t height);
 
 // waterSimulationActive, FireXEngine.ts:2537-2539.
 inline bool water_active(const Params& p) { return p.waterEnabled && p.waterFlow > 0.001; }

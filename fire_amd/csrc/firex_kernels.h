@@ -54,6 +54,11 @@ void launch_liquid_deposit(const float4* current, const float4* previous, uint32
 // through up to 1024 block partials in `partial`.
 void launch_checksum(const void* data, uint64_t bytes, uint64_t* partial, uint64_t* out, hipStream_t s);
 
+// The volume ray marcher: packed = the three render volumes (4 x f16 per
+// cell, flow / chemistry / media), detail = 64^3 rgba8 texels, out = width x
+// height rgba16f pixels, row 0 at the top.
+void launch_render_volume(const void* packed, const uint32_t* detail, void* out, const FxRender& r, hipStream_t s);
+
 constexpr uint32_t kReducePartials = 1024;
 
 }  // namespace fx

@@ -52,6 +52,12 @@ struct firex_solver {
   int particle_index = 0;
   bool liquid_may_contain_water = false;  // FireXEngine.ts:399
   bool particle_field_active = false;     // FireXEngine.ts:400
+  // Volume rendering: the 64^3 rgba8 detail volume (uploaded on first use)
+  // and the rgba16f scene image of the last firex_render_volume.
+  uint32_t* detail = nullptr;
+  void* scene = nullptr;
+  uint64_t scene_bytes = 0;
+  int32_t scene_width = 0, scene_height = 0;
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -533,6 +539,13 @@ int field_view(firex_solver* s, int32_t field, FieldView* v) {
       v->records = (uint64_t)s->cfg.particle_count;
       v->comps = 4;
       break;
+    case FIREX_FIELD_SCENE:
+      if (!s->scene) return set_error(FIREX_ERR_INVALID, "the scene image is written by firex_render_volume");
+      v->ptr = s->scene;
+      v->records = (uint64_t)s->scene_width * (uint64_t)s->scene_height;
+      v->comps = 4;
+      v->kind = FIREX_KIND_F16;
+      break;
     default: return set_error(FIREX_ERR_INVALID, "unknown field");
   }
   return FIREX_OK;
@@ -1007,6 +1020,53 @@ int firex_pack_fields(firex_solver* s) {
                        s->divergence, s->packed, st, divergence_view, s->stream);
   });
   return check_launch(s, "packFields");
+}
+
+int firex_render_volume(firex_solver* s, const firex_render_config* rc) {
+  int rc0 = check_usable(s);
+  if (rc0) return rc0;
+  if (!rc) return set_error(FIREX_ERR_INVALID, "render config is null");
+  if (rc->width < 1 || rc->height < 1 || rc->width > 16384 || rc->height > 16384)
+    return set_error(FIREX_ERR_INVALID, "render width and height must be in [1, 16384]");
+  const int32_t steps = rc->ray_steps > 0 ? rc->ray_steps : s->cfg.ray_steps;
+  if (steps < 1) return set_error(FIREX_ERR_INVALID, "ray steps must be positive");
+  int rc1 = firex_pack_fields(s);
+  if (rc1) return rc1;
+  if (!s->detail) {
+    const uint64_t bytes = (uint64_t)kFxDetailSize * kFxDetailSize * kFxDetailSize * 4;
+    std::vector<uint8_t> host(bytes);
+    periodic_detail_volume(kFxDetailSize, host.data());
+    rc1 = alloc(s, (void**)&s->detail, bytes, "detail volume");
+    if (rc1) return rc1;
+    s->bytes += bytes;
+    FX_HIP(s, hipMemcpyAsync(s->detail, host.data(), bytes, hipMemcpyHostToDevice, s->stream), "detail upload");
+    FX_HIP(s, hipStreamSynchronize(s->stream), "detail upload");
+  }
+  const uint64_t image_bytes = (uint64_t)rc->width * (uint64_t)rc->height * 8;
+  if (image_bytes > s->scene_bytes) {
+    if (s->cfg.memory_budget_bytes && s->bytes + image_bytes - s->scene_bytes > s->cfg.memory_budget_bytes)
+      return set_error(FIREX_ERR_BUDGET, "the scene image (" + std::to_string(image_bytes) +
+                                             " bytes) does not fit the memory budget");
+    void* p = nullptr;
+    rc1 = alloc(s, &p, image_bytes, "scene image");
+    if (rc1) return rc1;
+    if (s->scene) {
+      FX_HIP(s, hipStreamSynchronize(s->stream), "scene image resize");
+      s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), s->scene));
+      hipFree(s->scene);
+    }
+    s->bytes += image_bytes - s->scene_bytes;
+    s->scene = p;
+    s->scene_bytes = image_bytes;
+  }
+  s->scene_width = rc->width;
+  s->scene_height = rc->height;
+  float u[FXU_COUNT];
+  write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
+                 s->cfg.cinematic != 0, u, s->cfg.particle_count, rc->width, rc->height, steps);
+  const FxRender r = make_render(u, s->dims, rc->width, rc->height);
+  timed(s, FIREX_PASS_RENDER, [&] { launch_render_volume(s->packed, s->detail, s->scene, r, s->stream); });
+  return check_launch(s, "volume render");
 }
 
 int firex_reduce(firex_solver* s, int32_t which, double* out) {

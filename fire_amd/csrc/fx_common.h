@@ -82,6 +82,32 @@ struct FxDroplets {
   float aim, nozzle;           // params.controls.yz
 };
 
+// Per-frame constants of the volume ray marcher (uniform slots read by
+// volumeFragment, qualityShaders.ts:407-575, and the output size).
+struct FxRender {
+  int32_t width, height;       // output pixels (render.xy)
+  int32_t steps;               // min(u32(render.z), 384)
+  int32_t nx, ny, nz;
+  float gx, gy, gz;            // params.grid.xyz
+  float ex, ey, ez;            // domainExtent()
+  float yaw, pitch, cam_distance, aspect;  // params.camera
+  float render_z;              // params.render.z (ray steps as f32)
+  float view_mode;             // params.controls.x
+  float exposure;              // params.controls.w
+  float front_definition;      // params.mixing.y
+  float cinematic;             // params.mixing.z
+  float detail_target;         // params.mixing.w
+  float time;                  // params.time.y
+  float chem_y, chem_z;        // params.chemistry.yz
+  float zoom;                  // params.fire.w
+};
+
+// Edge of the periodic detail volume (FireXEngine.ts:1622) and the WGSL
+// constant 1.0 / 2.2 as an f32 (a const-expression, evaluated exactly and
+// rounded once).
+#define kFxDetailSize 64
+#define kFxInverseGamma ((float)(1.0 / 2.2))
+
 #if defined(__HIP__)
 // Keeps an f32 value as computed.  Without it the compiler may fold
 // "f32 multiply, then convert to f16" into one mixed-precision FMA that

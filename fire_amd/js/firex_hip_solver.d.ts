@@ -6,7 +6,9 @@ export type FireXFieldName =
   | 'flow' | 'species' | 'flowPredicted' | 'pressure' | 'correction' | 'divergence' | 'reaction' | 'liquid'
   | 'packedFlow' | 'packedChemistry' | 'packedMedia'
   /** per SPH droplet: 8 (positionAge, velocityMass) and 4 (density, near density, pressures) values */
-  | 'particles' | 'particleDensity';
+  | 'particles' | 'particleDensity'
+  /** the last renderVolume image: width x height pixels of 4 values (rgba16f), row 0 at the top */
+  | 'scene';
 
 export type FireXReductionName =
   | 'sumFuel' | 'sumOxygen' | 'sumProducts' | 'sumVapor' | 'sumTemperature' | 'sumReaction'
@@ -32,6 +34,8 @@ export interface FireXHipSolverOptions {
   captureGraphs?: number;
   /** SPH droplets of the resource set (384-4096 by tier, FireXEngine.ts:187-277). */
   particleCount?: number;
+  /** Ray-march samples per pixel (raySteps by tier, 80-384). */
+  raySteps?: number;
   scene?: Readonly<Record<string, number | boolean>>;
 }
 
@@ -75,6 +79,8 @@ export declare class FireXHipSolver {
   reduce(name: FireXReductionName): number;
   /** packFields: fill the render volumes read back as packedFlow / packedChemistry / packedMedia. */
   packFields(): void;
+  /** Volume ray marcher into the rgba16f scene image (readField('scene')). */
+  renderVolume(width: number, height: number, raySteps?: number): void;
   /** Stored format of a field on the device. */
   fieldLayout(name: FireXFieldName): FireXFieldLayout;
   /** The field's stored bytes, verbatim. */

@@ -113,17 +113,23 @@ class FireXHipSolver {
 
   /** Copies a field to a new (or the given) Float32Array / Float64Array / Uint32Array. */
   readField(name, out) {
-    const info = native.getInfo(this.requireHandle());
-    const perParticle = PARTICLE_COMPONENTS[name];
-    const components = perParticle || FIELD_COMPONENTS[name];
-    if (!components) throw new RangeError(`Unknown Fire-X field: ${name}`);
-    const length = (perParticle ? info.particleCount : info.cellCount) * components;
+    if (!(name in FIELD_COMPONENTS) && !(name in PARTICLE_COMPONENTS) && name !== 'scene') {
+      throw new RangeError(`Unknown Fire-X field: ${name}`);
+    }
+    const layout = native.fieldLayout(this.requireHandle(), name);
+    const length = layout.records * layout.components;
     const target = out || (name === 'liquid' ? new Uint32Array(length) : new Float32Array(length));
     return native.readField(this.handle, name, target);
   }
 
   writeField(name, data) {
     native.writeField(this.requireHandle(), name, data);
+  }
+
+  /** The cinematic scene pass's volume ray marcher (qualityShaders.ts:374-575): packs the render
+   *  volumes and renders width x height rgba16f pixels, read back as readField('scene'). */
+  renderVolume(width, height, raySteps = 0) {
+    native.renderVolume(this.requireHandle(), { width, height, raySteps });
   }
 
   /** Stored format of a field: {kind: 'f32'|'f16'|'bf16'|'u32', components, records, storedBytes}. */

@@ -161,6 +161,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.capture_graphs = (int32_t)num_or(env, argv[0], "captureGraphs", 0);
     cfg.jacobi_autotune = (int32_t)num_or(env, argv[0], "jacobiAutotune", 0);
     cfg.particle_count = (int32_t)num_or(env, argv[0], "particleCount", cfg.particle_count);
+    cfg.ray_steps = (int32_t)num_or(env, argv[0], "raySteps", cfg.ray_steps);
     if (str_prop(env, argv[0], "speciesStorage", buf, sizeof buf)) cfg.species_storage = lookup(buf, storage_names, 3);
     if (str_prop(env, argv[0], "heavyMode", buf, sizeof buf)) cfg.heavy_mode = lookup(buf, heavy_names, 4);
     if (str_prop(env, argv[0], "plan", buf, sizeof buf)) cfg.plan = lookup(buf, plan_names, 2);
@@ -334,7 +335,7 @@ static int field_id(const char* name) {
   static const char* names[] = {"flow",          "species",    "pressure",        "correction",
                                 "divergence",    "reaction",   "liquid",          "flowPredicted",
                                 "packedFlow",    "packedChemistry", "packedMedia",
-                                "particles",     "particleDensity"};
+                                "particles",     "particleDensity", "scene"};
   for (int i = 0; i < (int)(sizeof names / sizeof names[0]); ++i)
     if (strcmp(name, names[i]) == 0) return i;
   return -1;
@@ -392,6 +393,22 @@ static napi_value hex64(napi_env env, uint64_t v) {
   napi_value out;
   napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &out);
   return out;
+}
+
+/* renderVolume(h, {width, height, raySteps}): the volume ray marcher
+ * (qualityShaders.ts:374-575) into the rgba16f scene image ("scene"). */
+static napi_value js_render_volume(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  firex_render_config rc;
+  memset(&rc, 0, sizeof rc);
+  rc.width = (int32_t)num_or(env, argv[1], "width", 0);
+  rc.height = (int32_t)num_or(env, argv[1], "height", 0);
+  rc.ray_steps = (int32_t)num_or(env, argv[1], "raySteps", 0);
+  FX_CHECK(env, firex_render_volume(s, &rc));
+  return NULL;
 }
 
 /* resolveOfflineDims(tier, override: [x, y, z] | null) -> [x, y, z]
@@ -657,6 +674,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"packFields", NULL, js_pack_fields, NULL, NULL, NULL, napi_default, NULL},
       {"fieldLayout", NULL, js_field_layout, NULL, NULL, NULL, napi_default, NULL},
       {"resolveOfflineDims", NULL, js_resolve_offline_dims, NULL, NULL, NULL, napi_default, NULL},
+      {"renderVolume", NULL, js_render_volume, NULL, NULL, NULL, napi_default, NULL},
       {"readFieldRaw", NULL, js_read_field_raw, NULL, NULL, NULL, napi_default, NULL},
       {"fieldChecksum", NULL, js_field_checksum, NULL, NULL, NULL, napi_default, NULL},
       {"checksumBytes", NULL, js_checksum_bytes, NULL, NULL, NULL, napi_default, NULL},

@@ -36,7 +36,7 @@ FIELDS = {
     "flow": 0, "species": 1, "pressure": 2, "correction": 3,
     "divergence": 4, "reaction": 5, "liquid": 6, "flow_predicted": 7,
     "packed_flow": 8, "packed_chemistry": 9, "packed_media": 10,
-    "particles": 11, "particle_density": 12,
+    "particles": 11, "particle_density": 12, "scene": 13,
 }
 VECTOR_FIELDS = {"flow", "species", "flow_predicted", "packed_flow", "packed_chemistry", "packed_media"}
 # per-particle fields: components per particle record
@@ -91,7 +91,8 @@ class Config(ctypes.Structure):
         ("jacobi_kernel", ctypes.c_int32),
         ("jacobi_autotune", ctypes.c_int32),
         ("particle_count", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("ray_steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -126,6 +127,11 @@ class FieldLayout(ctypes.Structure):
                 ("stored_bytes", ctypes.c_int64)]
 
 
+class RenderConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("ray_steps", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
+
+
 class MemoryEstimate(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "cell_count", "grid_bytes", "particle_bytes", "fixed_bytes", "total_bytes", "largest_storage_buffer_bytes")]
@@ -136,13 +142,15 @@ class DeviceLimits(ctypes.Structure):
         "max_texture_dimension_3d", "max_storage_buffer_binding_size", "max_buffer_size")]
 
 
-PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack", "droplets")
+PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack", "droplets",
+          "render")
+PASS_COUNT = 10  # FIREX_PASS_COUNT
 
 
 class PassTiming(ctypes.Structure):
     _fields_ = [
-        ("milliseconds", ctypes.c_double * 8),
-        ("launches", ctypes.c_int64 * 8),
+        ("milliseconds", ctypes.c_double * PASS_COUNT),
+        ("launches", ctypes.c_int64 * PASS_COUNT),
         ("jacobi_sweeps", ctypes.c_int64),
         ("steps", ctypes.c_int64),
     ]
@@ -186,6 +194,7 @@ _SIGNATURES = {
     "firex_reduce": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "firex_pack_fields": (ctypes.c_int, [_P]),
     "firex_get_field_layout": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(FieldLayout)]),
+    "firex_render_volume": (ctypes.c_int, [_P, ctypes.POINTER(RenderConfig)]),
     "firex_field_checksum": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]),
     "firex_checksum_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
     "firex_get_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
@@ -337,7 +346,7 @@ class FireXHipSolver:
                  cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
                  jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0,
                  jacobi_kernel: str = "auto", jacobi_autotune: bool = False,
-                 particle_count: Optional[int] = None):
+                 particle_count: Optional[int] = None, ray_steps: Optional[int] = None):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -365,6 +374,8 @@ class FireXHipSolver:
         cfg.jacobi_autotune = int(bool(jacobi_autotune))
         if particle_count is not None:
             cfg.particle_count = int(particle_count)
+        if ray_steps is not None:
+            cfg.ray_steps = int(ray_steps)
         self.config = cfg
         self.dims = tuple(int(v) for v in cfg.dims)
         self.cells = self.dims[0] * self.dims[1] * self.dims[2]
@@ -433,6 +444,9 @@ class FireXHipSolver:
 
     # fields -----------------------------------------------------------------
     def _shape(self, name: str):
+        if name == "scene":
+            w, h = getattr(self, "_scene_size", (0, 0))
+            return (h, w, 4)
         if name in PARTICLE_FIELDS:
             return (int(self.config.particle_count), PARTICLE_FIELDS[name])
         nx, ny, nz = self.dims
@@ -480,6 +494,15 @@ class FireXHipSolver:
         out = ctypes.c_uint64()
         _check(self._lib.firex_field_checksum(self._h, FIELDS[name], ctypes.byref(out)))
         return out.value
+
+    def render_volume(self, width: int, height: int, ray_steps: int = 0) -> None:
+        """The cinematic scene pass's volume ray marcher: packs the render
+        volumes, then renders width x height rgba16f pixels (read them as the
+        'scene' field, shape (height, width, 4), row 0 at the top).
+        ray_steps 0 uses the configuration's (the tier's raySteps)."""
+        cfg = RenderConfig(int(width), int(height), int(ray_steps))
+        _check(self._lib.firex_render_volume(self._h, ctypes.byref(cfg)))
+        self._scene_size = (int(width), int(height))
 
     def pack_fields(self) -> None:
         """packFields: fill the three render volumes (read them as packed_flow,

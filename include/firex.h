@@ -60,7 +60,10 @@ enum {
   /* per-particle fields: particle_count records (not cells) */
   FIREX_FIELD_PARTICLES = 11,      /* 8 f32: positionAge (domain fractions, age s),
                                       velocityMass (m/s, mass); the current buffer       */
-  FIREX_FIELD_PARTICLE_DENSITY = 12 /* 4 f32: density, near density, pressure, near p.   */
+  FIREX_FIELD_PARTICLE_DENSITY = 12,/* 4 f32: density, near density, pressure, near p.   */
+  /* image of the last firex_render_volume: width * height pixels of 4 x f16
+     (rgba16float), row 0 at the top */
+  FIREX_FIELD_SCENE = 13
 };
 
 /* Host dtypes of firex_read_field / firex_write_field.  FIREX_DTYPE_RAW
@@ -119,7 +122,9 @@ typedef struct firex_config {
                                     above left at 0 are searched (cached per shape)  */
   int32_t particle_count;        /* SPH droplets of the resource set (384-4096 by
                                     tier, FireXEngine.ts:187-277); 0 = none         */
-  int32_t reserved[2];
+  int32_t ray_steps;             /* ray-march samples per pixel (raySteps by tier,
+                                    80-384; firex_render_volume caps it at 384)    */
+  int32_t reserved;
 } firex_config;
 
 /* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */
@@ -291,6 +296,23 @@ int firex_checksum_bytes(const void* data, size_t bytes, uint64_t* out);
  * firex_read_field(FIREX_FIELD_PACKED_*). */
 int firex_pack_fields(firex_solver* solver);
 
+/* The volume ray marcher of the cinematic scene pass (volumeFragment,
+ * src/sim/firex/qualityShaders.ts:374-575, drawn by encodeRender,
+ * FireXEngine.ts:2607-2640).  Runs firex_pack_fields, then marches one ray
+ * per pixel through the three render volumes into an rgba16float HDR image of
+ * width x height (the reference's cinematic scene texture), readable as
+ * FIREX_FIELD_SCENE.  Camera, view mode, exposure, zoom and optical detail
+ * come from the solver's parameters (cameraYaw, cameraPitch, cameraDistance,
+ * viewMode, exposure, viewZoom, frontDefinition, opticalDetailTarget).
+ * Texture filtering is defined in f32 arithmetic (linear, clamp-to-edge for
+ * the volumes, repeat for the 64^3 detail volume). */
+typedef struct firex_render_config {
+  int32_t width, height;  /* output pixels, each in [1, 16384]                 */
+  int32_t ray_steps;      /* 0 = firex_config.ray_steps                         */
+  int32_t reserved[5];
+} firex_render_config;
+int firex_render_volume(firex_solver* solver, const firex_render_config* config);
+
 /* FP64 reduction on the device (FIREX_REDUCE_*). */
 int firex_reduce(firex_solver* solver, int32_t which, double* out);
 int firex_get_info(firex_solver* solver, firex_info* info);
@@ -305,7 +327,8 @@ enum {
   FIREX_PASS_FILL_COPY = 5,           /* clears and copyFlow                      */
   FIREX_PASS_PACK = 6,                /* firex_pack_fields                        */
   FIREX_PASS_DROPLETS = 7,            /* SPH density, integration, deposition     */
-  FIREX_PASS_COUNT = 8
+  FIREX_PASS_RENDER = 8,              /* firex_render_volume's ray march          */
+  FIREX_PASS_COUNT = 10
 };
 typedef struct firex_pass_timing {
   double milliseconds[FIREX_PASS_COUNT];  /* summed device time per pass kind     */

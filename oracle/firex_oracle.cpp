@@ -22,6 +22,9 @@
 //   * field reset (initializeFields host side):  FireXEngine.ts:1933-1974
 //   * pass order / ping-pong / warm start:       FireXEngine.ts:2115-2276
 //   * parameter defaults and clamps:             FireXEngine.ts:422-451, 937-1032
+//   * volume ray marcher of the cinematic scene pass and its periodic detail
+//     volume:                                    qualityShaders.ts:198-575,
+//                                                FireXEngine.ts:311-355
 //   * SPH droplets (density, integration, liquid deposition) and the host
 //     particle state:                            shaders.ts:514-825,
 //                                                FireXEngine.ts:1976-2034, 2158-2171
@@ -176,6 +179,16 @@ bool set_parameter(Params& p, const char* name, double v) {
   else if (n == "domainHeight") p.domainHeight = clampd(finite_or(v, p.domainHeight), 0.5, 6);
   else if (n == "domainDepth") p.domainDepth = clampd(finite_or(v, p.domainDepth), 0.5, 4);
   else if (n == "viewMode") p.viewMode = clampd(std::round(finite_or(v, p.viewMode)), 0, 7);  // FireXEngine.ts:994-995
+  else if (n == "cameraYaw") p.cameraYaw = finite_or(v, p.cameraYaw);
+  else if (n == "cameraPitch") p.cameraPitch = finite_or(v, p.cameraPitch);
+  else if (n == "cameraDistance") p.cameraDistance = finite_or(v, p.cameraDistance);
+  else if (n == "exposure") p.exposure = clampd(finite_or(v, p.exposure), 0.25, 3);
+  else if (n == "frontDefinition") p.frontDefinition = clampd(finite_or(v, p.frontDefinition), 0.5, 2.5);
+  else if (n == "viewZoom") p.viewZoom = clampd(finite_or(v, p.viewZoom), 0.5, 5);
+  else if (n == "opticalDetailTarget") {
+    double t = finite_or(v, p.opticalDetailTarget);
+    p.opticalDetailTarget = t >= 768 ? 1024 : t >= 384 ? 512 : t >= 192 ? 256 : 0;  // FireXEngine.ts:1009-1011
+  }
   else return false;
   return true;
 }
@@ -248,6 +261,8 @@ template <typename T> struct State {
     u[12] = (float)params.waterFlow; u[13] = (float)(params.sprayAngle * M_PI / 180.0);
     u[14] = params.waterEnabled ? 1.0f : 0.0f; u[15] = 300.0f;
     u[16] = (float)params.cameraYaw; u[17] = (float)params.cameraPitch; u[18] = (float)params.cameraDistance;
+    u[19] = (float)((double)canvasWidth / (double)std::max(canvasHeight, 1));
+    u[20] = (float)canvasWidth; u[21] = (float)canvasHeight; u[22] = (float)std::min(384, raySteps);
     u[23] = (float)particleCount;
     u[24] = (float)params.viewMode; u[25] = (float)params.aimHeight; u[26] = (float)params.nozzleType;
     u[27] = (float)params.exposure;
@@ -902,6 +917,360 @@ template <typename T> struct State {
     }
   }
 
+  // ------------------------------------------------------------- rendering
+  // The cinematic scene pass's volume ray marcher (volumeFragment,
+  // qualityShaders.ts:374-575) over the packed render volumes, FP32 only.
+  // Texture filtering is linear interpolation in f32 (texel t = uvw * size
+  // - 0.5; x, then y, then z; clamp-to-edge for the render volumes, repeat
+  // for the detail volume, rgba8 texels as byte / 255); a pixel (x, y) has
+  // uv = ((x + 0.5) / w, 1 - (y + 0.5) / h) and position (x + 0.5, y + 0.5).
+  int canvasWidth = 0, canvasHeight = 0, raySteps = 0;
+
+  // createPeriodicDetailVolume, FireXEngine.ts:311-355.
+  static std::vector<uint8_t> detail_volume(int size) {
+    std::vector<uint8_t> data((size_t)size * size * size * 4);
+    const int period = size - 1;
+    auto hash = [](int32_t x, int32_t y, int32_t z, int32_t seed) {
+      uint32_t v = (uint32_t)(int32_t)((uint32_t)x * 73856093u) ^ (uint32_t)(int32_t)((uint32_t)y * 19349663u) ^
+                   (uint32_t)(int32_t)((uint32_t)z * 83492791u) ^ (uint32_t)seed;
+      v ^= v << 13;
+      v ^= v >> 17;
+      v ^= v << 5;
+      return (double)((v >> 24) & 255u);
+    };
+    auto fade = [](double v) { return v * v * (3 - 2 * v); };
+    auto lerp = [](double a, double b, double t) { return a + (b - a) * t; };
+    auto noise = [&](int x, int y, int z, int f, int32_t seed) {
+      double px = (double)x / period * f, py = (double)y / period * f, pz = (double)z / period * f;
+      int x0 = (int)std::floor(px) % f, y0 = (int)std::floor(py) % f, z0 = (int)std::floor(pz) % f;
+      int x1 = (x0 + 1) % f, y1 = (y0 + 1) % f, z1 = (z0 + 1) % f;
+      double tx = fade(px - std::floor(px)), ty = fade(py - std::floor(py)), tz = fade(pz - std::floor(pz));
+      double lo = lerp(lerp(hash(x0, y0, z0, seed), hash(x1, y0, z0, seed), tx),
+                       lerp(hash(x0, y1, z0, seed), hash(x1, y1, z0, seed), tx), ty);
+      double hi = lerp(lerp(hash(x0, y0, z1, seed), hash(x1, y0, z1, seed), tx),
+                       lerp(hash(x0, y1, z1, seed), hash(x1, y1, z1, seed), tx), ty);
+      return (uint8_t)std::floor(lerp(lo, hi, tz) + 0.5);  // Math.round
+    };
+    for (int z = 0; z < size; ++z)
+      for (int y = 0; y < size; ++y)
+        for (int x = 0; x < size; ++x) {
+          uint8_t* o = &data[((size_t)x + (size_t)size * ((size_t)y + (size_t)size * z)) * 4];
+          o[0] = noise(x, y, z, 4, (int32_t)0x51f15e5d);
+          o[1] = noise(x, y, z, 8, (int32_t)0x6d2b79f5);
+          o[2] = noise(x, y, z, 14, (int32_t)0x1b873593);
+          o[3] = 255;
+        }
+    return data;
+  }
+
+  struct Renderer {
+    typedef V3<float> F3;
+    const State<float>& s;
+    const std::vector<V4<float>>& vf;
+    const std::vector<V4<float>>& vc;
+    const std::vector<V4<float>>& vm;
+    const std::vector<uint8_t>& det;
+    const float* u;
+    int width, height;
+
+    static F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+    static F3 plus(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+    static F3 minus(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+    static F3 scale(F3 a, float k) { return f3(a.x * k, a.y * k, a.z * k); }
+    static F3 times(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+    static float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+    static float len(F3 a) { return fxm_sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+    static F3 unit(F3 a) { float l = len(a); return f3(a.x / l, a.y / l, a.z / l); }
+    static F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+    static F3 lerp3(F3 a, F3 b, float t) { return f3(fxm_mix(a.x, b.x, t), fxm_mix(a.y, b.y, t), fxm_mix(a.z, b.z, t)); }
+    static V4<float> lerp4(V4<float> a, V4<float> b, float t) {
+      return {fxm_mix(a.x, b.x, t), fxm_mix(a.y, b.y, t), fxm_mix(a.z, b.z, t), fxm_mix(a.w, b.w, t)};
+    }
+
+    float ext(int a) const { return fxm_max(u[32 + a], 0.001f); }
+
+    V4<float> filtered(const std::vector<V4<float>>& v, float ux, float uy, float uz) const {
+      const int n[3] = {s.nx, s.ny, s.nz};
+      const float g[3] = {u[0], u[1], u[2]};
+      const float c[3] = {ux, uy, uz};
+      int i0[3], i1[3];
+      float fr[3];
+      for (int a = 0; a < 3; ++a) {
+        float t = c[a] * g[a] - 0.5f;
+        float fl = fxm_floor(t);
+        fr[a] = t - fl;
+        int i = (int)fl;
+        i0[a] = std::min(std::max(i, 0), n[a] - 1);
+        i1[a] = std::min(std::max(i + 1, 0), n[a] - 1);
+      }
+      auto at = [&](int x, int y, int z) { return v[(size_t)x + (size_t)s.nx * ((size_t)y + (size_t)s.ny * z)]; };
+      V4<float> a = lerp4(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), fr[0]);
+      V4<float> b = lerp4(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), fr[0]);
+      V4<float> cc = lerp4(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), fr[0]);
+      V4<float> d = lerp4(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), fr[0]);
+      return lerp4(lerp4(a, b, fr[1]), lerp4(cc, d, fr[1]), fr[2]);
+    }
+
+    // sampleFlow / sampleChemistry / sampleMedia, qualityShaders.ts:262-278
+    V4<float> field(const std::vector<V4<float>>& v, F3 p) const {
+      float h[3] = {0.5f / u[0], 0.5f / u[1], 0.5f / u[2]};
+      float q[3] = {p.x / ext(0), p.y / ext(1), p.z / ext(2)};
+      for (int a = 0; a < 3; ++a) q[a] = fxm_clamp(q[a], h[a], 1.0f - h[a]);
+      return filtered(v, q[0], q[1], q[2]);
+    }
+
+    // sampleDetail, qualityShaders.ts:280-282
+    V4<float> detail(F3 p) const {
+      const int S = 64;
+      float c[3] = {p.x, p.y, p.z};
+      int i0[3];
+      float fr[3];
+      for (int a = 0; a < 3; ++a) {
+        float t = c[a] * (float)S - 0.5f;
+        float fl = fxm_floor(t);
+        fr[a] = t - fl;
+        i0[a] = (int)fl;
+      }
+      auto wrap = [&](int i) { return ((i % S) + S) % S; };
+      auto at = [&](int x, int y, int z) {
+        const uint8_t* t = &det[((size_t)wrap(x) + (size_t)S * ((size_t)wrap(y) + (size_t)S * wrap(z))) * 4];
+        return V4<float>{(float)t[0] / 255.0f, (float)t[1] / 255.0f, (float)t[2] / 255.0f, (float)t[3] / 255.0f};
+      };
+      int x = i0[0], y = i0[1], z = i0[2];
+      V4<float> a = lerp4(at(x, y, z), at(x + 1, y, z), fr[0]);
+      V4<float> b = lerp4(at(x, y + 1, z), at(x + 1, y + 1, z), fr[0]);
+      V4<float> cc = lerp4(at(x, y, z + 1), at(x + 1, y, z + 1), fr[0]);
+      V4<float> d = lerp4(at(x, y + 1, z + 1), at(x + 1, y + 1, z + 1), fr[0]);
+      return lerp4(lerp4(a, b, fr[1]), lerp4(cc, d, fr[1]), fr[2]);
+    }
+
+    static F3 blackbody(float temp) {
+      float t = fxm_clamp((temp - 720.0f) / 1900.0f, 0.0f, 1.0f);
+      F3 red = f3(1.0f, 0.035f, 0.002f), orange = f3(1.0f, 0.24f, 0.008f);
+      F3 gold = f3(1.0f, 0.68f, 0.12f), white = f3(1.0f, 0.94f, 0.72f);
+      return lerp3(lerp3(red, orange, fxm_smoothstep(0.0f, 0.32f, t)), lerp3(gold, white, fxm_smoothstep(0.58f, 1.0f, t)),
+                   fxm_smoothstep(0.26f, 0.72f, t));
+    }
+    static F3 thermal_map(float value) {
+      float t = fxm_clamp(value, 0.0f, 1.0f);
+      F3 cold = f3(0.015f, 0.025f, 0.12f), blue = f3(0.02f, 0.36f, 1.0f);
+      F3 amber = f3(1.0f, 0.34f, 0.01f), white = f3(1.0f, 0.96f, 0.76f);
+      return lerp3(lerp3(cold, blue, fxm_smoothstep(0.0f, 0.30f, t)), lerp3(amber, white, fxm_smoothstep(0.62f, 1.0f, t)),
+                   fxm_smoothstep(0.24f, 0.72f, t));
+    }
+    static float aces_curve(float v) {
+      return fxm_clamp((v * (2.51f * v + 0.03f)) / (v * (2.43f * v + 0.59f) + 0.14f), 0.0f, 1.0f);
+    }
+    F3 aces(F3 c) const {
+      float e = fxm_max(u[27], 0.25f);
+      F3 x = f3(fxm_max(c.x * e, 0.0f), fxm_max(c.y * e, 0.0f), fxm_max(c.z * e, 0.0f));
+      float peak = fxm_max(x.x, fxm_max(x.y, x.z));
+      float k = aces_curve(peak) / fxm_max(peak, 0.000001f);
+      const float g = (float)(1.0 / 2.2);
+      return f3(fxm_pow(fxm_clamp(x.x * k, 0.0f, 1.0f), g), fxm_pow(fxm_clamp(x.y * k, 0.0f, 1.0f), g),
+                fxm_pow(fxm_clamp(x.z * k, 0.0f, 1.0f), g));
+    }
+
+    // diagnosticSample, qualityShaders.ts:338-380
+    V4<float> diagnostic(uint32_t mode, V4<float> fl, V4<float> ch, V4<float> me) const {
+      auto out = [](F3 c, float a) { return V4<float>{c.x, c.y, c.z, a}; };
+      if (mode == 1u) return out(thermal_map(fl.w), fxm_smoothstep(0.015f, 0.72f, fl.w));
+      if (mode == 2u) {
+        float st = fxm_smoothstep(0.003f, 0.38f, me.x);
+        return out(lerp3(f3(0.45f, 0.01f, 0.0f), f3(1.0f, 0.95f, 0.52f), st), st);
+      }
+      if (mode == 3u) {
+        float fuel = fxm_clamp(ch.x, 0.0f, 1.0f);
+        float ao = fxm_max(u[9], 0.001f);
+        float deficit = fxm_clamp((ao - ch.y) / ao, 0.0f, 1.0f);
+        float overlap = fxm_min(fuel, fxm_clamp(ch.y / ao, 0.0f, 1.0f));
+        return out(f3(fuel, overlap * 0.36f, deficit), fxm_max(fuel, fxm_max(deficit * 0.72f, overlap * 0.32f)));
+      }
+      if (mode == 4u) {
+        float st = fxm_smoothstep(0.005f, 0.62f, ch.z);
+        return out(lerp3(f3(0.06f, 0.02f, 0.12f), f3(0.74f, 0.26f, 1.0f), st), st);
+      }
+      if (mode == 5u) {
+        float vapor = fxm_smoothstep(0.004f, 0.72f, ch.w), soot = fxm_smoothstep(0.003f, 0.55f, me.y);
+        return out(lerp3(f3(0.56f, 0.72f, 0.82f), f3(0.17f, 0.12f, 0.10f), soot), fxm_max(vapor, soot));
+      }
+      if (mode == 6u) {
+        float liquid = fxm_smoothstep(0.002f, 0.32f, me.z);
+        float speed = fxm_smoothstep(0.035f, 1.25f, len(f3(fl.x, fl.y, fl.z)));
+        F3 dirn = plus(f3(0.5f, 0.5f, 0.5f), scale(unit(plus(f3(fl.x, fl.y, fl.z), f3(0.0001f, 0.0001f, 0.0001f))), 0.5f));
+        return out(lerp3(times(dirn, f3(0.25f, 0.31f, 0.36f)), f3(0.78f, 0.91f, 1.0f), liquid),
+                   fxm_max(liquid, speed * 0.038f));
+      }
+      float dv = me.w;
+      return out(dv >= 0.0f ? f3(1.0f, 0.18f, 0.045f) : f3(0.12f, 0.42f, 1.0f), fxm_smoothstep(0.01f, 1.2f, std::fabs(dv)));
+    }
+
+    F3 background(float uvy, F3 eye, F3 dir) const {
+      F3 c = lerp3(f3(0.020f, 0.024f, 0.030f), f3(0.0025f, 0.004f, 0.007f), fxm_clamp(uvy, 0.0f, 1.0f));
+      float fd = -eye.y / fxm_min(dir.y, -0.00001f);
+      if (dir.y < -0.00001f && fd > 0.0f) {
+        F3 fp = plus(eye, scale(dir, fd));
+        float dx = fp.x - ext(0) * 0.5f, dz = fp.z - ext(2) * 0.5f;
+        float rings = fxm_smoothstep(0.012f, 0.0f, std::fabs(fxm_sqrt(dx * dx + dz * dz) - 0.10f));
+        float grid = fxm_pow(fxm_max(std::fabs(fxm_sin(fp.x * 31.4159f)), std::fabs(fxm_sin(fp.z * 31.4159f))), 28.0f);
+        c = plus(lerp3(f3(0.018f, 0.020f, 0.022f), f3(0.045f, 0.048f, 0.050f), grid * 0.22f),
+                 scale(f3(0.055f, 0.020f, 0.006f), rings));
+      }
+      return c;
+    }
+
+    // volumeFragment, one pixel; returns the f32 colour before the f16 store.
+    F3 pixel(int px, int py) const {
+      const float uvx = ((float)px + 0.5f) / (float)width, uvy = 1.0f - ((float)py + 0.5f) / (float)height;
+      const float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;
+      const F3 E = f3(ext(0), ext(1), ext(2));
+      const F3 target = f3(E.x * 0.5f, E.y * 0.40f, E.z * 0.5f);
+      const float yaw = u[16], pitch = u[17];
+      const F3 orbit = f3(fxm_sin(yaw) * fxm_cos(pitch), fxm_sin(pitch), fxm_cos(yaw) * fxm_cos(pitch));
+      const float frame = fxm_max(E.x, fxm_max(E.y, E.z));
+      const F3 eye = plus(target, scale(scale(orbit, 0.90f + u[18] * 0.18f), frame));
+      const F3 fwd = unit(minus(target, eye));
+      const F3 right = unit(cross(fwd, f3(0.0f, 1.0f, 0.0f)));
+      const F3 up = unit(cross(right, fwd));
+      const float lens = 0.72f / fxm_clamp(u[39], 0.5f, 3.0f);
+      const F3 dir = unit(plus(plus(fwd, scale(scale(scale(right, sx), u[19]), lens)), scale(scale(up, sy), lens)));
+      float lo[3], hi[3];
+      const float o[3] = {eye.x, eye.y, eye.z}, d[3] = {dir.x, dir.y, dir.z}, e3[3] = {E.x, E.y, E.z};
+      for (int a = 0; a < 3; ++a) {
+        float sd = std::fabs(d[a]) > 0.000001f ? d[a] : 0.000001f;
+        float inv = 1.0f / sd;
+        float l = -o[a] * inv, h = (e3[a] - o[a]) * inv;
+        lo[a] = fxm_min(l, h);
+        hi[a] = fxm_max(l, h);
+      }
+      const float tn = fxm_max(fxm_max(lo[0], lo[1]), lo[2]);
+      const float tf = fxm_min(fxm_min(hi[0], hi[1]), hi[2]);
+      const F3 bg = background(uvy, eye, dir);
+      const float cin = fxm_clamp(u[42], 0.0f, 1.0f);
+      if (tn > tf || tf <= 0.0f) return cin >= 0.5f ? bg : aces(bg);
+      const float start = fxm_max(tn, 0.0f);
+      const uint32_t count = (uint32_t)std::min(384, raySteps_of());
+      const float stepLen = (tf - start) / fxm_max((float)count, 1.0f);
+      const float hx = fxm_fract(fxm_floor((float)px + 0.5f) * 123.34f), hy = fxm_fract(fxm_floor((float)py + 0.5f) * 456.21f);
+      const float jitter = fxm_fract((hx + hy) * (hx + hy + 45.32f));
+      float dist = start + jitter * stepLen;
+      float trans = 1.0f;
+      F3 rad = f3(0.0f, 0.0f, 0.0f);
+      const uint32_t view = (uint32_t)fxm_clamp(fxm_rint(u[24]), 0.0f, 7.0f);
+      for (uint32_t k = 0; k < 384u; ++k) {
+        if (k >= count || dist >= tf || trans < 0.006f) break;
+        const F3 pos = plus(eye, scale(dir, dist));
+        const V4<float> rawMedia = field(vm, pos);
+        if (view == 0u && cin >= 0.5f) {
+          float occ = fxm_max(fxm_max(rawMedia.x, rawMedia.y), fxm_max(rawMedia.z, rawMedia.w));
+          if (occ < 0.0005f) {
+            float nd = dist + stepLen;
+            if (nd < tf) {
+              V4<float> m2 = field(vm, plus(eye, scale(dir, nd)));
+              float occ2 = fxm_max(fxm_max(m2.x, m2.y), fxm_max(m2.z, m2.w));
+              dist += occ2 < 0.0005f ? stepLen * 2.0f : stepLen;
+            } else {
+              dist = tf;
+            }
+            continue;
+          }
+        }
+        const V4<float> fl = field(vf, pos);
+        V4<float> ch = field(vc, pos);
+        ch = {fxm_max(ch.x, 0.0f), fxm_max(ch.y, 0.0f), fxm_max(ch.z, 0.0f), fxm_max(ch.w, 0.0f)};
+        const V4<float> me{fxm_max(rawMedia.x, 0.0f), fxm_max(rawMedia.y, 0.0f), fxm_max(rawMedia.z, 0.0f), rawMedia.w};
+        if (view == 0u) {
+          const float temperature = 300.0f + fl.w * 2600.0f;
+          const float rr = fxm_max(me.x, 0.0f);
+          const float front = fxm_mix(fxm_smoothstep(0.006f, 0.32f, rr), fxm_smoothstep(0.004f, 0.28f, rr), cin);
+          const float thermal = fxm_smoothstep(620.0f, 1780.0f, temperature);
+          const F3 dp = minus(pos, scale(f3(fl.x, fl.y, fl.z), 0.08f));
+          const float da = fxm_sin(dot(dp, f3(27.0f, 35.0f, 13.0f)) + u[5] * 3.4f);
+          const float db = fxm_sin(dot(dp, f3(-21.0f, 29.0f, 17.0f)) - u[5] * 4.7f);
+          float optical = 0.90f + 0.20f * da * db;
+          if (cin > 0.5f && front > 0.002f) {
+            const F3 nrm = f3(dp.x / E.x, dp.y / E.y, dp.z / E.z);
+            const float req = u[43];
+            const float expl = fxm_clamp(req, 64.0f, 1024.0f);
+            const F3 cells = req >= 1.0f ? f3(expl, expl, expl) : f3(u[0], u[1], u[2]);
+            const F3 tile = f3(fxm_max(cells.x / 64.0f, 0.015625f), fxm_max(cells.y / 64.0f, 0.015625f),
+                               fxm_max(cells.z / 64.0f, 0.015625f));
+            const F3 toff = scale(f3(0.013f, -0.021f, 0.017f), u[5]);
+            const float n1 = detail(plus(times(nrm, tile), toff)).x - 0.5f;
+            const F3 c = minus(nrm, f3(0.5f, 0.5f, 0.5f));
+            const float ra = c.x * 0.771f + c.z * 0.637f, rb = -c.x * 0.637f + c.z * 0.771f;
+            const F3 rot = f3(ra, c.y * 0.909f - rb * 0.417f, c.y * 0.417f + rb * 0.909f);
+            const float n2 = detail(minus(plus(scale(times(rot, tile), 0.47f), f3(0.371f, 0.113f, 0.733f)),
+                                          scale(f3(toff.y, toff.z, toff.x), 0.61f))).y - 0.5f;
+            const float maxCells = fxm_max(cells.x, fxm_max(cells.y, cells.z));
+            const float budget = fxm_clamp(u[22], 1.0f, 384.0f);
+            const float band1 = fxm_clamp((budget * 2.0f) / maxCells, 0.25f, 1.0f);
+            const float band2 = fxm_clamp((budget * 2.0f) / fxm_max(maxCells * 0.47f, 1.0f), 0.35f, 1.0f);
+            const float fbm = n1 * (0.55f * band1) + n2 * (0.45f * band2);
+            const float mask = fxm_smoothstep(0.025f, 0.24f, front) * (1.0f - fxm_smoothstep(0.72f, 0.96f, front));
+            const float breakup = fxm_clamp(1.0f + fbm * 0.44f, 0.84f, 1.16f);
+            optical = fxm_mix(1.0f, breakup, mask * 0.85f);
+          }
+          const float fdef = fxm_clamp(u[41], 0.5f, 2.5f);
+          const float fexp = fxm_mix(0.85f + fdef * 0.35f, 0.92f + fdef * 0.16f, cin);
+          const float flame = fxm_pow(fxm_clamp(fxm_mix(front * thermal, front * (0.42f + 0.58f * thermal), cin), 0.0f, 1.0f),
+                                      fexp) * optical;
+          const float soot = fxm_clamp(1.0f - fxm_exp(-fxm_max(me.y, 0.0f) * fxm_mix(1.15f, 0.90f, cin)), 0.0f,
+                                       fxm_mix(1.0f, 0.96f, cin));
+          const float vapor = 1.0f - fxm_exp(-fxm_max(ch.w, 0.0f) * fxm_mix(0.34f, 0.30f, cin));
+          const float rich = fxm_clamp(ch.x / fxm_max(ch.x + ch.y * 0.25f, 0.0001f), 0.0f, 1.0f);
+          const float reveal = fxm_mix(1.0f, 1.0f - fxm_smoothstep(0.045f, 0.30f, front) * 0.78f, cin);
+          const float sf = flame * fxm_mix(3.20f, 0.88f, cin);
+          const float ss = soot * reveal * fxm_mix(1.6f, 1.62f, cin);
+          const float sv = vapor * fxm_mix(0.20f, 0.32f, cin);
+          const float extinction = fxm_min(sf + ss + sv, fxm_mix(100.0f, 5.5f, cin));
+          const float depth = fxm_min(extinction * stepLen * fxm_mix(5.2f, 3.15f, cin), fxm_mix(100.0f, 0.28f, cin));
+          const float alpha = 1.0f - fxm_exp(-depth);
+          const float hot = soot * thermal;
+          const float etemp = fxm_mix(temperature + front * 650.0f,
+                                      820.0f + 2150.0f * fxm_pow(fxm_clamp(front * 0.74f + thermal * 0.38f, 0.0f, 1.0f), 0.72f),
+                                      cin);
+          const float warmth = 1.0f - fxm_smoothstep(0.18f, 0.72f, front * thermal);
+          const F3 stdColor = lerp3(blackbody(etemp), f3(1.0f, 0.075f, 0.002f), warmth * 0.68f);
+          const F3 amber = lerp3(f3(1.0f, 0.055f, 0.002f), f3(1.0f, 0.43f, 0.018f), fxm_smoothstep(0.025f, 0.30f, flame));
+          const F3 cinColor = lerp3(amber, f3(1.0f, 0.91f, 0.60f),
+                                    fxm_smoothstep(0.52f, 0.92f, flame) * fxm_smoothstep(980.0f, 2050.0f, temperature));
+          const F3 color = lerp3(stdColor, cinColor, cin);
+          const float eCin = flame * (1.35f + 3.65f * front) * (0.92f + 0.32f * u[10]) + hot * 0.08f;
+          const float eStd = flame * (1.20f + 1.80f * front) * (0.90f + 0.30f * u[10]) + hot * 0.25f;
+          const F3 emissive = scale(color, fxm_mix(eStd, eCin, cin));
+          const float baseZone = 1.0f - fxm_smoothstep(0.16f, 0.48f, pos.y / E.y);
+          const F3 blue = scale(scale(scale(f3(0.018f, 0.12f, 0.74f), flame), 1.0f - rich),
+                                fxm_mix(0.48f, baseZone * 0.34f, cin));
+          const F3 toFire = minus(f3(E.x * 0.5f, 0.08f, E.z * 0.5f), pos);
+          const float falloff = 1.0f / (1.0f + dot(toFire, toFire) * 8.0f);
+          const F3 amb = f3(0.018f, 0.022f, 0.028f);
+          const F3 fire = scale(f3(1.0f, 0.24f, 0.035f), falloff);
+          const F3 sStd = plus(scale(f3(0.012f, 0.009f, 0.007f), ss), scale(f3(0.12f, 0.15f, 0.17f), sv));
+          const F3 sCin = plus(scale(plus(scale(amb, 0.12f), scale(fire, 0.055f)), ss),
+                               scale(plus(scale(amb, 0.55f), scale(fire, 0.05f)), sv));
+          const F3 scattered = lerp3(sStd, sCin, cin);
+          const float m = fxm_max(extinction, 0.0001f);
+          const F3 src = plus(plus(emissive, blue), scattered);
+          rad = plus(rad, scale(f3(src.x / m, src.y / m, src.z / m), trans * alpha));
+          trans *= 1.0f - alpha;
+        } else {
+          const V4<float> fv = diagnostic(view, fl, ch, me);
+          const float alpha = 1.0f - fxm_exp(-fv.w * stepLen * 8.5f);
+          rad = plus(rad, scale(scale(f3(fv.x, fv.y, fv.z), trans * alpha), 1.35f));
+          trans *= 1.0f - alpha;
+        }
+        dist += stepLen;
+      }
+      const F3 c = plus(rad, scale(bg, trans));
+      return cin >= 0.5f ? c : aces(c);
+    }
+    int raySteps_of() const {
+      float v = u[22];
+      return v > 0.0f ? (int)(uint32_t)v : 0;
+    }
+  };
+
   V4<T> store_species(const V4<T>& s) const;
   T texel(T v) const;  // an rgba16float texel component: f16 rounding (FP32 oracle)
 
@@ -1456,6 +1825,40 @@ int orc_pack_fields(void* p, double* out_flow, double* out_chem, double* out_med
     return 0;
   };
   return h->precision == 32 ? run(h->f) : run(h->d);
+}
+
+// The volume ray marcher over freshly packed volumes: out receives width *
+// height * 4 values (rgba, f16-rounded as the rgba16float target stores them),
+// row 0 at the top.  FP32 oracle only.
+int orc_render_volume(void* p, int width, int height, int ray_steps, double* out) {
+  Handle* h = static_cast<Handle*>(p);
+  if (h->precision != 32 || width < 1 || height < 1 || ray_steps < 1) return 1;
+  State<float>& s = h->f;
+  std::vector<V4<float>> pf, pc, pm;
+  s.pack_fields(pf, pc, pm);
+  s.canvasWidth = width;
+  s.canvasHeight = height;
+  s.raySteps = ray_steps;
+  s.write_uniforms(0.0);
+  static const std::vector<uint8_t> det = State<float>::detail_volume(64);
+  const State<float>::Renderer r{s, pf, pc, pm, det, s.u, width, height};
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int y = 0; y < height; ++y)
+    for (int x = 0; x < width; ++x) {
+      const V3<float> c = r.pixel(x, y);
+      double* o = out + ((size_t)y * width + x) * 4;
+      o[0] = round_to_f16(c.x);
+      o[1] = round_to_f16(c.y);
+      o[2] = round_to_f16(c.z);
+      o[3] = 1.0;
+    }
+  return 0;
+}
+
+// The 64^3 rgba8 detail volume (createPeriodicDetailVolume), 4 bytes per texel.
+void orc_detail_volume(uint8_t* out) {
+  const std::vector<uint8_t> d = State<float>::detail_volume(64);
+  std::memcpy(out, d.data(), d.size());
 }
 
 // Threads of the OpenMP loops (the CPU baseline times 1 thread and all cores).

@@ -83,6 +83,8 @@ def load() -> ctypes.CDLL:
         "orc_kernel_droplets": (i, [vp, d, i]),
         "orc_write_particle_initial_state": (None, [vp, i]),
         "orc_write_particle_buffers": (None, [vp, dp, dp]),
+        "orc_render_volume": (i, [vp, i, i, i, dp]),
+        "orc_detail_volume": (None, [ctypes.POINTER(ctypes.c_uint8)]),
         "orc_set_threads": (None, [i]),
         "orc_max_threads": (i, []),
     }
@@ -92,6 +94,14 @@ def load() -> ctypes.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def detail_volume() -> np.ndarray:
+    """The 64^3 rgba8 periodic detail volume (createPeriodicDetailVolume),
+    shape (64, 64, 64, 4) indexed [z, y, x, channel]."""
+    out = np.empty((64, 64, 64, 4), dtype=np.uint8)
+    load().orc_detail_volume(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out
 
 
 def max_threads() -> int:
@@ -193,6 +203,16 @@ class Oracle:
         current particle buffers with the uniforms of a dt step."""
         if self.lib.orc_kernel_droplets(self.h, float(dt), int(which)):
             raise ValueError(f"unknown droplet pass {which}")
+
+    def render_volume(self, width: int, height: int, ray_steps: int) -> np.ndarray:
+        """The volume ray marcher (qualityShaders.ts:374-575) over freshly
+        packed volumes: (height, width, 4) rgba, f16-rounded, row 0 at the
+        top.  FP32 oracle only."""
+        out = np.empty((height, width, 4), dtype=np.float64)
+        if self.lib.orc_render_volume(self.h, int(width), int(height), int(ray_steps),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
+            raise ValueError("render_volume needs the FP32 oracle and a positive size and step count")
+        return out
 
     def write_particle_buffers(self, current, previous) -> None:
         """The live particle buffer and the other one (the previous state

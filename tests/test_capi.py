@@ -106,7 +106,8 @@ def _struct_pairs():
         (fs.MemoryEstimate, "firex_memory_estimate"), (fs.DeviceLimits, "firex_device_limits"),
         (fs.PassTiming, "firex_pass_timing"), (sv.ServerConfig, "firex_server_config"),
         (sv.Request, "firex_request"), (sv.RequestStatus, "firex_request_status"),
-        (sv.ServerStats, "firex_server_stats"),
+        (sv.ServerStats, "firex_server_stats"), (fs.FieldLayout, "firex_field_layout"),
+        (fs.RenderConfig, "firex_render_config"),
     ]
 
 
