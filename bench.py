@@ -146,6 +146,32 @@ def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict) -> dict:
             "deposited_liquid_mass": round(liquid, 3)}
 
 
+RENDER_4K = (3840, 2160, 384)  # the reference's 4K offline output and the uhd tier's ray budget
+REFERENCE_508_FRAMES_PER_S = 0.21  # README.md:64-67: one 508^3 step + 4K presentation per frame
+
+
+def render_profile(solver, frames: int = 3) -> dict:
+    """The volume ray marcher at the 4K offline size (pack + march), in the
+    cinematic beauty view (empty-space skipping on) and the temperature view
+    (every sample shaded): device time per frame from HIP events."""
+    w, h, steps = RENDER_4K
+    out = {"width": w, "height": h, "ray_steps": steps}
+    for view, name in ((0, "beauty"), (1, "temperature")):
+        solver.set_parameter("viewMode", view)
+        solver.render_volume(w, h, steps)  # warm: allocations, detail volume
+        solver.set_pass_timing(True)
+        for _ in range(frames):
+            solver.render_volume(w, h, steps)
+        t = solver.pass_timing()
+        solver.set_pass_timing(False)
+        ms = t["render"]["ms"] / frames
+        out[name] = {"render_ms": round(ms, 3), "pack_ms": round(t["pack"]["ms"] / frames, 3),
+                     "mpixel_per_s": round(w * h / (ms * 1e-3) / 1e6, 1),
+                     "ray_budget_samples_per_s": w * h * steps / (ms * 1e-3)}
+    solver.set_parameter("viewMode", 0)
+    return out
+
+
 def audit_profile(solver, reps: int = 20) -> dict:
     """The capture audit: device checksums of every solver field (wall time
     per call, launch + 8-byte readback + sync included) and of the flow field
@@ -299,6 +325,8 @@ def main() -> int:
                     help="steps of the 508^3 plume for its per-pass HBM GB/s (0 = skip)")
     ap.add_argument("--droplet-steps", type=int, default=60,
                     help="steps of the same workload with the hose on (SPH droplets; 0 = skip)")
+    ap.add_argument("--no-render", dest="render", action="store_false",
+                    help="skip the 4K volume-render measurements")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16, help="serving-stream sample size (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrivals per second")
@@ -352,6 +380,7 @@ def main() -> int:
     passes = pass_profile(solver, args.profile_steps, cells, wl["storage"])
     passes["pack_fields"] = pack_profile(solver, cells, wl["storage"])
     audit = audit_profile(solver)
+    render = render_profile(solver) if args.render else None
     t_fuel = solver.reduce("sum_fuel")
     solver.close()
 
@@ -388,13 +417,31 @@ def main() -> int:
         lel = time_steps(ls, args.large_steps, 2)
         lpasses = pass_profile(ls, 2, lcells, lw["storage"], f16_pressure=linfo["f16_pressure"])
         lpasses.pop("_raw")
+        frame = None
+        if args.render:
+            # the reference's published 508^3 figure: one step + a 4K presentation per output frame
+            w, h, rsteps = RENDER_4K
+            ls.render_volume(w, h, rsteps)
+            ls.sync()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                ls.step(DT, 1)
+                ls.render_volume(w, h, rsteps)
+            ls.sync()
+            fs = 3 / (time.perf_counter() - t0)
+            frame = {"frames_per_s": fs, "ms_per_frame": 1e3 / fs, "output": f"{w}x{h}", "ray_steps": rsteps,
+                     "reference_published_frames_per_s": REFERENCE_508_FRAMES_PER_S,
+                     "over_reference_published": fs / REFERENCE_508_FRAMES_PER_S,
+                     "note": "presentation here = packFields + the volume ray march; the reference's also draws "
+                             "droplet sprites and bloom/composite; its hardware is unspecified"}
         ls.close()
         large = {"grid": list(lw["dims"]), "cells": lcells, "steps": args.large_steps, "jacobi": list(lw["iters"]),
                  "species_storage": lw["storage"], "pressure": "f16" if linfo["f16_pressure"] else "f32",
                  "velocity_noise": lw["velocity_noise"], "ms_per_step": lel / args.large_steps * 1e3,
                  "cell_updates_per_s": lcells * args.large_steps / lel, "jacobi_plan": linfo["jacobi"],
                  "passes": {k: dict(v, frac_of_peak=round(v["algorithmic_GBps"] / PEAK_HBM_GBS, 4))
-                            if isinstance(v, dict) else v for k, v in lpasses.items()}}
+                            if isinstance(v, dict) else v for k, v in lpasses.items()},
+                 "step_plus_4k_frame": frame}
 
     droplets = None
     if args.droplet_steps > 0:
@@ -439,6 +486,7 @@ def main() -> int:
         "plume_508": large,
         "droplets": droplets,
         "capture_audit": audit,
+        "render_4k": render,
         "check": {"sum_fuel_after_run": t_fuel},
     }
     print(json.dumps(line), flush=True)
