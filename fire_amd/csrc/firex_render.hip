@@ -3,7 +3,10 @@
 // encodeRender, FireXEngine.ts:2607-2640) as a gfx950 compute kernel: one
 // thread per pixel, 16 x 16 pixels per block (four 8 x 8 waves, so the rays of
 // a wave stay close and share cached texels), writing the rgba16float HDR
-// scene image the reference renders into.
+// scene image the reference renders into.  The kernel is VALU-bound (texel
+// conversion and interpolation): one texel site serves the three volumes at a
+// sample position, and each diagnostic view has its own instantiation that
+// computes only the components it reads.
 //
 // The reference samples its three rgba16float render volumes and the 64^3
 // rgba8unorm detail volume with the GPU's linear filter, whose weight
@@ -48,39 +51,59 @@ __device__ inline float4 mix4(float4 a, float4 b, float t) {
   return make_float4(fxm_mix(a.x, b.x, t), fxm_mix(a.y, b.y, t), fxm_mix(a.z, b.z, t), fxm_mix(a.w, b.w, t));
 }
 
-struct Half4 {
+// A texel of the rgba16float volumes; every texel starts at a multiple of 8
+// bytes, so one 8-byte load fetches it (2-byte alignment would let the
+// compiler split texel loads into pieces).
+struct alignas(8) Half4 {
   _Float16 x, y, z, w;
 };
 
-__device__ inline float4 texel(const Half4* __restrict__ vol, int x, int y, int z, const FxRender& r) {
-  const Half4 h = vol[(uint32_t)x + (uint32_t)r.nx * ((uint32_t)y + (uint32_t)r.ny * (uint32_t)z)];
+__device__ inline float4 texel(const Half4* __restrict__ vol, uint32_t i) {
+  const Half4 h = vol[i];
   return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
 }
 
-// textureSampleLevel(volume, volumeSampler (linear, clamp-to-edge), uvw, 0)
-__device__ float4 sample_volume(const Half4* __restrict__ vol, V3 uvw, const FxRender& r) {
-  const float tx = uvw.x * r.gx - 0.5f, ty = uvw.y * r.gy - 0.5f, tz = uvw.z * r.gz - 0.5f;
-  const float flx = fxm_floor(tx), fly = fxm_floor(ty), flz = fxm_floor(tz);
-  const float fx = tx - flx, fy = ty - fly, fz = tz - flz;
-  const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-  const int x0 = min(max(ix, 0), r.nx - 1), x1 = min(max(ix + 1, 0), r.nx - 1);
-  const int y0 = min(max(iy, 0), r.ny - 1), y1 = min(max(iy + 1, 0), r.ny - 1);
-  const int z0 = min(max(iz, 0), r.nz - 1), z1 = min(max(iz + 1, 0), r.nz - 1);
-  const float4 c00 = mix4(texel(vol, x0, y0, z0, r), texel(vol, x1, y0, z0, r), fx);
-  const float4 c10 = mix4(texel(vol, x0, y1, z0, r), texel(vol, x1, y1, z0, r), fx);
-  const float4 c01 = mix4(texel(vol, x0, y0, z1, r), texel(vol, x1, y0, z1, r), fx);
-  const float4 c11 = mix4(texel(vol, x0, y1, z1, r), texel(vol, x1, y1, z1, r), fx);
-  return mix4(mix4(c00, c10, fy), mix4(c01, c11, fy), fz);
-}
+// Where a sample of the three render volumes falls: the texel of the lower
+// corner, the steps to the upper neighbours along x, y, z (0 at a clamped
+// edge) and the weights.  The flow, chemistry and media samples at one
+// position share it (sampleFlow / sampleChemistry / sampleMedia,
+// qualityShaders.ts:262-278: the position clamped half a texel inside the
+// volume, then textureSampleLevel with the linear clamp-to-edge sampler).
+struct Site {
+  uint32_t i000, dx, dy, dz;
+  float fx, fy, fz;
+};
 
-// sampleFlow / sampleChemistry / sampleMedia: the position clamped half a
-// texel inside the volume.
-__device__ inline float4 sample_field(const Half4* __restrict__ vol, V3 position, const FxRender& r) {
+__device__ inline Site sample_site(V3 position, const FxRender& r) {
   const V3 half_texel = v3(0.5f / r.gx, 0.5f / r.gy, 0.5f / r.gz);
   const V3 q = divv(position, v3(r.ex, r.ey, r.ez));
   const V3 uvw = v3(fxm_clamp(q.x, half_texel.x, 1.0f - half_texel.x), fxm_clamp(q.y, half_texel.y, 1.0f - half_texel.y),
                     fxm_clamp(q.z, half_texel.z, 1.0f - half_texel.z));
-  return sample_volume(vol, uvw, r);
+  const float tx = uvw.x * r.gx - 0.5f, ty = uvw.y * r.gy - 0.5f, tz = uvw.z * r.gz - 0.5f;
+  const float flx = fxm_floor(tx), fly = fxm_floor(ty), flz = fxm_floor(tz);
+  const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+  const int x0 = min(max(ix, 0), r.nx - 1), x1 = min(max(ix + 1, 0), r.nx - 1);
+  const int y0 = min(max(iy, 0), r.ny - 1), y1 = min(max(iy + 1, 0), r.ny - 1);
+  const int z0 = min(max(iz, 0), r.nz - 1), z1 = min(max(iz + 1, 0), r.nz - 1);
+  Site s;
+  s.i000 = (uint32_t)x0 + (uint32_t)r.nx * ((uint32_t)y0 + (uint32_t)r.ny * (uint32_t)z0);
+  s.dx = (uint32_t)(x1 - x0);
+  s.dy = y1 != y0 ? (uint32_t)r.nx : 0u;
+  s.dz = z1 != z0 ? (uint32_t)r.nx * (uint32_t)r.ny : 0u;
+  s.fx = tx - flx;
+  s.fy = ty - fly;
+  s.fz = tz - flz;
+  return s;
+}
+
+// The eight texels mixed along x, then y, then z.
+__device__ inline float4 sample_volume(const Half4* __restrict__ vol, const Site& s) {
+  const uint32_t i = s.i000;
+  const float4 c00 = mix4(texel(vol, i), texel(vol, i + s.dx), s.fx);
+  const float4 c10 = mix4(texel(vol, i + s.dy), texel(vol, i + s.dy + s.dx), s.fx);
+  const float4 c01 = mix4(texel(vol, i + s.dz), texel(vol, i + s.dz + s.dx), s.fx);
+  const float4 c11 = mix4(texel(vol, i + s.dz + s.dy), texel(vol, i + s.dz + s.dy + s.dx), s.fx);
+  return mix4(mix4(c00, c10, s.fy), mix4(c01, c11, s.fy), s.fz);
 }
 
 __device__ inline float4 detail_texel(const uint32_t* __restrict__ detail, int x, int y, int z) {
@@ -213,7 +236,14 @@ __device__ inline V3 rotate_detail(V3 p) {
   return v3(a, p.y * 0.909f - b * 0.417f, p.y * 0.417f + b * 0.909f);
 }
 
-// volumeFragment, qualityShaders.ts:407-575
+// volumeFragment, qualityShaders.ts:407-575.  VIEW 1-7 instantiates one
+// diagnostic view mode (u32(clamp(round(controls.x), 0, 7))): the compiler
+// then drops the texel conversions and interpolations of components that view
+// never reads (about 3x faster).  kRuntimeView reads the mode at run time;
+// the beauty view uses it, as a beauty-only instantiation measured slower.
+constexpr uint32_t kRuntimeView = 8;
+
+template <uint32_t VIEW>
 __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__ flow_vol,
                                                        const Half4* __restrict__ chem_vol,
                                                        const Half4* __restrict__ media_vol,
@@ -260,19 +290,21 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
     float distance = start + jitter * step_length;
     float transmittance = 1.0f;
     V3 radiance = v3(0.0f, 0.0f, 0.0f);
-    const uint32_t view_mode = (uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f);
+    const uint32_t view_mode =
+        VIEW == kRuntimeView ? (uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f) : VIEW;
     const bool beauty = view_mode == 0u;
     const float front_def = fxm_clamp(r.front_definition, 0.5f, 2.5f);
     for (uint32_t step = 0; step < 384u; ++step) {
       if (step >= step_count || distance >= far_t || transmittance < 0.006f) break;
       const V3 position = add(eye, mul(dir, distance));
-      const float4 raw_media = sample_field(media_vol, position, r);
+      const Site site = sample_site(position, r);
+      const float4 raw_media = sample_volume(media_vol, site);
       if (beauty && cinematic >= 0.5f) {
         const float occupancy = fxm_max(fxm_max(raw_media.x, raw_media.y), fxm_max(raw_media.z, raw_media.w));
         if (occupancy < 0.0005f) {
           const float next_distance = distance + step_length;
           if (next_distance < far_t) {
-            const float4 nm = sample_field(media_vol, add(eye, mul(dir, next_distance)), r);
+            const float4 nm = sample_volume(media_vol, sample_site(add(eye, mul(dir, next_distance)), r));
             const float next_occupancy = fxm_max(fxm_max(nm.x, nm.y), fxm_max(nm.z, nm.w));
             distance += next_occupancy < 0.0005f ? step_length * 2.0f : step_length;
           } else {
@@ -281,8 +313,8 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
           continue;
         }
       }
-      const float4 flow = sample_field(flow_vol, position, r);
-      float4 chem = sample_field(chem_vol, position, r);
+      const float4 flow = sample_volume(flow_vol, site);
+      float4 chem = sample_volume(chem_vol, site);
       chem = make_float4(fxm_max(chem.x, 0.0f), fxm_max(chem.y, 0.0f), fxm_max(chem.z, 0.0f), fxm_max(chem.w, 0.0f));
       const float4 media = make_float4(fxm_max(raw_media.x, 0.0f), fxm_max(raw_media.y, 0.0f),
                                        fxm_max(raw_media.z, 0.0f), raw_media.w);
@@ -400,7 +432,19 @@ void launch_render_volume(const void* packed, const uint32_t* detail, void* out,
   const Half4* vol = (const Half4*)packed;
   const uint64_t cells = (uint64_t)r.nx * r.ny * r.nz;
   dim3 grid((uint32_t)((r.width + 15) / 16), (uint32_t)((r.height + 15) / 16));
-  k_render_volume<<<grid, 256, 0, s>>>(vol, vol + cells, vol + 2 * cells, detail, (Half4*)out, r);
+  const Half4 *f = vol, *c = vol + cells, *m = vol + 2 * cells;
+  Half4* o = (Half4*)out;
+  // u32(clamp(round(controls.x), 0.0, 7.0)), qualityShaders.ts:436
+  switch ((uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f)) {
+    case 0: k_render_volume<kRuntimeView><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 1: k_render_volume<1><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 2: k_render_volume<2><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 3: k_render_volume<3><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 4: k_render_volume<4><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 5: k_render_volume<5><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 6: k_render_volume<6><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    default: k_render_volume<7><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+  }
 }
 
 }  // namespace fx
