@@ -153,7 +153,8 @@ REFERENCE_508_FRAMES_PER_S = 0.21  # README.md:64-67: one 508^3 step + 4K presen
 def render_profile(solver, frames: int = 3) -> dict:
     """The volume ray marcher at the 4K offline size (pack + march), in the
     cinematic beauty view (empty-space skipping on) and the temperature view
-    (every sample shaded): device time per frame from HIP events."""
+    (every sample shaded), then the whole presented beauty frame (pack, march,
+    bloom and composite): device time per frame from HIP events."""
     w, h, steps = RENDER_4K
     out = {"width": w, "height": h, "ray_steps": steps}
     for view, name in ((0, "beauty"), (1, "temperature")):
@@ -169,6 +170,17 @@ def render_profile(solver, frames: int = 3) -> dict:
                      "mpixel_per_s": round(w * h / (ms * 1e-3) / 1e6, 1),
                      "ray_budget_samples_per_s": w * h * steps / (ms * 1e-3)}
     solver.set_parameter("viewMode", 0)
+    # the presented frame: ray march, bloom and composite (cinematic)
+    solver.render_frame(w, h, steps)  # warm: presentation buffers
+    solver.set_pass_timing(True)
+    for _ in range(frames):
+        solver.render_frame(w, h, steps, read=False)
+    t = solver.pass_timing()
+    solver.set_pass_timing(False)
+    ms = {k: t[k]["ms"] / frames for k in ("pack", "render", "present")}
+    out["presented_frame"] = {"pack_ms": round(ms["pack"], 3), "render_ms": round(ms["render"], 3),
+                              "present_ms": round(ms["present"], 3), "device_ms": round(sum(ms.values()), 3),
+                              "frames_per_s_device": round(1e3 / sum(ms.values()), 1)}
     return out
 
 
@@ -421,19 +433,28 @@ def main() -> int:
         if args.render:
             # the reference's published 508^3 figure: one step + a 4K presentation per output frame
             w, h, rsteps = RENDER_4K
-            ls.render_volume(w, h, rsteps)
+            host_frame = ls.render_frame(w, h, rsteps)
             ls.sync()
             t0 = time.perf_counter()
             for _ in range(3):
                 ls.step(DT, 1)
-                ls.render_volume(w, h, rsteps)
+                ls.render_frame(w, h, rsteps, read=False)
             ls.sync()
             fs = 3 / (time.perf_counter() - t0)
+            # the same with each frame copied back to host memory (a recorder)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                ls.step(DT, 1)
+                ls.render_frame(w, h, rsteps, read=False)
+                ls.read_frame(host_frame)
+            fr = 3 / (time.perf_counter() - t0)
             frame = {"frames_per_s": fs, "ms_per_frame": 1e3 / fs, "output": f"{w}x{h}", "ray_steps": rsteps,
+                     "frames_per_s_with_readback": fr, "ms_per_frame_with_readback": 1e3 / fr,
                      "reference_published_frames_per_s": REFERENCE_508_FRAMES_PER_S,
                      "over_reference_published": fs / REFERENCE_508_FRAMES_PER_S,
-                     "note": "presentation here = packFields + the volume ray march; the reference's also draws "
-                             "droplet sprites and bloom/composite; its hardware is unspecified"}
+                     "note": "presentation here = packFields, the volume ray march, the quarter-size bloom and the "
+                             "filmic composite into an rgba8 frame; the reference's also draws droplet sprites "
+                             "(none in this dry plume); its hardware is unspecified"}
         ls.close()
         large = {"grid": list(lw["dims"]), "cells": lcells, "steps": args.large_steps, "jacobi": list(lw["iters"]),
                  "species_storage": lw["storage"], "pressure": "f16" if linfo["f16_pressure"] else "f32",

@@ -413,6 +413,8 @@ FxRender make_render(const float u[FXU_COUNT], const int32_t dims[3], int32_t wi
   r.chem_y = u[FXU_OXYGEN_RATE];
   r.chem_z = u[FXU_HEAT_EFFICIENCY];
   r.zoom = u[FXU_VIEW_ZOOM];
+  r.canvas_height = u[FXU_CANVAS_H];
+  r.nozzle = u[FXU_NOZZLE];
   return r;
 }
 

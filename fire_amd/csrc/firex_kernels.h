@@ -58,6 +58,20 @@ void launch_checksum(const void* data, uint64_t bytes, uint64_t* partial, uint64
 // cell, flow / chemistry / media), detail = 64^3 rgba8 texels, out = width x
 // height rgba16f pixels, row 0 at the top.
 void launch_render_volume(const void* packed, const uint32_t* detail, void* out, const FxRender& r, hipStream_t s);
+// The presentation after the ray march: with cinematic, the quarter-resolution
+// bloom (bloom: ceil(w/4) x ceil(h/4) rgba16f) and the tone-map composite;
+// without, the scene image stored as 8 bits.  frame: w x h rgba8 pixels.
+void launch_render_post(const void* scene, void* bloom, int width, int height, float exposure, int cinematic,
+                        uint32_t* frame, hipStream_t s);
+
+// Droplet streaks (dropletVertex / dropletFragment) over an image: particles
+// = the live droplet buffer (count records of 8 f32), target = width x height
+// rgba16f (the scene image) or, with unorm8, rgba8 (the standard canvas).
+// Nothing is drawn outside the beauty and liquid views.  work:
+// sprite_work_bytes(count) bytes.
+uint64_t sprite_work_bytes(int count);
+void launch_droplet_streaks(const void* particles, int count, const FxRender& r, void* target, int unorm8,
+                            void* work, hipStream_t s);
 
 constexpr uint32_t kReducePartials = 1024;
 

@@ -424,6 +424,389 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
   out[(uint32_t)py * (uint32_t)r.width + (uint32_t)px] = o;
 }
 
+// ---------------------------------------------------------- droplet streaks
+// dropletVertex / dropletFragment (qualityShaders.ts:156-232): each live
+// droplet draws a camera-facing quad, blended over the target in instance
+// order.  Here the draw is a tiled compute rasterizer in four launches:
+//   k_sprite_setup  one thread per droplet: the quad (FxSprite) and its
+//                   conservative pixel box; counts the 128 x 128 bins it meets
+//   k_sprite_scan   bin offsets (exclusive scan of the counts)
+//   k_sprite_fill   one block per bin: the ids of the droplets meeting it, in
+//                   instance order (a bin whose list would overrun the
+//                   capacity is left empty and read as "every droplet")
+//   k_sprite_draw   one block per 16 x 16 tile: walks its bin's list in order,
+//                   keeps the droplets meeting the tile (ordered compaction in
+//                   LDS) and blends each covered pixel in registers.
+// Rasterization and blending are defined in f32 arithmetic: a pixel is
+// covered when its centre p (the fullscreen pass's NDC of the pixel) has
+// quad coordinates local = (dot(p - c, normal) / radius, dot(p - c, tangent)
+// / half) strictly inside (-1, 1)^2 (local is the fragment's interpolated
+// `local`); the blend (src-alpha, one-minus-src-alpha; alpha: one,
+// one-minus-src-alpha) is src * a + dst * (1 - a) in f32, each result stored
+// in the target's format (f16 nearest-even, or unorm8 rint(clamp * 255) with
+// dst read as byte / 255).
+
+struct Camera {
+  V3 eye, fwd, right, up;
+  float lens;
+};
+
+// cameraEye / projectWorld's basis (qualityShaders.ts:133-176): the same f32
+// operations as the ray marcher's camera.
+__device__ Camera make_camera(const FxRender& r) {
+  const V3 target = v3(r.ex * 0.5f, r.ey * 0.40f, r.ez * 0.5f);
+  const V3 orbit = v3(fxm_sin(r.yaw) * fxm_cos(r.pitch), fxm_sin(r.pitch), fxm_cos(r.yaw) * fxm_cos(r.pitch));
+  const float frame_scale = fxm_max(r.ex, fxm_max(r.ey, r.ez));
+  Camera c;
+  c.eye = add(target, mul(mul(orbit, 0.90f + r.cam_distance * 0.18f), frame_scale));
+  c.fwd = normalize3(sub(target, c.eye));
+  c.right = normalize3(cross3(c.fwd, v3(0.0f, 1.0f, 0.0f)));
+  c.up = normalize3(cross3(c.right, c.fwd));
+  c.lens = 0.72f / fxm_clamp(r.zoom, 0.5f, 3.0f);
+  return c;
+}
+
+__device__ inline void project_world(const Camera& c, V3 point, float aspect, float& x, float& y) {
+  const V3 rel = sub(point, c.eye);
+  const float depth = fxm_max(dot3(rel, c.fwd), 0.0001f);
+  x = dot3(rel, c.right) / (depth * aspect * c.lens);
+  y = dot3(rel, c.up) / (depth * c.lens);
+}
+
+// A pixel box packed as (x0 | x1 << 16, y0 | y1 << 16); x0 > x1 when empty.
+__device__ inline bool box_meets(uint2 b, int x0, int y0, int x1, int y1) {
+  const int bx0 = (int)(b.x & 0xffffu), bx1 = (int)(b.x >> 16), by0 = (int)(b.y & 0xffffu), by1 = (int)(b.y >> 16);
+  return bx0 <= bx1 && bx0 <= x1 && bx1 >= x0 && by0 <= y1 && by1 >= y0;
+}
+
+__global__ __launch_bounds__(256) void k_sprite_setup(const float4* __restrict__ particles, int count, FxRender r,
+                                                      FxSprite* __restrict__ sprites, uint2* __restrict__ boxes,
+                                                      uint32_t* __restrict__ bin_counts, int bins_x) {
+  const int i = (int)(blockIdx.x * 256u + threadIdx.x);
+  if (i >= count) return;
+  const float4 pa = particles[2 * i], vm = particles[2 * i + 1];
+  const uint2 empty = make_uint2(1u, 0u);
+  if (!(vm.w > 0.0001f && pa.x >= 0.0f && pa.y >= 0.0f && pa.z >= 0.0f && pa.x <= 1.0f && pa.y <= 1.0f &&
+        pa.z <= 1.0f)) {
+    boxes[i] = empty;
+    return;
+  }
+  const Camera cam = make_camera(r);
+  const V3 world = mulv(v3(pa.x, pa.y, pa.z), v3(r.ex, r.ey, r.ez));
+  float hx, hy, tx, ty;
+  project_world(cam, world, r.aspect, hx, hy);
+  project_world(cam, sub(world, mul(v3(vm.x, vm.y, vm.z), 0.018f)), r.aspect, tx, ty);
+  const float sx = tx - hx, sy = ty - hy;
+  const float length = fxm_sqrt(sx * sx + sy * sy);
+  const float streak = fxm_min(length, 0.032f);
+  const bool along = streak > 0.00001f;
+  FxSprite q;
+  q.tx = along ? sx / length : 1.0f;
+  q.ty = along ? sy / length : 0.0f;
+  const float pixel_radius = r.nozzle >= 0.5f ? 1.15f : 1.65f;
+  q.radius = pixel_radius * 2.0f / fxm_max(r.canvas_height, 1.0f);
+  q.half = q.radius + streak * 0.5f;
+  q.cx = hx;
+  q.cy = hy;
+  q.mass = vm.w;
+  q.pad = 0.0f;
+  sprites[i] = q;
+  // conservative box: the quad's NDC half extents, one pixel of margin
+  const float ex = fabsf(q.ty) * q.radius + fabsf(q.tx) * q.half;
+  const float ey = fabsf(q.tx) * q.radius + fabsf(q.ty) * q.half;
+  const float w = (float)r.width, h = (float)r.height;
+  const float fx0 = floorf((q.cx - ex + 1.0f) * 0.5f * w - 0.5f) - 1.0f;
+  const float fx1 = ceilf((q.cx + ex + 1.0f) * 0.5f * w - 0.5f) + 1.0f;
+  const float fy0 = floorf((1.0f - (q.cy + ey)) * 0.5f * h - 0.5f) - 1.0f;
+  const float fy1 = ceilf((1.0f - (q.cy - ey)) * 0.5f * h - 0.5f) + 1.0f;
+  // clamp in f32 first (the quad may project far off screen); a box left of
+  // or below the image ends as x1 = -1, one right of or above it as x0 = w
+  const int x0 = (int)fminf(fmaxf(fx0, 0.0f), w), x1 = (int)fmaxf(fminf(fx1, w - 1.0f), -1.0f);
+  const int y0 = (int)fminf(fmaxf(fy0, 0.0f), h), y1 = (int)fmaxf(fminf(fy1, h - 1.0f), -1.0f);
+  if (!(fx0 <= fx1 && fy0 <= fy1) || x0 > x1 || y0 > y1) {
+    boxes[i] = empty;
+    return;
+  }
+  boxes[i] = make_uint2((uint32_t)x0 | ((uint32_t)x1 << 16), (uint32_t)y0 | ((uint32_t)y1 << 16));
+  for (int by = y0 / kFxSpriteBin; by <= y1 / kFxSpriteBin; ++by)
+    for (int bx = x0 / kFxSpriteBin; bx <= x1 / kFxSpriteBin; ++bx) atomicAdd(&bin_counts[by * bins_x + bx], 1u);
+}
+
+// Exclusive scan of the bin counts, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_sprite_scan(const uint32_t* __restrict__ counts,
+                                                      uint32_t* __restrict__ offsets, int bins) {
+  __shared__ uint32_t part[1024];
+  const int t = (int)threadIdx.x;
+  const int per = (bins + 1023) / 1024;
+  const int b0 = t * per, b1 = min(b0 + per, bins);
+  uint32_t sum = 0;
+  for (int b = b0; b < b1; ++b) sum += counts[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const uint32_t v = t >= d ? part[t - d] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (int b = b0; b < b1; ++b) {
+    offsets[b] = run;
+    run += counts[b];
+  }
+}
+
+// Ordered compaction over a 256-thread block: returns the slot of a hit among
+// the block's hits (in thread order) and the block's hit count.
+__device__ inline uint32_t block_compact(bool hit, uint32_t* wave_hits, uint32_t& total) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t m = __ballot(hit);
+  const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_hits[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t base = 0;
+  total = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t c = wave_hits[k];
+    base += k < wave ? c : 0u;
+    total += c;
+  }
+  return base + below;
+}
+
+__global__ __launch_bounds__(256) void k_sprite_fill(const uint2* __restrict__ boxes, int count,
+                                                     const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offsets, uint32_t* __restrict__ list,
+                                                     uint32_t capacity, int bins_x) {
+  __shared__ uint32_t wave_hits[4];
+  const int bin = (int)blockIdx.x;
+  const uint32_t n = counts[bin], off = offsets[bin];
+  if (n == 0 || (uint64_t)off + n > capacity) return;
+  const int x0 = (bin % bins_x) * kFxSpriteBin, y0 = (bin / bins_x) * kFxSpriteBin;
+  uint32_t run = 0;
+  for (int base = 0; base < count; base += 256) {
+    const int i = base + (int)threadIdx.x;
+    const bool hit = i < count && box_meets(boxes[i], x0, y0, x0 + kFxSpriteBin - 1, y0 + kFxSpriteBin - 1);
+    uint32_t total;
+    const uint32_t slot = block_compact(hit, wave_hits, total);
+    if (hit) list[off + run + slot] = (uint32_t)i;
+    run += total;
+    __syncthreads();
+  }
+}
+
+// dropletFragment at quad coordinates (lx, ly): false when discarded.
+__device__ inline bool droplet_fragment(float lx, float ly, float mass, float boost, bool linear, const FxRender& r,
+                                        V3& color, float& alpha) {
+  const float m = fxm_max(fabsf(ly) - 0.48f, 0.0f);
+  const float d = fxm_sqrt(lx * lx + m * m);
+  const float coverage = 1.0f - fxm_smoothstep(0.64f, 1.0f, d);
+  if (coverage <= 0.001f) return false;
+  const float highlight = fxm_pow(fxm_clamp(1.0f - fabsf(lx + 0.30f), 0.0f, 1.0f), 5.0f);
+  const V3 base = mix3(v3(0.27f, 0.34f, 0.38f), v3(0.76f, 0.86f, 0.91f), highlight * 0.62f);
+  alpha = coverage * fxm_clamp(0.13f + mass * 0.15f, 0.12f, 0.42f) * boost;
+  const V3 lin = mul(base, 0.72f);
+  color = linear ? lin : aces(lin, r);
+  return true;
+}
+
+__device__ inline float blend_f16(float src, float dst, float keep_a) {
+  return (float)(_Float16)fx_keep_f32(src + dst * keep_a);
+}
+
+__device__ inline float blend_unorm8(float src, float dst, float keep_a) {
+  return fxm_rint(fxm_clamp(src + dst * keep_a, 0.0f, 1.0f) * 255.0f);
+}
+
+template <bool UNORM8>
+__global__ __launch_bounds__(256) void k_sprite_draw(const FxSprite* __restrict__ sprites, const uint2* __restrict__ boxes,
+                                                     const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ list, uint32_t capacity, int count,
+                                                     int bins_x, void* __restrict__ target, FxRender r) {
+  __shared__ FxSprite tile_sprites[256];
+  __shared__ uint32_t wave_hits[4];
+  const int tx0 = (int)blockIdx.x * kFxSpriteTile, ty0 = (int)blockIdx.y * kFxSpriteTile;
+  const int bin = (ty0 / kFxSpriteBin) * bins_x + tx0 / kFxSpriteBin;
+  const uint32_t n = counts[bin];
+  if (n == 0) return;
+  const uint32_t off = offsets[bin];
+  const bool every = (uint64_t)off + n > capacity;  // the bin's list was not written
+  const int total = every ? count : (int)n;
+  const int px = tx0 + (int)(threadIdx.x & 15u), py = ty0 + (int)(threadIdx.x >> 4);
+  const bool on = px < r.width && py < r.height;
+  const int tx1 = tx0 + kFxSpriteTile - 1, ty1 = ty0 + kFxSpriteTile - 1;
+  // the pixel centre in NDC, as the ray marcher computes it
+  const float uvx = ((float)px + 0.5f) / (float)r.width;
+  const float uvy = 1.0f - ((float)py + 0.5f) / (float)r.height;
+  const float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;
+  const uint32_t view = (uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f);
+  const float boost = view == 6u ? 1.38f : 1.0f;
+  const bool linear = r.cinematic >= 0.5f;
+  const size_t pixel = (size_t)(uint32_t)py * (uint32_t)r.width + (uint32_t)px;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+  if (on) {
+    if (UNORM8) {
+      const uint32_t b = ((const uint32_t*)target)[pixel];
+      c0 = (float)(b & 255u);
+      c1 = (float)((b >> 8) & 255u);
+      c2 = (float)((b >> 16) & 255u);
+      c3 = (float)(b >> 24);
+    } else {
+      const Half4 h = ((const Half4*)target)[pixel];
+      c0 = (float)h.x;
+      c1 = (float)h.y;
+      c2 = (float)h.z;
+      c3 = (float)h.w;
+    }
+  }
+  bool dirty = false;
+  for (int base = 0; base < total; base += 256) {
+    const int k = base + (int)threadIdx.x;
+    uint32_t id = 0;
+    bool hit = false;
+    if (k < total) {
+      id = every ? (uint32_t)k : list[off + (uint32_t)k];
+      hit = box_meets(boxes[id], tx0, ty0, tx1, ty1);
+    }
+    uint32_t found;
+    const uint32_t slot = block_compact(hit, wave_hits, found);
+    if (hit) tile_sprites[slot] = sprites[id];
+    __syncthreads();
+    if (on) {
+      for (uint32_t j = 0; j < found; ++j) {
+        const FxSprite q = tile_sprites[j];
+        const float dx = sx - q.cx, dy = sy - q.cy;
+        const float lx = (dx * -q.ty + dy * q.tx) / q.radius, ly = (dx * q.tx + dy * q.ty) / q.half;
+        if (!(fabsf(lx) < 1.0f && fabsf(ly) < 1.0f)) continue;
+        V3 src;
+        float a;
+        if (!droplet_fragment(lx, ly, q.mass, boost, linear, r, src, a)) continue;
+        const float keep = 1.0f - a;
+        if (UNORM8) {
+          c0 = blend_unorm8(src.x * a, c0 / 255.0f, keep);
+          c1 = blend_unorm8(src.y * a, c1 / 255.0f, keep);
+          c2 = blend_unorm8(src.z * a, c2 / 255.0f, keep);
+          c3 = blend_unorm8(a, c3 / 255.0f, keep);
+        } else {
+          c0 = blend_f16(src.x * a, c0, keep);
+          c1 = blend_f16(src.y * a, c1, keep);
+          c2 = blend_f16(src.z * a, c2, keep);
+          c3 = blend_f16(a, c3, keep);
+        }
+        dirty = true;
+      }
+    }
+    __syncthreads();
+  }
+  if (!dirty) return;
+  if (UNORM8) {
+    ((uint32_t*)target)[pixel] = (uint32_t)c0 | ((uint32_t)c1 << 8) | ((uint32_t)c2 << 16) | ((uint32_t)c3 << 24);
+  } else {
+    Half4 o;
+    o.x = (_Float16)c0;
+    o.y = (_Float16)c1;
+    o.z = (_Float16)c2;
+    o.w = (_Float16)c3;
+    ((Half4*)target)[pixel] = o;
+  }
+}
+
+// ------------------------------------------------------------- post passes
+// FIREX_POST_WGSL (qualityShaders.ts:580-671): the quarter-resolution bloom
+// and the tone-map / composite of the cinematic presentation, and the 8-bit
+// store of the canvas.  A post pass pixel (x, y) of a w x h target reads uv =
+// ((x + 0.5) / w, (y + 0.5) / h) (the pass flips y once, so v = 0 is the
+// scene image's top row); 2D textures are sampled like the volumes: t = uv *
+// size - 0.5, the four texels clamped to the edge, mixed along x, then y.
+
+__device__ inline float4 image_texel(const Half4* __restrict__ img, int x, int y, int w) {
+  const Half4 h = img[(uint32_t)y * (uint32_t)w + (uint32_t)x];
+  return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+}
+
+__device__ float4 sample_image(const Half4* __restrict__ img, int w, int h, float u, float v) {
+  const float tx = u * (float)w - 0.5f, ty = v * (float)h - 0.5f;
+  const float flx = fxm_floor(tx), fly = fxm_floor(ty);
+  const float fx = tx - flx, fy = ty - fly;
+  const int ix = (int)flx, iy = (int)fly;
+  const int x0 = min(max(ix, 0), w - 1), x1 = min(max(ix + 1, 0), w - 1);
+  const int y0 = min(max(iy, 0), h - 1), y1 = min(max(iy + 1, 0), h - 1);
+  return mix4(mix4(image_texel(img, x0, y0, w), image_texel(img, x1, y0, w), fx),
+              mix4(image_texel(img, x0, y1, w), image_texel(img, x1, y1, w), fx), fy);
+}
+
+__device__ inline V3 rgb(float4 c) { return v3(c.x, c.y, c.z); }
+
+// brightExtract, qualityShaders.ts:621-628
+__device__ V3 bright_extract(V3 c) {
+  const float luminance = dot3(c, v3(0.2126f, 0.7152f, 0.0722f));
+  const float threshold = 1.15f, knee = 0.35f;
+  const float soft = fxm_clamp((luminance - threshold + knee) / (2.0f * knee), 0.0f, 1.0f);
+  const float contribution = fxm_max(luminance - threshold, 0.0f) + soft * soft * knee;
+  const float d = fxm_max(luminance, 0.0001f);
+  return v3(c.x * contribution / d, c.y * contribution / d, c.z * contribution / d);
+}
+
+// bloomFragment, qualityShaders.ts:630-644: nine taps of the scene image.
+__global__ __launch_bounds__(256) void k_bloom(const Half4* __restrict__ scene, int w, int h,
+                                               Half4* __restrict__ bloom, int bw, int bh) {
+  const int x = (int)(blockIdx.x * 16 + (threadIdx.x & 15u)), y = (int)(blockIdx.y * 16 + (threadIdx.x >> 4));
+  if (x >= bw || y >= bh) return;
+  const float u = ((float)x + 0.5f) / (float)bw, v = ((float)y + 0.5f) / (float)bh;
+  const float rx = 1.0f / fxm_max((float)w, 1.0f) * 5.5f, ry = 1.0f / fxm_max((float)h, 1.0f) * 5.5f;
+  V3 c = mul(rgb(sample_image(scene, w, h, u, v)), 0.20f);
+  c = add(c, mul(rgb(sample_image(scene, w, h, u + rx, v + 0.0f)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u - rx, v - 0.0f)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u + 0.0f, v + ry)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u - 0.0f, v - ry)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u + rx, v + ry)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u - rx, v - ry)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u + rx, v + -ry)), 0.10f));
+  c = add(c, mul(rgb(sample_image(scene, w, h, u + -rx, v + ry)), 0.10f));
+  const V3 b = bright_extract(c);
+  Half4 o;
+  o.x = (_Float16)fx_keep_f32(b.x);
+  o.y = (_Float16)fx_keep_f32(b.y);
+  o.z = (_Float16)fx_keep_f32(b.z);
+  o.w = (_Float16)1.0f;
+  bloom[(uint32_t)y * (uint32_t)bw + (uint32_t)x] = o;
+}
+
+// filmic, qualityShaders.ts:654-662
+__device__ inline float filmic1(float c, float e) {
+  const float x = fxm_max(c * e, 0.0f);
+  return fxm_pow(fxm_clamp((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f), 0.0f, 1.0f), kFxInverseGamma);
+}
+
+// The 8-bit store of a colour target: round(clamp(c, 0, 1) * 255), ties to even.
+__device__ inline uint32_t unorm8(float c) { return (uint32_t)fxm_rint(fxm_clamp(c, 0.0f, 1.0f) * 255.0f); }
+
+// compositeFragment, qualityShaders.ts:664-671 (with dither, :659-662), into
+// rgba8 pixels; cinematic = 0 stores the scene image as the standard path's
+// canvas receives it.
+__global__ __launch_bounds__(256) void k_composite(const Half4* __restrict__ scene, const Half4* __restrict__ bloom,
+                                                   int w, int h, int bw, int bh, float exposure, int cinematic,
+                                                   uint32_t* __restrict__ frame) {
+  const int x = (int)(blockIdx.x * 16 + (threadIdx.x & 15u)), y = (int)(blockIdx.y * 16 + (threadIdx.x >> 4));
+  if (x >= w || y >= h) return;
+  V3 c;
+  if (cinematic) {
+    const float u = ((float)x + 0.5f) / (float)w, v = ((float)y + 0.5f) / (float)h;
+    const V3 sc = rgb(sample_image(scene, w, h, u, v));
+    const V3 bl = rgb(sample_image(bloom, bw, bh, u, v));
+    const V3 hdr = add(sc, mul(bl, 0.10f));
+    const float e = fxm_max(exposure, 0.25f);
+    c = v3(filmic1(hdr.x, e), filmic1(hdr.y, e), filmic1(hdr.z, e));
+    const float px = fxm_fract(((float)x + 0.5f) * 0.06711056f), py = fxm_fract(((float)y + 0.5f) * 0.00583715f);
+    const float d = (fxm_fract(52.9829189f * (px + py)) - 0.5f) / 255.0f;
+    c = v3(fxm_clamp(c.x + d, 0.0f, 1.0f), fxm_clamp(c.y + d, 0.0f, 1.0f), fxm_clamp(c.z + d, 0.0f, 1.0f));
+  } else {
+    c = rgb(image_texel(scene, x, y, w));
+  }
+  frame[(uint32_t)y * (uint32_t)w + (uint32_t)x] = unorm8(c.x) | (unorm8(c.y) << 8) | (unorm8(c.z) << 16) | (255u << 24);
+}
+
 }  // namespace
 
 namespace fx {
@@ -445,6 +828,58 @@ void launch_render_volume(const void* packed, const uint32_t* detail, void* out,
     case 6: k_render_volume<6><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
     default: k_render_volume<7><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
   }
+}
+
+uint64_t sprite_work_bytes(int count) {
+  const uint64_t n = count > 0 ? (uint64_t)count : 0;
+  const uint64_t capacity = 8 * n + 4096;
+  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+  return up(n * sizeof(FxSprite)) + up(n * 8) + 2 * up((uint64_t)kFxSpriteMaxBins * 4) + up(capacity * 4);
+}
+
+void launch_droplet_streaks(const void* particles, int count, const FxRender& r, void* target, int unorm8,
+                            void* work, hipStream_t s) {
+  const uint32_t view = (uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f);
+  if (count <= 0 || (view != 0u && view != 6u)) return;  // dropletFragment discards every fragment
+  const uint64_t n = (uint64_t)count, capacity = 8 * n + 4096;
+  auto up = [](uint64_t b) { return (b + 255) / 256 * 256; };
+  uint8_t* p = (uint8_t*)work;
+  FxSprite* sprites = (FxSprite*)p;
+  p += up(n * sizeof(FxSprite));
+  uint2* boxes = (uint2*)p;
+  p += up(n * 8);
+  uint32_t* counts = (uint32_t*)p;
+  p += up((uint64_t)kFxSpriteMaxBins * 4);
+  uint32_t* offsets = (uint32_t*)p;
+  p += up((uint64_t)kFxSpriteMaxBins * 4);
+  uint32_t* list = (uint32_t*)p;
+  const int bins_x = (r.width + kFxSpriteBin - 1) / kFxSpriteBin, bins_y = (r.height + kFxSpriteBin - 1) / kFxSpriteBin;
+  const int bins = bins_x * bins_y;
+  launch_fill_u32(counts, (uint64_t)bins, 0u, s);
+  k_sprite_setup<<<(uint32_t)((count + 255) / 256), 256, 0, s>>>((const float4*)particles, count, r, sprites, boxes,
+                                                                  counts, bins_x);
+  k_sprite_scan<<<1, 1024, 0, s>>>(counts, offsets, bins);
+  k_sprite_fill<<<(uint32_t)bins, 256, 0, s>>>(boxes, count, counts, offsets, list, (uint32_t)capacity, bins_x);
+  dim3 tiles((uint32_t)((r.width + kFxSpriteTile - 1) / kFxSpriteTile),
+             (uint32_t)((r.height + kFxSpriteTile - 1) / kFxSpriteTile));
+  if (unorm8)
+    k_sprite_draw<true><<<tiles, 256, 0, s>>>(sprites, boxes, counts, offsets, list, (uint32_t)capacity, count, bins_x,
+                                              target, r);
+  else
+    k_sprite_draw<false><<<tiles, 256, 0, s>>>(sprites, boxes, counts, offsets, list, (uint32_t)capacity, count,
+                                               bins_x, target, r);
+}
+
+void launch_render_post(const void* scene, void* bloom, int width, int height, float exposure, int cinematic,
+                        uint32_t* frame, hipStream_t s) {
+  const int bw = (width + 3) / 4 > 1 ? (width + 3) / 4 : 1, bh = (height + 3) / 4 > 1 ? (height + 3) / 4 : 1;
+  if (cinematic) {
+    dim3 g((uint32_t)((bw + 15) / 16), (uint32_t)((bh + 15) / 16));
+    k_bloom<<<g, 256, 0, s>>>((const Half4*)scene, width, height, (Half4*)bloom, bw, bh);
+  }
+  dim3 g((uint32_t)((width + 15) / 16), (uint32_t)((height + 15) / 16));
+  k_composite<<<g, 256, 0, s>>>((const Half4*)scene, (const Half4*)bloom, width, height, bw, bh, exposure, cinematic,
+                                frame);
 }
 
 }  // namespace fx

@@ -58,6 +58,16 @@ struct firex_solver {
   void* scene = nullptr;
   uint64_t scene_bytes = 0;
   int32_t scene_width = 0, scene_height = 0;
+  // presentation of the last firex_render_frame: bloom (rgba16f, quarter
+  // size) and the rgba8 frame
+  void* post = nullptr;
+  uint64_t post_bytes = 0;
+  bool has_frame = false;
+  // droplet streak rasterizer scratch (sprites, boxes, bins, lists) and the
+  // render constants of the last firex_render_volume
+  void* sprite_work = nullptr;
+  uint64_t sprite_work_size = 0;
+  FxRender last_render;
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -491,8 +501,18 @@ struct FieldView {
   int kind = 0;               // FIREX_KIND_*
 };
 
-uint64_t field_stored_bytes(const FieldView& v) {
-  return v.records * (uint64_t)v.comps * (v.kind == FIREX_KIND_F16 || v.kind == FIREX_KIND_BF16 ? 2 : 4);
+uint64_t field_element_bytes(int kind) {
+  return kind == FIREX_KIND_U8 ? 1 : kind == FIREX_KIND_F16 || kind == FIREX_KIND_BF16 ? 2 : 4;
+}
+
+uint64_t field_stored_bytes(const FieldView& v) { return v.records * (uint64_t)v.comps * field_element_bytes(v.kind); }
+
+// Bytes of the presentation buffers for a w x h frame: the rgba8 frame, then
+// the quarter-size rgba16f bloom.
+uint64_t post_frame_bytes(int32_t w, int32_t h) { return (uint64_t)w * (uint64_t)h * 4; }
+uint64_t post_bytes_for(int32_t w, int32_t h) {
+  const uint64_t bw = (uint64_t)std::max((w + 3) / 4, 1), bh = (uint64_t)std::max((h + 3) / 4, 1);
+  return (post_frame_bytes(w, h) + 255) / 256 * 256 + bw * bh * 8;
 }
 
 int field_view(firex_solver* s, int32_t field, FieldView* v) {
@@ -538,6 +558,13 @@ int field_view(firex_solver* s, int32_t field, FieldView* v) {
       v->ptr = s->particle_density;
       v->records = (uint64_t)s->cfg.particle_count;
       v->comps = 4;
+      break;
+    case FIREX_FIELD_FRAME:
+      if (!s->has_frame) return set_error(FIREX_ERR_INVALID, "the frame is written by firex_render_frame");
+      v->ptr = s->post;
+      v->records = (uint64_t)s->scene_width * (uint64_t)s->scene_height;
+      v->comps = 4;
+      v->kind = FIREX_KIND_U8;
       break;
     case FIREX_FIELD_SCENE:
       if (!s->scene) return set_error(FIREX_ERR_INVALID, "the scene image is written by firex_render_volume");
@@ -872,7 +899,8 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
   size_t elem = dtype == FIREX_DTYPE_F64 ? 8 : 4;
   if (dtype != FIREX_DTYPE_F32 && dtype != FIREX_DTYPE_F64 && dtype != FIREX_DTYPE_U32)
     return set_error(FIREX_ERR_INVALID, "unknown dtype");
-  if ((kind == 3) != (dtype == FIREX_DTYPE_U32) && !(kind == 3 && dtype == FIREX_DTYPE_F64))
+  const bool integral = kind == FIREX_KIND_U32 || kind == FIREX_KIND_U8;
+  if (integral != (dtype == FIREX_DTYPE_U32) && !(integral && dtype == FIREX_DTYPE_F64))
     return set_error(FIREX_ERR_INVALID, "dtype does not match the field");
   if (bytes != n * elem) return set_error(FIREX_ERR_INVALID, "byte count does not match the field size");
   FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize before read");
@@ -894,8 +922,10 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
     if (kind == 0) x = ((const float*)host)[i];
     else if (kind == 1) x = (double)(float)((const _Float16*)host)[i];
     else if (kind == 2) x = fx_bf16_bits_to_float(((const uint16_t*)host)[i]);
+    else if (kind == FIREX_KIND_U8) x = (double)((const uint8_t*)host)[i];
     else x = (double)((const uint32_t*)host)[i];
     if (dtype == FIREX_DTYPE_F64) ((double*)dst)[i] = x;
+    else if (dtype == FIREX_DTYPE_U32) ((uint32_t*)dst)[i] = (uint32_t)x;
     else ((float*)dst)[i] = (float)x;
   }
   return FIREX_OK;
@@ -1022,6 +1052,28 @@ int firex_pack_fields(firex_solver* s) {
   return check_launch(s, "packFields");
 }
 
+// The droplet streaks over target (rgba16f scene, or the rgba8 frame with
+// unorm8), when the particle field is live (particleFieldActive,
+// FireXEngine.ts:2636-2639, 2686-2689).
+static int draw_streaks(firex_solver* s, const FxRender& r, void* target, int unorm8) {
+  const int n = s->cfg.particle_count;
+  if (!s->particle_field_active || n <= 0 || !s->particles[0]) return FIREX_OK;
+  const uint64_t need = sprite_work_bytes(n);
+  if (!s->sprite_work) {
+    if (s->cfg.memory_budget_bytes && s->bytes + need > s->cfg.memory_budget_bytes)
+      return set_error(FIREX_ERR_BUDGET, "the droplet streak buffers (" + std::to_string(need) +
+                                             " bytes) do not fit the memory budget");
+    int rc = alloc(s, &s->sprite_work, need, "droplet streak buffers");
+    if (rc) return rc;
+    s->sprite_work_size = need;
+    s->bytes += need;
+  }
+  timed(s, FIREX_PASS_RENDER, [&] {
+    launch_droplet_streaks(s->particles[s->particle_index], n, r, target, unorm8, s->sprite_work, s->stream);
+  });
+  return check_launch(s, "droplet streaks");
+}
+
 int firex_render_volume(firex_solver* s, const firex_render_config* rc) {
   int rc0 = check_usable(s);
   if (rc0) return rc0;
@@ -1061,12 +1113,55 @@ int firex_render_volume(firex_solver* s, const firex_render_config* rc) {
   }
   s->scene_width = rc->width;
   s->scene_height = rc->height;
+  s->has_frame = false;
   float u[FXU_COUNT];
   write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
                  s->cfg.cinematic != 0, u, s->cfg.particle_count, rc->width, rc->height, steps);
   const FxRender r = make_render(u, s->dims, rc->width, rc->height);
+  s->last_render = r;
   timed(s, FIREX_PASS_RENDER, [&] { launch_render_volume(s->packed, s->detail, s->scene, r, s->stream); });
-  return check_launch(s, "volume render");
+  rc1 = check_launch(s, "volume render");
+  if (rc1) return rc1;
+  // the cinematic scene pass draws the droplets over the volume; the
+  // standard path draws them into the canvas (firex_render_frame)
+  return s->cfg.cinematic ? draw_streaks(s, r, s->scene, 0) : FIREX_OK;
+}
+
+int firex_render_frame(firex_solver* s, const firex_render_config* rc) {
+  int r0 = firex_render_volume(s, rc);
+  if (r0) return r0;
+  const uint64_t need = post_bytes_for(rc->width, rc->height);
+  if (need > s->post_bytes) {
+    if (s->cfg.memory_budget_bytes && s->bytes + need - s->post_bytes > s->cfg.memory_budget_bytes)
+      return set_error(FIREX_ERR_BUDGET, "the presentation buffers (" + std::to_string(need) +
+                                             " bytes) do not fit the memory budget");
+    void* p = nullptr;
+    r0 = alloc(s, &p, need, "presentation buffers");
+    if (r0) return r0;
+    if (s->post) {
+      FX_HIP(s, hipStreamSynchronize(s->stream), "presentation resize");
+      s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), s->post));
+      hipFree(s->post);
+    }
+    s->bytes += need - s->post_bytes;
+    s->post = p;
+    s->post_bytes = need;
+  }
+  uint8_t* base = (uint8_t*)s->post;
+  void* bloom = base + (post_frame_bytes(rc->width, rc->height) + 255) / 256 * 256;
+  const float exposure = (float)s->params.exposure;
+  timed(s, FIREX_PASS_PRESENT, [&] {
+    launch_render_post(s->scene, bloom, rc->width, rc->height, exposure, s->cfg.cinematic != 0 ? 1 : 0,
+                       (uint32_t*)base, s->stream);
+  });
+  r0 = check_launch(s, "presentation");
+  if (r0) return r0;
+  if (!s->cfg.cinematic) {
+    r0 = draw_streaks(s, s->last_render, base, 1);
+    if (r0) return r0;
+  }
+  s->has_frame = true;
+  return FIREX_OK;
 }
 
 int firex_reduce(firex_solver* s, int32_t which, double* out) {

@@ -100,7 +100,21 @@ struct FxRender {
   float time;                  // params.time.y
   float chem_y, chem_z;        // params.chemistry.yz
   float zoom;                  // params.fire.w
+  float canvas_height;         // params.render.y
+  float nozzle;                // params.controls.z
 };
+
+// Droplet streak of one droplet in NDC (dropletVertex): quad corner (a, b)
+// sits at (cx, cy) + normal * a * radius + tangent * b * half, normal =
+// (-ty, tx).
+struct FxSprite {
+  float cx, cy, tx, ty, radius, half, mass, pad;
+};
+// Streak rasterizer bins: a bin is kFxSpriteBin x kFxSpriteBin pixels, a
+// tile (one block) kFxSpriteTile x kFxSpriteTile.
+#define kFxSpriteBin 128
+#define kFxSpriteTile 16
+#define kFxSpriteMaxBins ((16384 / kFxSpriteBin) * (16384 / kFxSpriteBin))
 
 // Edge of the periodic detail volume (FireXEngine.ts:1622) and the WGSL
 // constant 1.0 / 2.2 as an f32 (a const-expression, evaluated exactly and

@@ -8,7 +8,9 @@ export type FireXFieldName =
   /** per SPH droplet: 8 (positionAge, velocityMass) and 4 (density, near density, pressures) values */
   | 'particles' | 'particleDensity'
   /** the last renderVolume image: width x height pixels of 4 values (rgba16f), row 0 at the top */
-  | 'scene';
+  | 'scene'
+  /** the last renderFrame: width x height rgba8 pixels */
+  | 'frame';
 
 export type FireXReductionName =
   | 'sumFuel' | 'sumOxygen' | 'sumProducts' | 'sumVapor' | 'sumTemperature' | 'sumReaction'
@@ -52,7 +54,7 @@ export interface FireXHipGridInfo {
 }
 
 export interface FireXFieldLayout {
-  kind: 'f32' | 'f16' | 'bf16' | 'u32';
+  kind: 'f32' | 'f16' | 'bf16' | 'u32' | 'u8';
   kindCode: number;
   components: number;
   records: number;
@@ -81,6 +83,8 @@ export declare class FireXHipSolver {
   packFields(): void;
   /** Volume ray marcher into the rgba16f scene image (readField('scene')). */
   renderVolume(width: number, height: number, raySteps?: number): void;
+  /** A presented frame: width x height rgba8 pixels. */
+  renderFrame(width: number, height: number, raySteps?: number): Uint8Array;
   /** Stored format of a field on the device. */
   fieldLayout(name: FireXFieldName): FireXFieldLayout;
   /** The field's stored bytes, verbatim. */

@@ -113,7 +113,7 @@ class FireXHipSolver {
 
   /** Copies a field to a new (or the given) Float32Array / Float64Array / Uint32Array. */
   readField(name, out) {
-    if (!(name in FIELD_COMPONENTS) && !(name in PARTICLE_COMPONENTS) && name !== 'scene') {
+    if (!(name in FIELD_COMPONENTS) && !(name in PARTICLE_COMPONENTS) && name !== 'scene' && name !== 'frame') {
       throw new RangeError(`Unknown Fire-X field: ${name}`);
     }
     const layout = native.fieldLayout(this.requireHandle(), name);
@@ -132,7 +132,14 @@ class FireXHipSolver {
     native.renderVolume(this.requireHandle(), { width, height, raySteps });
   }
 
-  /** Stored format of a field: {kind: 'f32'|'f16'|'bf16'|'u32', components, records, storedBytes}. */
+  /** A presented frame (ray march, bloom and filmic composite with the cinematic presentation): a new
+   *  Uint8Array of width x height rgba8 pixels, row 0 at the top. */
+  renderFrame(width, height, raySteps = 0) {
+    native.renderFrame(this.requireHandle(), { width, height, raySteps });
+    return this.readFieldRaw('frame');
+  }
+
+  /** Stored format of a field: {kind: 'f32'|'f16'|'bf16'|'u32'|'u8', components, records, storedBytes}. */
   fieldLayout(name) {
     return native.fieldLayout(this.requireHandle(), name);
   }

@@ -335,7 +335,7 @@ static int field_id(const char* name) {
   static const char* names[] = {"flow",          "species",    "pressure",        "correction",
                                 "divergence",    "reaction",   "liquid",          "flowPredicted",
                                 "packedFlow",    "packedChemistry", "packedMedia",
-                                "particles",     "particleDensity", "scene"};
+                                "particles",     "particleDensity", "scene",         "frame"};
   for (int i = 0; i < (int)(sizeof names / sizeof names[0]); ++i)
     if (strcmp(name, names[i]) == 0) return i;
   return -1;
@@ -411,6 +411,21 @@ static napi_value js_render_volume(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* renderFrame(h, {width, height, raySteps}): a presented rgba8 frame ("frame"). */
+static napi_value js_render_frame(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  firex_render_config rc;
+  memset(&rc, 0, sizeof rc);
+  rc.width = (int32_t)num_or(env, argv[1], "width", 0);
+  rc.height = (int32_t)num_or(env, argv[1], "height", 0);
+  rc.ray_steps = (int32_t)num_or(env, argv[1], "raySteps", 0);
+  FX_CHECK(env, firex_render_frame(s, &rc));
+  return NULL;
+}
+
 /* resolveOfflineDims(tier, override: [x, y, z] | null) -> [x, y, z]
  * (resolveFireXOfflineDimensions, FireXEngine.ts:279-290) */
 static napi_value js_resolve_offline_dims(napi_env env, napi_callback_info info) {
@@ -454,10 +469,10 @@ static napi_value js_field_layout(napi_env env, napi_callback_info info) {
   }
   firex_field_layout l;
   FX_CHECK(env, firex_get_field_layout(s, field_id(name), &l));
-  static const char* kinds[] = {"f32", "f16", "bf16", "u32"};
+  static const char* kinds[] = {"f32", "f16", "bf16", "u32", "u8"};
   napi_value o, k;
   napi_create_object(env, &o);
-  napi_create_string_utf8(env, kinds[l.kind & 3], NAPI_AUTO_LENGTH, &k);
+  napi_create_string_utf8(env, l.kind >= 0 && l.kind <= 4 ? kinds[l.kind] : "?", NAPI_AUTO_LENGTH, &k);
   napi_set_named_property(env, o, "kind", k);
   set_num(env, o, "kindCode", l.kind);
   set_num(env, o, "components", l.components);
@@ -675,6 +690,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"fieldLayout", NULL, js_field_layout, NULL, NULL, NULL, napi_default, NULL},
       {"resolveOfflineDims", NULL, js_resolve_offline_dims, NULL, NULL, NULL, napi_default, NULL},
       {"renderVolume", NULL, js_render_volume, NULL, NULL, NULL, napi_default, NULL},
+      {"renderFrame", NULL, js_render_frame, NULL, NULL, NULL, napi_default, NULL},
       {"readFieldRaw", NULL, js_read_field_raw, NULL, NULL, NULL, napi_default, NULL},
       {"fieldChecksum", NULL, js_field_checksum, NULL, NULL, NULL, napi_default, NULL},
       {"checksumBytes", NULL, js_checksum_bytes, NULL, NULL, NULL, napi_default, NULL},

@@ -36,13 +36,13 @@ FIELDS = {
     "flow": 0, "species": 1, "pressure": 2, "correction": 3,
     "divergence": 4, "reaction": 5, "liquid": 6, "flow_predicted": 7,
     "packed_flow": 8, "packed_chemistry": 9, "packed_media": 10,
-    "particles": 11, "particle_density": 12, "scene": 13,
+    "particles": 11, "particle_density": 12, "scene": 13, "frame": 14,
 }
 VECTOR_FIELDS = {"flow", "species", "flow_predicted", "packed_flow", "packed_chemistry", "packed_media"}
 # per-particle fields: components per particle record
 PARTICLE_FIELDS = {"particles": 8, "particle_density": 4}
 DTYPE_F32, DTYPE_F64, DTYPE_U32, DTYPE_RAW = 0, 1, 2, 3
-KINDS = {0: "f32", 1: "f16", 2: "bf16", 3: "u32"}  # stored formats (firex_field_layout.kind)
+KINDS = {0: "f32", 1: "f16", 2: "bf16", 3: "u32", 4: "u8"}  # stored formats (firex_field_layout.kind)
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 HEAVY = {"auto": 0, "off": 1, "f16": 2, "f32": 3}
 PLANS = {"reference": 0, "fused": 1}
@@ -143,7 +143,7 @@ class DeviceLimits(ctypes.Structure):
 
 
 PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack", "droplets",
-          "render")
+          "render", "present")
 PASS_COUNT = 10  # FIREX_PASS_COUNT
 
 
@@ -195,6 +195,7 @@ _SIGNATURES = {
     "firex_pack_fields": (ctypes.c_int, [_P]),
     "firex_get_field_layout": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(FieldLayout)]),
     "firex_render_volume": (ctypes.c_int, [_P, ctypes.POINTER(RenderConfig)]),
+    "firex_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(RenderConfig)]),
     "firex_field_checksum": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]),
     "firex_checksum_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
     "firex_get_info": (ctypes.c_int, [_P, ctypes.POINTER(Info)]),
@@ -444,7 +445,7 @@ class FireXHipSolver:
 
     # fields -----------------------------------------------------------------
     def _shape(self, name: str):
-        if name == "scene":
+        if name in ("scene", "frame"):
             w, h = getattr(self, "_scene_size", (0, 0))
             return (h, w, 4)
         if name in PARTICLE_FIELDS:
@@ -486,7 +487,7 @@ class FireXHipSolver:
         n = self.field_layout(name)["stored_bytes"]
         buf = ctypes.create_string_buffer(max(1, n))
         _check(self._lib.firex_read_field(self._h, FIELDS[name], buf, n, DTYPE_RAW))
-        return buf.raw[:n]
+        return buf.raw if n else b""
 
     def field_checksum(self, name: str) -> int:
         """64-bit checksum of the field's stored bytes, computed on the device
@@ -503,6 +504,28 @@ class FireXHipSolver:
         cfg = RenderConfig(int(width), int(height), int(ray_steps))
         _check(self._lib.firex_render_volume(self._h, ctypes.byref(cfg)))
         self._scene_size = (int(width), int(height))
+
+    def render_frame(self, width: int, height: int, ray_steps: int = 0, read: bool = True):
+        """A presented frame: the ray march, then (cinematic presentation)
+        the quarter-resolution bloom and the filmic tone-map / dither
+        composite, as (height, width, 4) uint8 rgba, row 0 at the top.
+        read=False leaves the frame on the device (read it later with
+        read_frame) and returns None."""
+        cfg = RenderConfig(int(width), int(height), int(ray_steps))
+        _check(self._lib.firex_render_frame(self._h, ctypes.byref(cfg)))
+        self._scene_size = (int(width), int(height))
+        return self.read_frame() if read else None
+
+    def read_frame(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """The last presented frame, copied from the device straight into
+        `out` (a C-contiguous (height, width, 4) uint8 array) or a new one."""
+        shape = self._shape("frame")
+        if out is None:
+            out = np.empty(shape, dtype=np.uint8)
+        elif out.shape != shape or out.dtype != np.uint8 or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a C-contiguous uint8 array of shape {shape}")
+        _check(self._lib.firex_read_field(self._h, FIELDS["frame"], out.ctypes.data, out.nbytes, DTYPE_RAW))
+        return out
 
     def pack_fields(self) -> None:
         """packFields: fill the three render volumes (read them as packed_flow,

@@ -63,7 +63,9 @@ enum {
   FIREX_FIELD_PARTICLE_DENSITY = 12,/* 4 f32: density, near density, pressure, near p.   */
   /* image of the last firex_render_volume: width * height pixels of 4 x f16
      (rgba16float), row 0 at the top */
-  FIREX_FIELD_SCENE = 13
+  FIREX_FIELD_SCENE = 13,
+  /* the last firex_render_frame: width * height rgba8 pixels, row 0 at the top */
+  FIREX_FIELD_FRAME = 14
 };
 
 /* Host dtypes of firex_read_field / firex_write_field.  FIREX_DTYPE_RAW
@@ -72,7 +74,7 @@ enum {
 enum { FIREX_DTYPE_F32 = 0, FIREX_DTYPE_F64 = 1, FIREX_DTYPE_U32 = 2, FIREX_DTYPE_RAW = 3 };
 
 /* Storage formats of a field on the device. */
-enum { FIREX_KIND_F32 = 0, FIREX_KIND_F16 = 1, FIREX_KIND_BF16 = 2, FIREX_KIND_U32 = 3 };
+enum { FIREX_KIND_F32 = 0, FIREX_KIND_F16 = 1, FIREX_KIND_BF16 = 2, FIREX_KIND_U32 = 3, FIREX_KIND_U8 = 4 };
 
 typedef struct firex_field_layout {
   int32_t kind;          /* FIREX_KIND_* of each component                    */
@@ -305,13 +307,31 @@ int firex_pack_fields(firex_solver* solver);
  * come from the solver's parameters (cameraYaw, cameraPitch, cameraDistance,
  * viewMode, exposure, viewZoom, frontDefinition, opticalDetailTarget).
  * Texture filtering is defined in f32 arithmetic (linear, clamp-to-edge for
- * the volumes, repeat for the 64^3 detail volume). */
+ * the volumes, repeat for the 64^3 detail volume).  With the cinematic
+ * presentation and a live particle field (water on), the droplet streaks
+ * (dropletVertex / dropletFragment, qualityShaders.ts:156-232) are drawn over
+ * the image in instance order, as the reference's scene pass does
+ * (FireXEngine.ts:2636-2639); rasterization and blending are defined in f32
+ * arithmetic (a pixel is covered when its centre's quad coordinates lie
+ * strictly inside (-1, 1)^2; src * a + dst * (1 - a), stored as f16). */
 typedef struct firex_render_config {
   int32_t width, height;  /* output pixels, each in [1, 16384]                 */
   int32_t ray_steps;      /* 0 = firex_config.ray_steps                         */
   int32_t reserved[5];
 } firex_render_config;
 int firex_render_volume(firex_solver* solver, const firex_render_config* config);
+/* A presented frame (encodeRender, FireXEngine.ts:2607-2680): the ray march
+ * above, then, with the cinematic presentation (firex_config.cinematic), the
+ * quarter-resolution bloom and the filmic tone-map / dither composite
+ * (qualityShaders.ts:580-671); without it, the scene image itself.  Stored as
+ * rgba8 (round(clamp(c, 0, 1) * 255), ties to even), readable as
+ * FIREX_FIELD_FRAME.  The standard presentation then draws the droplet
+ * streaks into the frame (FireXEngine.ts:2686-2689), blending in f32 with
+ * the destination read as byte / 255.  The standard path stores the scene
+ * image rounded to f16 first, where the reference's standard pipeline writes
+ * its f32 fragment colour to the canvas: a byte can differ where that
+ * rounding moves c * 255 across a half step. */
+int firex_render_frame(firex_solver* solver, const firex_render_config* config);
 
 /* FP64 reduction on the device (FIREX_REDUCE_*). */
 int firex_reduce(firex_solver* solver, int32_t which, double* out);
@@ -328,6 +348,7 @@ enum {
   FIREX_PASS_PACK = 6,                /* firex_pack_fields                        */
   FIREX_PASS_DROPLETS = 7,            /* SPH density, integration, deposition     */
   FIREX_PASS_RENDER = 8,              /* firex_render_volume's ray march          */
+  FIREX_PASS_PRESENT = 9,             /* firex_render_frame's bloom + composite   */
   FIREX_PASS_COUNT = 10
 };
 typedef struct firex_pass_timing {

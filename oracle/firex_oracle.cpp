@@ -1269,6 +1269,124 @@ template <typename T> struct State {
       float v = u[22];
       return v > 0.0f ? (int)(uint32_t)v : 0;
     }
+
+    // ---- droplet streaks (dropletVertex / dropletFragment, qualityShaders.ts:156-232)
+    // The camera of cameraEye / projectWorld (qualityShaders.ts:133-176).
+    struct Camera {
+      F3 eye, fwd, right, up;
+      float lens;
+    };
+    Camera camera() const {
+      const F3 E = f3(ext(0), ext(1), ext(2));
+      const F3 target = f3(E.x * 0.5f, E.y * 0.40f, E.z * 0.5f);
+      const float yaw = u[16], pitch = u[17];
+      const F3 orbit = f3(fxm_sin(yaw) * fxm_cos(pitch), fxm_sin(pitch), fxm_cos(yaw) * fxm_cos(pitch));
+      const float frame = fxm_max(E.x, fxm_max(E.y, E.z));
+      Camera c;
+      c.eye = plus(target, scale(scale(orbit, 0.90f + u[18] * 0.18f), frame));
+      c.fwd = unit(minus(target, c.eye));
+      c.right = unit(cross(c.fwd, f3(0.0f, 1.0f, 0.0f)));
+      c.up = unit(cross(c.right, c.fwd));
+      c.lens = 0.72f / fxm_clamp(u[39], 0.5f, 3.0f);
+      return c;
+    }
+    void project(const Camera& c, F3 point, float out[2]) const {
+      const F3 rel = minus(point, c.eye);
+      const float depth = fxm_max(dot(rel, c.fwd), 0.0001f);
+      out[0] = dot(rel, c.right) / (depth * u[19] * c.lens);
+      out[1] = dot(rel, c.up) / (depth * c.lens);
+    }
+    // dropletVertex for droplet i: the quad's centre, unit tangent, radius and
+    // half length in NDC (corner (cx, cy) sits at centre + normal * cx *
+    // radius + tangent * cy * half).  False when it draws nothing (massless or
+    // outside the unit box: the vertex collapses off screen).
+    struct Sprite {
+      float cx, cy, tx, ty, radius, half, mass;
+    };
+    bool sprite(const Camera& cam, int i, Sprite& sp) const {
+      const float* q = &s.particles[s.particleIndex][(size_t)i * 8];
+      if (!(q[7] > 0.0001f && q[0] >= 0.0f && q[1] >= 0.0f && q[2] >= 0.0f && q[0] <= 1.0f && q[1] <= 1.0f &&
+            q[2] <= 1.0f))
+        return false;
+      const F3 world = times(f3(q[0], q[1], q[2]), f3(ext(0), ext(1), ext(2)));
+      float head[2], tail[2];
+      project(cam, world, head);
+      project(cam, minus(world, scale(f3(q[4], q[5], q[6]), 0.018f)), tail);
+      const float sx = tail[0] - head[0], sy = tail[1] - head[1];
+      const float length = fxm_sqrt(sx * sx + sy * sy);
+      const float streak = fxm_min(length, 0.032f);
+      const bool along = streak > 0.00001f;
+      sp.tx = along ? sx / length : 1.0f;
+      sp.ty = along ? sy / length : 0.0f;
+      const float pixel_radius = u[26] >= 0.5f ? 1.15f : 1.65f;
+      sp.radius = pixel_radius * 2.0f / fxm_max(u[21], 1.0f);
+      sp.half = sp.radius + streak * 0.5f;
+      sp.cx = head[0];
+      sp.cy = head[1];
+      sp.mass = q[7];
+      return true;
+    }
+    // dropletFragment at quad coordinates (lx, ly): false when discarded.
+    bool droplet_fragment(float lx, float ly, float mass, uint32_t view, float rgba[4]) const {
+      const float m = fxm_max(std::fabs(ly) - 0.48f, 0.0f);
+      const float d = fxm_sqrt(lx * lx + m * m);
+      const float coverage = 1.0f - fxm_smoothstep(0.64f, 1.0f, d);
+      if (coverage <= 0.001f) return false;
+      const float highlight = fxm_pow(fxm_clamp(1.0f - std::fabs(lx + 0.30f), 0.0f, 1.0f), 5.0f);
+      const F3 base = lerp3(f3(0.27f, 0.34f, 0.38f), f3(0.76f, 0.86f, 0.91f), highlight * 0.62f);
+      const float boost = view == 6u ? 1.38f : 1.0f;
+      const float alpha = coverage * fxm_clamp(0.13f + mass * 0.15f, 0.12f, 0.42f) * boost;
+      const F3 linear = scale(base, 0.72f);
+      const F3 c = u[42] >= 0.5f ? linear : aces(linear);
+      rgba[0] = c.x;
+      rgba[1] = c.y;
+      rgba[2] = c.z;
+      rgba[3] = alpha;
+      return true;
+    }
+    // Draws the streaks over img (width * height rgba, row 0 at the top) in
+    // instance order.  A pixel is rasterized for droplet i when its centre's
+    // quad coordinates (the fragment's interpolated `local`) lie strictly
+    // inside (-1, 1)^2: local = (dot(p - c, normal) / radius, dot(p - c,
+    // tangent) / half) with p the pixel centre in NDC.  Blending (src-alpha,
+    // one-minus-src-alpha; alpha: one, one-minus-src-alpha) runs in f32 and
+    // each result is stored in the target's format: f16 (img holds f16
+    // values) or unorm8 (img holds bytes 0..255, read as byte / 255).
+    void draw_droplets(std::vector<float>& img, bool unorm8_target) const {
+      const uint32_t view = (uint32_t)fxm_clamp(fxm_rint(u[24]), 0.0f, 7.0f);
+      if (view != 0u && view != 6u) return;
+      const Camera cam = camera();
+      for (int i = 0; i < s.particleCount; ++i) {
+        Sprite sp;
+        if (!sprite(cam, i, sp)) continue;
+        const float nx = -sp.ty, ny = sp.tx;
+        // a generous pixel window: each corner lies within radius + half of the centre per axis
+        const double reach = (double)sp.radius + (double)sp.half;
+        const double fx0 = ((double)sp.cx - reach + 1.0) * 0.5 * width - 0.5;
+        const double fx1 = ((double)sp.cx + reach + 1.0) * 0.5 * width - 0.5;
+        const double fy0 = (1.0 - ((double)sp.cy + reach)) * 0.5 * height - 0.5;
+        const double fy1 = (1.0 - ((double)sp.cy - reach)) * 0.5 * height - 0.5;
+        const int x0 = (int)std::max(0.0, std::floor(fx0) - 2.0), x1 = (int)std::min(width - 1.0, std::ceil(fx1) + 2.0);
+        const int y0 = (int)std::max(0.0, std::floor(fy0) - 2.0), y1 = (int)std::min(height - 1.0, std::ceil(fy1) + 2.0);
+        for (int y = y0; y <= y1; ++y)
+          for (int x = x0; x <= x1; ++x) {
+            const float uvx = ((float)x + 0.5f) / (float)width, uvy = 1.0f - ((float)y + 0.5f) / (float)height;
+            const float px = uvx * 2.0f - 1.0f, py = uvy * 2.0f - 1.0f;
+            const float dx = px - sp.cx, dy = py - sp.cy;
+            const float lx = (dx * nx + dy * ny) / sp.radius, ly = (dx * sp.tx + dy * sp.ty) / sp.half;
+            if (!(std::fabs(lx) < 1.0f && std::fabs(ly) < 1.0f)) continue;
+            float src[4];
+            if (!droplet_fragment(lx, ly, sp.mass, view, src)) continue;
+            float* d = &img[((size_t)y * width + x) * 4];
+            const float a = src[3], keep = 1.0f - a;
+            for (int k = 0; k < 4; ++k) {
+              const float dst = unorm8_target ? d[k] / 255.0f : d[k];
+              const float v = (k < 3 ? src[k] * a : a) + dst * keep;
+              d[k] = unorm8_target ? (float)fxm_rint(fxm_clamp(v, 0.0f, 1.0f) * 255.0f) : round_to_f16(v);
+            }
+          }
+      }
+    }
   };
 
   V4<T> store_species(const V4<T>& s) const;
@@ -1623,6 +1741,17 @@ double orc_get_parameter(void* p, const char* name) {
   if (n == "domainWidth") return q.domainWidth;
   if (n == "domainHeight") return q.domainHeight;
   if (n == "domainDepth") return q.domainDepth;
+  if (n == "waterFlow") return q.waterFlow;
+  if (n == "sprayAngle") return q.sprayAngle;
+  if (n == "cameraYaw") return q.cameraYaw;
+  if (n == "cameraPitch") return q.cameraPitch;
+  if (n == "cameraDistance") return q.cameraDistance;
+  if (n == "viewMode") return q.viewMode;
+  if (n == "nozzleType") return q.nozzleType;
+  if (n == "exposure") return q.exposure;
+  if (n == "frontDefinition") return q.frontDefinition;
+  if (n == "viewZoom") return q.viewZoom;
+  if (n == "opticalDetailTarget") return q.opticalDetailTarget;
   return NAN;
 }
 
@@ -1842,16 +1971,152 @@ int orc_render_volume(void* p, int width, int height, int ray_steps, double* out
   s.write_uniforms(0.0);
   static const std::vector<uint8_t> det = State<float>::detail_volume(64);
   const State<float>::Renderer r{s, pf, pc, pm, det, s.u, width, height};
+  std::vector<float> img((size_t)width * height * 4);
 #pragma omp parallel for schedule(dynamic, 4)
   for (int y = 0; y < height; ++y)
     for (int x = 0; x < width; ++x) {
       const V3<float> c = r.pixel(x, y);
-      double* o = out + ((size_t)y * width + x) * 4;
+      float* o = &img[((size_t)y * width + x) * 4];
       o[0] = round_to_f16(c.x);
       o[1] = round_to_f16(c.y);
       o[2] = round_to_f16(c.z);
-      o[3] = 1.0;
+      o[3] = 1.0f;
     }
+  // the cinematic scene pass draws the droplet streaks after the volume
+  // (FireXEngine.ts:2636-2639); the standard path draws them into the canvas
+  if (s.u[42] >= 0.5f && s.particleFieldActive && s.particleCount > 0) r.draw_droplets(img, false);
+  for (size_t i = 0; i < img.size(); ++i) out[i] = img[i];
+  return 0;
+}
+
+// The standard presentation's droplet streaks over an rgba8 frame (the canvas
+// the standard pipeline draws into, FireXEngine.ts:2673-2690); the renderer's
+// uniforms are those of the last orc_render_volume.
+static void droplets_over_frame(Handle* h, int width, int height, uint8_t* frame) {
+  State<float>& s = h->f;
+  if (!(s.particleFieldActive && s.particleCount > 0)) return;
+  std::vector<V4<float>> none;
+  static const std::vector<uint8_t> det = State<float>::detail_volume(64);
+  const State<float>::Renderer r{s, none, none, none, det, s.u, width, height};
+  std::vector<float> img((size_t)width * height * 4);
+  for (size_t i = 0; i < img.size(); ++i) img[i] = (float)frame[i];
+  r.draw_droplets(img, true);
+  for (size_t i = 0; i < img.size(); ++i) frame[i] = (uint8_t)img[i];
+}
+
+// ---- presentation (FIREX_POST_WGSL, qualityShaders.ts:580-671) ----------
+namespace post {
+struct Img {
+  int w, h;
+  std::vector<float> px;  // w * h * 4, row 0 at the top
+  float at(int x, int y, int c) const { return px[((size_t)y * w + x) * 4 + c]; }
+};
+// linear, clamp-to-edge sampling of a 2D image at uv (v = 0: row 0)
+static void sample(const Img& im, float u, float v, float out[3]) {
+  const float tx = u * (float)im.w - 0.5f, ty = v * (float)im.h - 0.5f;
+  const float fx0 = fxm_floor(tx), fy0 = fxm_floor(ty);
+  const float fx = tx - fx0, fy = ty - fy0;
+  const int x0 = std::min(std::max((int)fx0, 0), im.w - 1), x1 = std::min(std::max((int)fx0 + 1, 0), im.w - 1);
+  const int y0 = std::min(std::max((int)fy0, 0), im.h - 1), y1 = std::min(std::max((int)fy0 + 1, 0), im.h - 1);
+  for (int c = 0; c < 3; ++c) {
+    const float a = fxm_mix(im.at(x0, y0, c), im.at(x1, y0, c), fx);
+    const float b = fxm_mix(im.at(x0, y1, c), im.at(x1, y1, c), fx);
+    out[c] = fxm_mix(a, b, fy);
+  }
+}
+// bloomFragment + brightExtract, qualityShaders.ts:621-644
+static Img bloom(const Img& scene) {
+  Img b;
+  b.w = std::max((scene.w + 3) / 4, 1);
+  b.h = std::max((scene.h + 3) / 4, 1);
+  b.px.assign((size_t)b.w * b.h * 4, 0.0f);
+  const float rx = 1.0f / fxm_max((float)scene.w, 1.0f) * 5.5f, ry = 1.0f / fxm_max((float)scene.h, 1.0f) * 5.5f;
+  // the eight taps of bloomFragment: uv + (a, b), or uv - (a, b)
+  struct Tap {
+    bool minus;
+    float a, b;
+  };
+  const Tap taps[8] = {{false, rx, 0.0f}, {true, rx, 0.0f},  {false, 0.0f, ry}, {true, 0.0f, ry},
+                       {false, rx, ry},   {true, rx, ry},    {false, rx, -ry},  {false, -rx, ry}};
+  for (int y = 0; y < b.h; ++y)
+    for (int x = 0; x < b.w; ++x) {
+      const float u = ((float)x + 0.5f) / (float)b.w, v = ((float)y + 0.5f) / (float)b.h;
+      float t[3], c[3];
+      sample(scene, u, v, t);
+      for (int k = 0; k < 3; ++k) c[k] = t[k] * 0.20f;
+      for (const Tap& tp : taps) {
+        sample(scene, tp.minus ? u - tp.a : u + tp.a, tp.minus ? v - tp.b : v + tp.b, t);
+        for (int k = 0; k < 3; ++k) c[k] = c[k] + t[k] * 0.10f;
+      }
+      const float lum = c[0] * 0.2126f + c[1] * 0.7152f + c[2] * 0.0722f;
+      const float threshold = 1.15f, knee = 0.35f;
+      const float soft = fxm_clamp((lum - threshold + knee) / (2.0f * knee), 0.0f, 1.0f);
+      const float contribution = fxm_max(lum - threshold, 0.0f) + soft * soft * knee;
+      const float d = fxm_max(lum, 0.0001f);
+      for (int k = 0; k < 3; ++k) b.px[((size_t)y * b.w + x) * 4 + k] = round_to_f16(c[k] * contribution / d);
+      b.px[((size_t)y * b.w + x) * 4 + 3] = 1.0f;
+    }
+  return b;
+}
+static uint8_t unorm8(float c) { return (uint8_t)fxm_rint(fxm_clamp(c, 0.0f, 1.0f) * 255.0f); }
+}  // namespace post
+
+// The presentation passes over a given scene image (w * h rgba, row 0 at
+// the top): with cinematic != 0, bloomFragment into the quarter-size bloom
+// image and compositeFragment (filmic tone map of scene + 0.1 bloom at the
+// exposure, then the dither), qualityShaders.ts:613-671; else the scene colour
+// itself (the standard pipeline draws straight into the canvas).  frame: w * h
+// * 4 bytes, unorm8 = rint(clamp(c, 0, 1) * 255) and alpha 255 (an opaque
+// canvas).  bloom_out (may be null): ceil(w/4) * ceil(h/4) * 4 floats.
+int orc_present(const float* scene_rgba, int width, int height, float exposure_param, int cinematic,
+                uint8_t* out, float* bloom_out) {
+  if (width <= 0 || height <= 0 || !scene_rgba || !out) return 1;
+  post::Img scene{width, height, std::vector<float>(scene_rgba, scene_rgba + (size_t)width * height * 4)};
+  const float exposure = fxm_max(exposure_param, 0.25f);
+  post::Img bl;
+  if (cinematic) {
+    bl = post::bloom(scene);
+    if (bloom_out) std::memcpy(bloom_out, bl.px.data(), bl.px.size() * sizeof(float));
+  }
+  const float gamma = (float)(1.0 / 2.2);
+  for (int y = 0; y < height; ++y)
+    for (int x = 0; x < width; ++x) {
+      float c[3];
+      if (cinematic) {
+        const float u = ((float)x + 0.5f) / (float)width, v = ((float)y + 0.5f) / (float)height;
+        float sc[3], b[3];
+        post::sample(scene, u, v, sc);
+        post::sample(bl, u, v, b);
+        // dither(position.xy) at the fragment centre
+        const float px = fxm_fract(((float)x + 0.5f) * 0.06711056f), py = fxm_fract(((float)y + 0.5f) * 0.00583715f);
+        const float dither = (fxm_fract(52.9829189f * (px + py)) - 0.5f) / 255.0f;
+        for (int k = 0; k < 3; ++k) {
+          const float e = fxm_max((sc[k] + b[k] * 0.10f) * exposure, 0.0f);
+          const float m = fxm_clamp((e * (2.51f * e + 0.03f)) / (e * (2.43f * e + 0.59f) + 0.14f), 0.0f, 1.0f);
+          c[k] = fxm_clamp(fxm_pow(m, gamma) + dither, 0.0f, 1.0f);
+        }
+      } else {
+        for (int k = 0; k < 3; ++k) c[k] = scene.at(x, y, k);
+      }
+      uint8_t* o = out + ((size_t)y * width + x) * 4;
+      o[0] = post::unorm8(c[0]);
+      o[1] = post::unorm8(c[1]);
+      o[2] = post::unorm8(c[2]);
+      o[3] = 255;
+    }
+  return 0;
+}
+
+// A presented frame: the ray march (rgba16f scene image), then the
+// presentation passes above.  out: width * height * 4 bytes.  FP32 oracle only.
+int orc_render_frame(void* p, int width, int height, int ray_steps, uint8_t* out) {
+  Handle* h = static_cast<Handle*>(p);
+  std::vector<double> img((size_t)width * height * 4);
+  if (orc_render_volume(p, width, height, ray_steps, img.data())) return 1;
+  const std::vector<float> scene(img.begin(), img.end());
+  const int cinematic = h->f.cinematic ? 1 : 0;
+  if (orc_present(scene.data(), width, height, (float)h->f.params.exposure, cinematic, out, nullptr)) return 1;
+  if (!cinematic) droplets_over_frame(h, width, height, out);
   return 0;
 }
 

@@ -85,6 +85,9 @@ def load() -> ctypes.CDLL:
         "orc_write_particle_buffers": (None, [vp, dp, dp]),
         "orc_render_volume": (i, [vp, i, i, i, dp]),
         "orc_detail_volume": (None, [ctypes.POINTER(ctypes.c_uint8)]),
+        "orc_render_frame": (i, [vp, i, i, i, ctypes.POINTER(ctypes.c_uint8)]),
+        "orc_present": (i, [ctypes.POINTER(ctypes.c_float), i, i, ctypes.c_float, i, ctypes.POINTER(ctypes.c_uint8),
+                            ctypes.POINTER(ctypes.c_float)]),
         "orc_set_threads": (None, [i]),
         "orc_max_threads": (i, []),
     }
@@ -102,6 +105,22 @@ def detail_volume() -> np.ndarray:
     out = np.empty((64, 64, 64, 4), dtype=np.uint8)
     load().orc_detail_volume(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
     return out
+
+
+def present(scene, exposure: float, cinematic: bool):
+    """The presentation passes over a (height, width, 4) float scene image:
+    returns (frame as (height, width, 4) uint8, bloom as (ceil(height/4),
+    ceil(width/4), 4) float32, or None for the standard presentation)."""
+    scene = np.ascontiguousarray(scene, dtype=np.float32)
+    h, w = scene.shape[:2]
+    frame = np.empty((h, w, 4), dtype=np.uint8)
+    bloom = np.zeros((max((h + 3) // 4, 1), max((w + 3) // 4, 1), 4), dtype=np.float32) if cinematic else None
+    fp = ctypes.POINTER(ctypes.c_float)
+    if load().orc_present(scene.ctypes.data_as(fp), w, h, float(exposure), int(bool(cinematic)),
+                          frame.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                          bloom.ctypes.data_as(fp) if bloom is not None else None):
+        raise ValueError("present needs a non-empty scene image")
+    return frame, bloom
 
 
 def max_threads() -> int:
@@ -212,6 +231,16 @@ class Oracle:
         if self.lib.orc_render_volume(self.h, int(width), int(height), int(ray_steps),
                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))):
             raise ValueError("render_volume needs the FP32 oracle and a positive size and step count")
+        return out
+
+    def render_frame(self, width: int, height: int, ray_steps: int) -> np.ndarray:
+        """A presented frame (the ray march, then bloom + filmic composite +
+        dither with the cinematic presentation, else the scene image) as
+        (height, width, 4) uint8.  FP32 oracle only."""
+        out = np.empty((height, width, 4), dtype=np.uint8)
+        if self.lib.orc_render_frame(self.h, int(width), int(height), int(ray_steps),
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))):
+            raise ValueError("render_frame needs the FP32 oracle and a positive size and step count")
         return out
 
     def write_particle_buffers(self, current, previous) -> None:
