@@ -120,7 +120,7 @@ def pack_profile(solver, cells: int, storage: str, reps: int = 10) -> dict:
             "algorithmic_GBps": round(bpc * cells * n / (ms * 1e-3) / 1e9, 1), "launches": n}
 
 
-def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict) -> dict:
+def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict, render: bool = True) -> dict:
     """The same workload with the hose on (waterFlow 0.8, spray nozzle,
     waterEnabled): the uhd tier's 4096 SPH droplets advance every step
     (density and integration each test all N^2 ordered pairs; deposition
@@ -137,13 +137,26 @@ def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict) -> dict:
     t = s.pass_timing()
     s.set_pass_timing(False)
     liquid = float(s.read_field("liquid", dtype="uint32").sum()) / 4096.0
+    frame = None
+    if render:
+        # a 4K presented frame with the droplet streaks drawn over the scene
+        w, h, rsteps = RENDER_4K
+        s.render_frame(w, h, rsteps, read=False)  # warm: buffers
+        s.set_pass_timing(True)
+        for _ in range(3):
+            s.render_frame(w, h, rsteps, read=False)
+        ft = s.pass_timing()
+        s.set_pass_timing(False)
+        frame = {k: round(ft[k]["ms"] / 3, 3) for k in ("pack", "render", "streaks", "present")}
+        frame["streak_launches"] = ft["streaks"]["launches"] // 3
+        frame["output"] = f"{w}x{h}"
     s.close()
     d = t["droplets"]
     us = d["ms"] / t["steps"] * 1e3
     return {"particles": n, "nozzle": "spray", "steps": steps, "ms_per_step_with_water": el / steps * 1e3,
             "droplet_passes_us_per_step": round(us, 2), "launches_per_step": d["launches"] / t["steps"],
             "pair_tests_per_s": 2.0 * n * n / (us * 1e-6) if us > 0 else None,
-            "deposited_liquid_mass": round(liquid, 3)}
+            "deposited_liquid_mass": round(liquid, 3), "presented_4k_frame_ms": frame}
 
 
 RENDER_4K = (3840, 2160, 384)  # the reference's 4K offline output and the uhd tier's ray budget
@@ -467,7 +480,7 @@ def main() -> int:
     droplets = None
     if args.droplet_steps > 0:
         print(f"bench: hose on, {args.droplet_steps} steps", file=sys.stderr, flush=True)
-        droplets = droplet_profile(wl, args.droplet_steps, 10, jacobi_plan)
+        droplets = droplet_profile(wl, args.droplet_steps, 10, jacobi_plan, render=args.render)
         droplets["step_time_over_dry"] = round(droplets["ms_per_step_with_water"] / ms_per_step, 4)
 
     serving = None

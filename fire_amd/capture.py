@@ -4,8 +4,8 @@ manifest.  verify_capture() re-checks a capture from disk with the library's
 host checksum (firex_checksum_bytes).
 
 Dump layout (little-endian): a 128-byte header — magic "FXFIELD1", header
-bytes, field id, storage kind (0 f32, 1 f16, 2 bf16, 3 u32), components,
-grid nx/ny/nz, records, stored bytes, frame, simulation time, checksum, field
+bytes, field id, storage kind (0 f32, 1 f16, 2 bf16, 3 u32, 4 u8), components,
+grid nx/ny/nz (a presented frame: width, height, 1), records, stored bytes, frame, simulation time, checksum, field
 name — then the field's stored bytes verbatim.
 """
 from __future__ import annotations
@@ -20,7 +20,7 @@ from .solver import checksum_bytes
 
 DUMP_MAGIC = b"FXFIELD1"
 _HEADER = struct.Struct("<8sIIII3iIQQQdQ32s16x")
-_KIND_DTYPES = {0: np.float32, 1: np.float16, 2: np.uint16, 3: np.uint32}  # bf16 kept as its bit pattern
+_KIND_DTYPES = {0: np.float32, 1: np.float16, 2: np.uint16, 3: np.uint32, 4: np.uint8}  # bf16 as its bit pattern
 
 
 def read_field_dump(path: str):

@@ -1068,7 +1068,7 @@ static int draw_streaks(firex_solver* s, const FxRender& r, void* target, int un
     s->sprite_work_size = need;
     s->bytes += need;
   }
-  timed(s, FIREX_PASS_RENDER, [&] {
+  timed(s, FIREX_PASS_STREAKS, [&] {
     launch_droplet_streaks(s->particles[s->particle_index], n, r, target, unorm8, s->sprite_work, s->stream);
   });
   return check_launch(s, "droplet streaks");

@@ -14,6 +14,11 @@
  *     and the FP64 residual, divergence and conservation sums (firex_reduce);
  *   * every `dumpEvery` frames (and the last): the stored bytes of the dumped
  *     fields, one `.fxf` file each (format below);
+ *   * with `--frame-every N`, every N frames (and the last): the presented
+ *     frame at the output resolution (firex_render_frame: ray march, droplet
+ *     streaks, bloom and tone-map composite), as an `.fxf` dump of field
+ *     "frame" (kind u8, 4 components, dims width x height x 1) plus a `.ppm`
+ *     copy of its rgb for viewing;
  *   * verification: every dump is read back from disk and its checksum
  *     recomputed on the host (firex_checksum_bytes) and compared with its
  *     header and with the device checksum of the same frame.
@@ -23,9 +28,9 @@
  *    0  char[8]  magic "FXFIELD1"
  *    8  u32      header bytes (128)
  *   12  u32      field id (FIREX_FIELD_*)
- *   16  u32      storage kind (FIREX_KIND_*: 0 f32, 1 f16, 2 bf16, 3 u32)
+ *   16  u32      storage kind (FIREX_KIND_*: 0 f32, 1 f16, 2 bf16, 3 u32, 4 u8)
  *   20  u32      components per record
- *   24  i32[3]   grid nx, ny, nz
+ *   24  i32[3]   grid nx, ny, nz (a frame: width, height, 1)
  *   36  u32      0
  *   40  u64      records (cells, or particles)
  *   48  u64      stored bytes
@@ -50,7 +55,13 @@ const DUMP_MAGIC = 'FXFIELD1';
 const DUMP_HEADER_BYTES = 128;
 const FIELD_IDS = {
   flow: 0, species: 1, pressure: 2, correction: 3, divergence: 4, reaction: 5, liquid: 6, flowPredicted: 7,
-  packedFlow: 8, packedChemistry: 9, packedMedia: 10, particles: 11, particleDensity: 12,
+  packedFlow: 8, packedChemistry: 9, packedMedia: 10, particles: 11, particleDensity: 12, frame: 14,
+};
+// Output presets and their aliases, scripts/record-fire-lib.mjs:32-34, 70-80.
+const OUTPUT_PRESETS = { hd: [1920, 1080], qhd: [2560, 1440], uhd: [3840, 2160] };
+const OUTPUT_ALIASES = {
+  hd: 'hd', 1080: 'hd', '1080p': 'hd', qhd: 'qhd', 1440: 'qhd', '1440p': 'qhd', uhd: 'uhd', 2160: 'uhd', '2160p': 'uhd',
+  '4k': 'uhd',
 };
 const AUDIT_FIELDS = ['flow', 'species', 'pressure', 'correction', 'divergence', 'reaction', 'liquid', 'particles'];
 const DUMP_FIELDS = ['flow', 'species', 'pressure', 'liquid'];
@@ -87,6 +98,10 @@ function usage() {
     '  --warmup SECONDS        warm-up before the capture (0)',
     '  --dump-every N          dump fields every N frames and at the last frame (0: last frame only)',
     '  --dump-fields LIST      comma list of fields to dump (flow,species,pressure,liquid)',
+    '  --frame-every N         write the presented frame every N frames and at the last (0: last only; default none)',
+    '  --resolution PRESET     hd | qhd | uhd (1080p, 1440p, 2160p, 4k): size of the written frames (hd)',
+    '  --camera-yaw R, --camera-pitch R, --camera-distance D   camera (the engine\'s defaults)',
+    '  --optical-detail N      0 | 256 | 512 | 1024: optical detail target of the beauty view (1024)',
     '  --audit-fields LIST     comma list of fields checksummed every frame',
     '  --plan reference|fused  kernel plan (fused)',
     '  --species-storage S     f32 | f16 | bf16 (f32)',
@@ -116,7 +131,7 @@ function parseArguments(argv) {
     scene: 'inferno', field: 'beauty', solverTier: 'hd', grid: null, exactDims: null, durationSeconds: 1, warmupSeconds: 0,
     dumpEvery: 0, dumpFields: [...DUMP_FIELDS], auditFields: [...AUDIT_FIELDS], plan: 'fused',
     speciesStorage: 'f32', particleCount: null, gridMemoryBudgetBytes: 0, output: null, outputDir: 'captures',
-    overwrite: false, help: false,
+    overwrite: false, help: false, frameEvery: null, outputPreset: 'hd', camera: {}, opticalDetailTarget: 1024,
   };
   for (let index = 0; index < argv.length; index += 1) {
     const raw = argv[index];
@@ -192,6 +207,30 @@ function parseArguments(argv) {
         options.gridMemoryBudgetBytes = n;
         break;
       }
+      case '--frame-every': {
+        const n = Number(value());
+        if (!Number.isInteger(n) || n < 0) throw new RangeError('--frame-every must be a non-negative integer.');
+        options.frameEvery = n;
+        break;
+      }
+      case '--resolution': case '--preset': {
+        const preset = OUTPUT_ALIASES[value().toLowerCase()];
+        if (!preset) throw new Error(`Unknown output resolution for ${flag}.`);
+        options.outputPreset = preset;
+        break;
+      }
+      case '--camera-yaw': case '--camera-pitch': case '--camera-distance': {
+        const v = Number(value());
+        if (!Number.isFinite(v)) throw new RangeError(`${flag} must be a finite number.`);
+        options.camera[{ '--camera-yaw': 'cameraYaw', '--camera-pitch': 'cameraPitch', '--camera-distance': 'cameraDistance' }[flag]] = v;
+        break;
+      }
+      case '--optical-detail': {
+        const n = Number(value());
+        if (![0, 256, 512, 1024].includes(n)) throw new Error('--optical-detail must be 0, 256, 512, or 1024.');
+        options.opticalDetailTarget = n;
+        break;
+      }
       case '--output': options.output = value(); break;
       case '--output-dir': options.outputDir = value(); break;
       case '--overwrite': options.overwrite = true; break;
@@ -230,6 +269,11 @@ function buildCaptureConfig(options, jobId, dimensions) {
     dumpEvery: options.dumpEvery,
     dumpFields: options.dumpFields,
     auditFields: options.auditFields,
+    frameEvery: options.frameEvery,
+    outputPreset: options.outputPreset,
+    resolution: OUTPUT_PRESETS[options.outputPreset].join('x'),
+    camera: options.camera,
+    opticalDetailTarget: options.opticalDetailTarget,
   };
 }
 
@@ -267,6 +311,18 @@ function decodeDumpHeader(b) {
     checksum: b.readBigUInt64LE(72).toString(16).padStart(16, '0'),
     field: name.slice(0, name.indexOf('\0') < 0 ? name.length : name.indexOf('\0')),
   };
+}
+
+// Binary PPM (P6) of an rgba8 frame's rgb, row 0 at the top.
+function encodePpm(rgba, width, height) {
+  const head = Buffer.from(`P6\n${width} ${height}\n255\n`, 'ascii');
+  const body = Buffer.alloc(width * height * 3);
+  for (let i = 0, j = 0; i < width * height; i += 1, j += 3) {
+    body[j] = rgba[4 * i];
+    body[j + 1] = rgba[4 * i + 1];
+    body[j + 2] = rgba[4 * i + 2];
+  }
+  return Buffer.concat([head, body]);
 }
 
 function readDump(filePath) {
@@ -320,6 +376,9 @@ async function runCapture(options, log = (line) => process.stdout.write(`${line}
   const solver = new FireXHipSolver(solverOptions);
   for (const [name, value] of Object.entries(SCENES[options.scene])) solver.setParameter(name, value);
   solver.setParameter('viewMode', FIELD_VIEWS[options.field]);
+  solver.setParameter('opticalDetailTarget', options.opticalDetailTarget);
+  for (const [name, value] of Object.entries(options.camera)) solver.setParameter(name, value);
+  const [frameWidth, frameHeight] = OUTPUT_PRESETS[options.outputPreset];
 
   log(`Capturing ${options.scene} / ${options.field} -> ${outputPath}`);
   log(`  Solver ${options.solverTier}, exact ${dimensions.join('x')} grid, ${expectedFrames} frames at ${FRAME_RATE} fps`);
@@ -341,6 +400,25 @@ async function runCapture(options, log = (line) => process.stdout.write(`${line}
       for (const name of options.auditFields) record.checksums[name] = solver.fieldChecksum(name);
       for (const name of AUDIT_REDUCTIONS) record.reductions[name] = solver.reduce(name);
       frames.push(record);
+      const frameNow = options.frameEvery !== null &&
+        (frame === expectedFrames || (options.frameEvery > 0 && frame % options.frameEvery === 0));
+      if (frameNow) {
+        const pixels = solver.renderFrame(frameWidth, frameHeight);
+        const layout = solver.fieldLayout('frame');
+        const checksum = solver.fieldChecksum('frame');
+        record.checksums.frame = checksum;
+        const stem = `frame-${String(frame).padStart(6, '0')}`;
+        const header = encodeDumpHeader({
+          fieldId: FIELD_IDS.frame, kindCode: layout.kindCode, components: layout.components,
+          dimensions: [frameWidth, frameHeight, 1], records: layout.records, storedBytes: layout.storedBytes, frame,
+          simulationTime: grid.simulationTime, checksum, field: 'frame',
+        });
+        fs.writeFileSync(path.join(partialPath, `${stem}-frame.fxf`),
+          Buffer.concat([header, Buffer.from(pixels.buffer, pixels.byteOffset, pixels.byteLength)]));
+        fs.writeFileSync(path.join(partialPath, `${stem}.ppm`), encodePpm(pixels, frameWidth, frameHeight));
+        dumps.push({ frame, field: 'frame', file: `${stem}-frame.fxf`, kind: layout.kind, components: layout.components,
+          records: layout.records, storedBytes: layout.storedBytes, checksum, image: `${stem}.ppm` });
+      }
       const dumpNow = frame === expectedFrames || (options.dumpEvery > 0 && frame % options.dumpEvery === 0);
       if (!dumpNow) continue;
       for (const name of options.dumpFields) {
@@ -432,5 +510,5 @@ if (require.main === module) {
 
 module.exports = {
   parseArguments, resolveOutputPath, buildCaptureConfig, encodeDumpHeader, decodeDumpHeader, readDump, runCapture,
-  SCENES, FIELD_VIEWS, FIELD_IDS, DUMP_HEADER_BYTES,
+  encodePpm, SCENES, FIELD_VIEWS, FIELD_IDS, DUMP_HEADER_BYTES, OUTPUT_PRESETS,
 };

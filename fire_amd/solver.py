@@ -143,8 +143,8 @@ class DeviceLimits(ctypes.Structure):
 
 
 PASSES = ("simulate", "divergence", "jacobi", "project", "project_divergence", "fill_copy", "pack", "droplets",
-          "render", "present")
-PASS_COUNT = 10  # FIREX_PASS_COUNT
+          "render", "present", "streaks")
+PASS_COUNT = 12  # FIREX_PASS_COUNT
 
 
 class PassTiming(ctypes.Structure):

@@ -349,7 +349,8 @@ enum {
   FIREX_PASS_DROPLETS = 7,            /* SPH density, integration, deposition     */
   FIREX_PASS_RENDER = 8,              /* firex_render_volume's ray march          */
   FIREX_PASS_PRESENT = 9,             /* firex_render_frame's bloom + composite   */
-  FIREX_PASS_COUNT = 10
+  FIREX_PASS_STREAKS = 10,            /* droplet streaks over the scene or frame  */
+  FIREX_PASS_COUNT = 12
 };
 typedef struct firex_pass_timing {
   double milliseconds[FIREX_PASS_COUNT];  /* summed device time per pass kind     */

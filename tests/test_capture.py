@@ -97,3 +97,35 @@ def test_driver_arguments():
         res = node("const r = require('./fire_amd/js/record_fire_hip.js');"
                    f"try {{ r.parseArguments({json.dumps(bad)}); }} catch (e) {{ console.log(e.message); }}")
         assert msg in res.stdout, (bad, res.stdout, res.stderr)
+
+
+@needs_node
+def test_driver_frame_arguments():
+    ok = node("const r = require('./fire_amd/js/record_fire_hip.js');"
+              "console.log(JSON.stringify(r.parseArguments(['--frame-every', '3', '--resolution', '4K', "
+              "'--camera-yaw', '0.5', '--camera-distance', '3.5', '--optical-detail', '256'])));")
+    assert ok.returncode == 0, ok.stderr
+    opts = json.loads(ok.stdout)
+    assert opts["frameEvery"] == 3 and opts["outputPreset"] == "uhd" and opts["opticalDetailTarget"] == 256
+    assert opts["camera"] == {"cameraYaw": 0.5, "cameraDistance": 3.5}
+    # frames are off unless asked for; the resolution aliases follow the reference recorder's
+    res = node("const r = require('./fire_amd/js/record_fire_hip.js');"
+               "console.log(JSON.stringify([r.parseArguments([]).frameEvery, "
+               "['1080p', '1440', 'qhd', '2160p'].map((v) => r.parseArguments(['--resolution', v]).outputPreset)]));")
+    assert json.loads(res.stdout) == [None, ["hd", "qhd", "qhd", "uhd"]]
+    for bad, msg in ((["--resolution", "8k"], "Unknown output resolution"), (["--frame-every", "-1"], "non-negative"),
+                     (["--optical-detail", "300"], "optical-detail must be"), (["--camera-yaw", "x"], "finite")):
+        res = node("const r = require('./fire_amd/js/record_fire_hip.js');"
+                   f"try {{ r.parseArguments({json.dumps(bad)}); }} catch (e) {{ console.log(e.message); }}")
+        assert msg in res.stdout, (bad, res.stdout, res.stderr)
+
+
+@needs_node
+def test_ppm_of_a_frame(tmp_path):
+    out = tmp_path / "f.ppm"
+    res = node("const r = require('./fire_amd/js/record_fire_hip.js'); const fs = require('fs');"
+               "const px = new Uint8Array([1, 2, 3, 255, 4, 5, 6, 255, 7, 8, 9, 255, 10, 11, 12, 255, 13, 14, 15, 255, "
+               "16, 17, 18, 255]);"
+               "fs.writeFileSync(process.argv[1], r.encodePpm(px, 3, 2));", str(out))
+    assert res.returncode == 0, res.stderr
+    assert out.read_bytes() == b"P6\n3 2\n255\n" + bytes(range(1, 19))

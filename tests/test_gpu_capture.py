@@ -114,3 +114,49 @@ def test_capture_driver_matches_oracle(tmp_path):
             assert header["data"] == _oracle_bytes(o, fid, kind), f"dump {d['file']}"
     assert o.read(orc.FIELD_LIQUID).sum() > 0
     o.close()
+
+
+@pytest.mark.skipif(NODE is None, reason="node is absent")
+def test_capture_frames_match_oracle(tmp_path):
+    """--frame-every: the presented hd frames (ray march, droplet streaks of
+    the spray, bloom and composite) written by the driver hold the FP32
+    oracle's frame bytes at the same frames; the .ppm holds their rgb."""
+    dims = (32, 48, 32)
+    out = tmp_path / "cap"
+    cmd = [NODE, os.path.join(ROOT, "fire_amd", "js", "record_fire_hip.js"), "--scene", "base-spray", "--solver", "hd",
+           "--dims", "x".join(map(str, dims)), "--duration", "0.1", "--frame-every", "3", "--particles", "640",
+           "--dump-fields", "liquid", "--camera-yaw", "0.4", "--output", str(out)]
+    print("capture:", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd=ROOT)
+    assert res.returncode == 0, res.stdout + res.stderr
+    manifest = json.loads((tmp_path / "cap.json").read_text())
+    assert manifest["config"]["resolution"] == "1920x1080" and manifest["config"]["frameEvery"] == 3
+    frame_dumps = {d["frame"]: d for d in manifest["dumps"] if d["field"] == "frame"}
+    assert sorted(frame_dumps) == [3, 6]
+    assert verify_capture(str(tmp_path / "cap.json"))["dumps"] == len(manifest["dumps"])
+
+    o = orc.Oracle(dims, precision=32, pressure_iterations=64, correction_iterations=64)
+    o.set_bool("cinematic", True)
+    o.set_heavy_mode(orc.HEAVY_OFF)
+    o.set_particle_count(640)
+    for k, v in BASE_SPRAY.items():
+        o.set_parameter(k, v)
+    # the preset's view values (scenes.ts:63-67), which the step above does not read
+    for k, v in (("viewZoom", 1.0), ("exposure", 1.0), ("frontDefinition", 1.25), ("cameraYaw", 0.4)):
+        o.set_parameter(k, v)
+    o.set_bool("waterEnabled", True)
+    o.reset()
+    for frame in range(1, 7):
+        o.step(DT, 1)
+        if frame not in frame_dumps:
+            continue
+        ref = o.render_frame(1920, 1080, 256)
+        header, arr = read_field_dump(str(out / frame_dumps[frame]["file"]))
+        assert header["dims"] == (1920, 1080, 1) and header["kind"] == 4
+        got = arr.reshape(1080, 1920, 4)
+        if not np.array_equal(got, ref):
+            bad = np.argwhere(got != ref)
+            raise AssertionError(f"frame {frame}: {len(bad)} bytes differ, first {bad[0].tolist()}")
+        ppm = (out / frame_dumps[frame]["image"]).read_bytes()
+        assert ppm[:15] == b"P6\n1920 1080\n25" and ppm[-1920 * 1080 * 3:] == ref[..., :3].tobytes()
+    o.close()
