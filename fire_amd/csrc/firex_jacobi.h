@@ -12,7 +12,7 @@ enum { JACOBI_KERNEL_AUTO = 0, JACOBI_KERNEL_SHIFT = 1, JACOBI_KERNEL_RING = 2 }
 
 struct JacobiPlan {
   int32_t sweeps_per_launch = 4;  // K: sweeps advanced per launch
-  int32_t rows = 4;               // R: y rows per thread
+  int32_t rows = 4;               // R: y rows per thread (RING: 4, 6 or 8)
   int32_t waves = 8;              // W: waves per block (stacked in y)
   int32_t chunk_z = 16;           // output planes per block
   int32_t kernel = JACOBI_KERNEL_SHIFT;  // SHIFT: any grid, K <= 8; RING: nx >= 64, ny >= W * R, K <= 5
@@ -20,9 +20,11 @@ struct JacobiPlan {
 };
 
 // kernel: JACOBI_KERNEL_* (AUTO picks RING where the grid allows it);
-// sweeps_per_launch and chunk_z: 0 = default.
+// sweeps_per_launch, chunk_z and tile_height (RING: 32, 48 or 64 rows per
+// tile): 0 = default; prefetch (RING, K <= 3): 0 = default (off), 1 = off,
+// 2 = on.
 JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z = 0,
-                            int32_t kernel = JACOBI_KERNEL_AUTO);
+                            int32_t kernel = JACOBI_KERNEL_AUTO, int32_t tile_height = 0, int32_t prefetch = 0);
 
 // One launch of the temporally blocked kernel: `sweeps` sweeps from
 // buffers[src] (or from the zero field, without reading it) into
@@ -46,12 +48,13 @@ bool jacobi_plan_valid(const JacobiPlan& plan);
 
 // Autotuner: times candidate plans for a grid of dims on the current device
 // and returns the fastest in *out (and its time per sweep in microseconds).
-// sweeps_per_launch / chunk_z > 0 and kernel != AUTO are kept fixed; the
-// rest is searched.  buffers (two pressure fields in the given storage) and
+// sweeps_per_launch / chunk_z / tile_height / prefetch > 0 and kernel !=
+// AUTO are kept fixed; the rest is searched.  buffers (two pressure fields in the given storage) and
 // div are scratch: their contents are overwritten.  Results are cached per
 // device, shape, storage and fixed parameters, so a second solver of the
 // same shape does not time again.  Returns a hipError_t (0 on success).
 int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
-                     void* const buffers[2], float* div, hipStream_t s, JacobiPlan* out, float* us_per_sweep);
+                     int32_t tile_height, int32_t prefetch, void* const buffers[2], float* div, hipStream_t s,
+                     JacobiPlan* out, float* us_per_sweep);
 
 }  // namespace fx

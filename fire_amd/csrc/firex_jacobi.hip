@@ -465,21 +465,28 @@ void launch_ring_rw(int k, const JacobiPlan& plan, const void* pin, void* pout, 
   }
 }
 
-// Ring variants: tile shape (rows per thread x waves per block, 64 x 32
-// cells either way) and prefetch distance.
+// Ring variants: tile height (8 waves of R = 4, 6 or 8 rows: 64 x 32, 64 x
+// 48 or 64 x 64 cells) and prefetch distance.  Taller tiles recompute fewer
+// halo rows (at K = 3, 26 of 32 rows are output, 58 of 64) but give fewer
+// blocks and hold more registers.
+template <int P, int R>
+void launch_ring_pf(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
+                    int zero_start, hipStream_t s) {
+  if (plan.prefetch)
+    launch_ring_rw<P, R, 8, 1>(k, plan, pin, pout, div, st, zero_start, s);
+  else
+    launch_ring_rw<P, R, 8, 0>(k, plan, pin, pout, div, st, zero_start, s);
+}
+
 template <int P>
 void launch_ring_k(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
                    int zero_start, hipStream_t s) {
-  if (plan.rows == 8) {
-    if (plan.prefetch)
-      launch_ring_rw<P, 8, 4, 1>(k, plan, pin, pout, div, st, zero_start, s);
-    else
-      launch_ring_rw<P, 8, 4, 0>(k, plan, pin, pout, div, st, zero_start, s);
-  } else if (plan.prefetch) {
-    launch_ring_rw<P, 4, 8, 1>(k, plan, pin, pout, div, st, zero_start, s);
-  } else {
-    launch_ring_rw<P, 4, 8, 0>(k, plan, pin, pout, div, st, zero_start, s);
-  }
+  if (plan.rows == 8)
+    launch_ring_pf<P, 8>(k, plan, pin, pout, div, st, zero_start, s);
+  else if (plan.rows == 6)
+    launch_ring_pf<P, 6>(k, plan, pin, pout, div, st, zero_start, s);
+  else
+    launch_ring_pf<P, 4>(k, plan, pin, pout, div, st, zero_start, s);
 }
 
 template <int P, int K>
@@ -517,7 +524,8 @@ void launch_tb_k(int k, const JacobiPlan& plan, const void* pin, void* pout, con
 bool jacobi_plan_valid(const JacobiPlan& plan) {
   const int max_k = plan.kernel == JACOBI_KERNEL_RING ? kRingMaxSweeps : 8;
   const bool shape = plan.kernel == JACOBI_KERNEL_RING
-      ? ((plan.rows == 4 && plan.waves == 8) || (plan.rows == 8 && plan.waves == 4)) && (plan.prefetch == 0 || plan.prefetch == 1)
+      ? (plan.rows == 4 || plan.rows == 6 || plan.rows == 8) && plan.waves == 8 &&
+            (plan.prefetch == 0 || plan.prefetch == 1)
       : plan.rows == 4 && plan.waves == 8;
   return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= max_k && shape && plan.chunk_z >= 1 &&
          (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);
@@ -531,8 +539,13 @@ int jacobi_plan_blocks(const JacobiPlan& p, const int32_t dims[3]) {
   return tiles * ((dims[2] + p.chunk_z - 1) / p.chunk_z);
 }
 
-JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel) {
+JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
+                            int32_t tile_height, int32_t prefetch) {
   JacobiPlan p;
+  // the ring kernel's tile height (8 waves of 4, 6 or 8 rows); the shift
+  // kernel keeps 32
+  const int rows = tile_height == 48 ? 6 : tile_height == 64 ? 8 : 4;
+  if (dims[1] >= p.waves * rows) p.rows = rows;
   // The ring kernel addresses fields through buffer resources with 32-bit
   // byte offsets: fields up to 2 GiB.
   const uint64_t field_bytes = (uint64_t)dims[0] * (uint64_t)dims[1] * (uint64_t)dims[2] * 4u;
@@ -540,11 +553,13 @@ JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, in
   p.kernel = (kernel == JACOBI_KERNEL_RING && ring_ok) ? JACOBI_KERNEL_RING
            : kernel == JACOBI_KERNEL_SHIFT ? JACOBI_KERNEL_SHIFT
            : ring_ok ? JACOBI_KERNEL_RING : JACOBI_KERNEL_SHIFT;
+  if (p.kernel != JACOBI_KERNEL_RING) p.rows = 4;
   // the ring kernel's mirrored z halo needs nz >= K + 2
   const int max_k = p.kernel == JACOBI_KERNEL_RING ? (dims[2] - 2 < kRingMaxSweeps ? dims[2] - 2 : kRingMaxSweeps) : 8;
   const int dflt = p.kernel == JACOBI_KERNEL_RING ? 3 : 4;
   const int k = sweeps_per_launch > 0 ? sweeps_per_launch : dflt;
   p.sweeps_per_launch = k > max_k ? max_k : k;
+  p.prefetch = p.kernel == JACOBI_KERNEL_RING && prefetch == 2 && p.sweeps_per_launch <= 3 ? 1 : 0;
   if (chunk_z > 0) {
     p.chunk_z = chunk_z;
     return p;
@@ -589,7 +604,8 @@ __global__ __launch_bounds__(256) void k_tune_pattern(float* __restrict__ div, u
   }
 }
 
-using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;  // device, dims, f16, K, chunk, kernel
+// device, dims, f16, K, chunk, kernel, tile height, prefetch
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
 struct TuneEntry {
   JacobiPlan plan;
   float us_per_sweep;
@@ -615,11 +631,12 @@ std::vector<int> chunk_candidates(int nz, long tiles) {
 }  // namespace
 
 int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
-                     void* const buffers[2], float* div, hipStream_t s, JacobiPlan* out, float* us_per_sweep) {
+                     int32_t tile_height, int32_t prefetch, void* const buffers[2], float* div, hipStream_t s,
+                     JacobiPlan* out, float* us_per_sweep) {
   int device = 0;
   hipError_t e = hipGetDevice(&device);
   if (e != hipSuccess) return e;
-  const TuneKey key{device, dims[0], dims[1], dims[2], f16, sweeps_per_launch, chunk_z, kernel};
+  const TuneKey key{device, dims[0], dims[1], dims[2], f16, sweeps_per_launch, chunk_z, kernel, tile_height, prefetch};
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = g_tune_cache.find(key);
@@ -632,7 +649,7 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
   // Candidates: the default plan first, then every kernel / depth / chunk /
   // prefetch combination the fixed parameters allow.
   std::vector<JacobiPlan> cands;
-  cands.push_back(make_jacobi_plan(dims, sweeps_per_launch, chunk_z, kernel));
+  cands.push_back(make_jacobi_plan(dims, sweeps_per_launch, chunk_z, kernel, tile_height, prefetch));
   const int kernels[2] = {JACOBI_KERNEL_RING, JACOBI_KERNEL_SHIFT};
   for (int kern : kernels) {
     if (kernel != JACOBI_KERNEL_AUTO && kernel != kern) continue;
@@ -643,17 +660,25 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
     const int kmax = kern == JACOBI_KERNEL_RING ? kRingMaxSweeps : 4;
     for (int K = sweeps_per_launch > 0 ? sweeps_per_launch : 2; K <= (sweeps_per_launch > 0 ? sweeps_per_launch : kmax);
          ++K) {
-      JacobiPlan base = make_jacobi_plan(dims, K, 1, kern);
-      if (base.sweeps_per_launch != K) continue;
-      base.chunk_z = 1;
-      const long tiles = jacobi_plan_blocks(base, dims) / dims[2];
-      std::vector<int> chunks = chunk_z > 0 ? std::vector<int>{chunk_z} : chunk_candidates(dims[2], tiles);
-      for (int c : chunks) {
-        for (int pf = 0; pf <= (kern == JACOBI_KERNEL_RING && K <= 3 ? 1 : 0); ++pf) {
-          JacobiPlan p = base;
-          p.chunk_z = c;
-          p.prefetch = pf;
-          if (jacobi_plan_valid(p)) cands.push_back(p);
+      // taller ring tiles are searched only when asked for: at UHD (192 x
+      // 288 x 192) they recompute fewer halo rows yet run slower (9.9, 11.1
+      // and 12.3 us per sweep for 32, 48 and 64 rows), as they hold more
+      // registers and give fewer blocks
+      for (int height : {32, 48, 64}) {
+        if (tile_height > 0 ? height != tile_height : height != 32) continue;
+        JacobiPlan base = make_jacobi_plan(dims, K, 1, kern, height);
+        if (base.sweeps_per_launch != K || base.rows * base.waves != height) continue;
+        base.chunk_z = 1;
+        const long tiles = jacobi_plan_blocks(base, dims) / dims[2];
+        std::vector<int> chunks = chunk_z > 0 ? std::vector<int>{chunk_z} : chunk_candidates(dims[2], tiles);
+        for (int c : chunks) {
+          for (int pf = 0; pf <= (kern == JACOBI_KERNEL_RING && K <= 3 ? 1 : 0); ++pf) {
+            if (prefetch > 0 && pf != prefetch - 1) continue;
+            JacobiPlan p = base;
+            p.chunk_z = c;
+            p.prefetch = pf;
+            if (jacobi_plan_valid(p)) cands.push_back(p);
+          }
         }
       }
     }
@@ -711,7 +736,7 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
         bool seen = false;
         for (const JacobiPlan& q : cands)
           seen = seen || (q.kernel == b.kernel && q.sweeps_per_launch == b.sweeps_per_launch &&
-                          q.prefetch == b.prefetch && q.chunk_z == c);
+                          q.prefetch == b.prefetch && q.rows == b.rows && q.chunk_z == c);
         if (seen) continue;
         JacobiPlan p = b;
         p.chunk_z = c;

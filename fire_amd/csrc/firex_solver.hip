@@ -608,6 +608,11 @@ int solver_plan(const firex_config* cfg, bool* heavy_out, bool* f16p_out, bool* 
   if (cfg->plan < 0 || cfg->plan > 1) return set_error(FIREX_ERR_INVALID, "unknown plan");
   if (cfg->particle_count < 0 || cfg->particle_count > (1 << 20))
     return set_error(FIREX_ERR_INVALID, "particle count must be in [0, 2^20]");
+  if (cfg->jacobi_tile_height != 0 && cfg->jacobi_tile_height != 32 && cfg->jacobi_tile_height != 48 &&
+      cfg->jacobi_tile_height != 64)
+    return set_error(FIREX_ERR_INVALID, "jacobi tile height must be 0, 32, 48 or 64");
+  if (cfg->jacobi_prefetch < 0 || cfg->jacobi_prefetch > 2)
+    return set_error(FIREX_ERR_INVALID, "jacobi prefetch must be 0 (tuned), 1 (off) or 2 (on)");
 
   bool heavy = cfg->heavy_mode == FIREX_HEAVY_AUTO ? heavy_grid(cfg->dims)
                                                      : (cfg->heavy_mode == FIREX_HEAVY_F16 || cfg->heavy_mode == FIREX_HEAVY_F32);
@@ -704,11 +709,12 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
       // fields, which initialize_fields() clears afterwards.
       void* bufs[2] = {s->pressure[0], s->pressure[1]};
       e = (hipError_t)tune_jacobi_plan(s->dims, f16p ? 1 : 0, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z,
-                                       cfg->jacobi_kernel, bufs, s->divergence, s->stream, &s->jacobi,
-                                       &s->jacobi_tuned_us);
+                                       cfg->jacobi_kernel, cfg->jacobi_tile_height, cfg->jacobi_prefetch, bufs,
+                                       s->divergence, s->stream, &s->jacobi, &s->jacobi_tuned_us);
       if (e != hipSuccess) rc = hip_fail(s, e, "Jacobi plan autotuning");
     } else {
-      s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z, cfg->jacobi_kernel);
+      s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z, cfg->jacobi_kernel,
+                                   cfg->jacobi_tile_height, cfg->jacobi_prefetch);
     }
     if (!rc) rc = initialize_fields(s);
   }
@@ -1248,6 +1254,7 @@ int firex_get_info(firex_solver* s, firex_info* info) {
   info->jacobi_chunk_z = s->jacobi.chunk_z;
   info->jacobi_prefetch = s->jacobi.prefetch;
   info->jacobi_tuned_us_per_sweep = s->jacobi_tuned_us;
+  info->jacobi_tile_height = s->jacobi.rows * s->jacobi.waves;
   info->particle_count = s->cfg.particle_count;
   return FIREX_OK;
 }

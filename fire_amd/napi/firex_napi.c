@@ -124,7 +124,7 @@ static int str_prop(napi_env env, napi_value obj, const char* key, char* buf, si
 /* create({tier, dims:[x,y,z], pressureIterations, correctionIterations,
  *         timeStep, maximumSubsteps, speciesStorage, heavyMode, plan,
  *         cinematic, device, memoryBudgetBytes, jacobiSweepsPerLaunch,
- *         captureGraphs, jacobiAutotune, particleCount}) */
+ *         captureGraphs, jacobiAutotune, jacobiTileHeight, particleCount}) */
 static napi_value js_create(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -160,6 +160,8 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     cfg.jacobi_sweeps_per_launch = (int32_t)num_or(env, argv[0], "jacobiSweepsPerLaunch", 0);
     cfg.capture_graphs = (int32_t)num_or(env, argv[0], "captureGraphs", 0);
     cfg.jacobi_autotune = (int32_t)num_or(env, argv[0], "jacobiAutotune", 0);
+    cfg.jacobi_tile_height = (int32_t)num_or(env, argv[0], "jacobiTileHeight", 0);
+    cfg.jacobi_prefetch = (int32_t)num_or(env, argv[0], "jacobiPrefetch", 0);
     cfg.particle_count = (int32_t)num_or(env, argv[0], "particleCount", cfg.particle_count);
     cfg.ray_steps = (int32_t)num_or(env, argv[0], "raySteps", cfg.ray_steps);
     if (str_prop(env, argv[0], "speciesStorage", buf, sizeof buf)) cfg.species_storage = lookup(buf, storage_names, 3);
@@ -602,6 +604,7 @@ static napi_value js_get_info(napi_env env, napi_callback_info info) {
   set_num(env, o, "deviceBytes", (double)fi.device_bytes);
   set_num(env, o, "jacobiSweepsPerLaunch", fi.jacobi_sweeps_per_launch);
   set_num(env, o, "jacobiChunkZ", fi.jacobi_chunk_z);
+  set_num(env, o, "jacobiTileHeight", fi.jacobi_tile_height);
   set_num(env, o, "particleCount", fi.particle_count);
   return o;
 }

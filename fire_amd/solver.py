@@ -92,6 +92,8 @@ class Config(ctypes.Structure):
         ("jacobi_autotune", ctypes.c_int32),
         ("particle_count", ctypes.c_int32),
         ("ray_steps", ctypes.c_int32),
+        ("jacobi_tile_height", ctypes.c_int32),
+        ("jacobi_prefetch", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
     ]
 
@@ -119,6 +121,7 @@ class Info(ctypes.Structure):
         ("jacobi_prefetch", ctypes.c_int32),
         ("jacobi_tuned_us_per_sweep", ctypes.c_float),
         ("particle_count", ctypes.c_int32),
+        ("jacobi_tile_height", ctypes.c_int32),
     ]
 
 
@@ -347,7 +350,8 @@ class FireXHipSolver:
                  cinematic: Optional[bool] = None, device: int = 0, memory_budget_bytes: int = 0,
                  jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0,
                  jacobi_kernel: str = "auto", jacobi_autotune: bool = False,
-                 particle_count: Optional[int] = None, ray_steps: Optional[int] = None):
+                 particle_count: Optional[int] = None, ray_steps: Optional[int] = None,
+                 jacobi_tile_height: int = 0, jacobi_prefetch: Optional[bool] = None):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -373,6 +377,8 @@ class FireXHipSolver:
         cfg.jacobi_chunk_z = int(jacobi_chunk_z)
         cfg.jacobi_kernel = {"auto": 0, "shift": 1, "ring": 2}[jacobi_kernel]
         cfg.jacobi_autotune = int(bool(jacobi_autotune))
+        cfg.jacobi_tile_height = int(jacobi_tile_height)
+        cfg.jacobi_prefetch = 0 if jacobi_prefetch is None else (2 if jacobi_prefetch else 1)
         if particle_count is not None:
             cfg.particle_count = int(particle_count)
         if ray_steps is not None:
@@ -561,7 +567,7 @@ class FireXHipSolver:
             "species_storage": inf.species_storage, "plan": inf.plan, "device_bytes": int(inf.device_bytes),
             "jacobi": {"kernel": {1: "shift", 2: "ring"}.get(inf.jacobi_kernel, inf.jacobi_kernel),
                        "sweeps_per_launch": inf.jacobi_sweeps_per_launch, "chunk_z": inf.jacobi_chunk_z,
-                       "prefetch": inf.jacobi_prefetch,
+                       "prefetch": inf.jacobi_prefetch, "tile_height": inf.jacobi_tile_height,
                        "tuned_us_per_sweep": round(float(inf.jacobi_tuned_us_per_sweep), 3)},
             "particle_count": inf.particle_count,
         }

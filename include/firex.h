@@ -126,6 +126,11 @@ typedef struct firex_config {
                                     tier, FireXEngine.ts:187-277); 0 = none         */
   int32_t ray_steps;             /* ray-march samples per pixel (raySteps by tier,
                                     80-384; firex_render_volume caps it at 384)    */
+  int32_t jacobi_tile_height;    /* FUSED plan, ring kernel: y rows per tile, 32, 48
+                                    or 64 (needs ny >= it); 0 = tuned default      */
+  int32_t jacobi_prefetch;       /* FUSED plan, ring kernel: 0 = tuned default, 1 =
+                                    off, 2 = load each plane one iteration early
+                                    (K <= 3)                                       */
   int32_t reserved;
 } firex_config;
 
@@ -154,6 +159,7 @@ typedef struct firex_info {
   int32_t jacobi_prefetch;
   float jacobi_tuned_us_per_sweep; /* autotuner's time per sweep; 0 = untuned */
   int32_t particle_count;       /* SPH droplets (firex_config.particle_count) */
+  int32_t jacobi_tile_height;   /* FUSED plan in use: y rows per tile          */
 } firex_info;
 
 typedef struct firex_memory_estimate {

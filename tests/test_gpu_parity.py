@@ -213,14 +213,33 @@ def test_captured_graph_replay_bitwise(plan):
     g.close()
 
 
-def _ring_pair(dims, iters, sweeps, heavy="off", chunk=0):
+def _ring_pair(dims, iters, sweeps, heavy="off", chunk=0, height=0, prefetch=None):
     o = orc.Oracle(dims, precision=32, pressure_iterations=iters[0], correction_iterations=iters[1])
     o.set_bool("cinematic", True)
     o.set_heavy_mode({"off": orc.HEAVY_OFF, "f16": orc.HEAVY_F16}[heavy])
     g = FireXHipSolver(dims, tier="hd", pressure_iterations=iters[0], correction_iterations=iters[1], plan="fused",
                        heavy_mode=heavy, jacobi_sweeps_per_launch=sweeps, jacobi_kernel="ring",
-                       jacobi_chunk_z=chunk)
+                       jacobi_chunk_z=chunk, jacobi_tile_height=height, jacobi_prefetch=prefetch)
     return g, o
+
+
+@pytest.mark.parametrize("height,dims,sweeps,chunk,prefetch", [
+    (48, (80, 100, 20), 3, 7, False), (48, (80, 100, 20), 2, 5, True), (48, (64, 48, 12), 5, 0, False),
+    (64, (64, 130, 17), 3, 6, False), (64, (64, 130, 17), 3, 6, True), (64, (128, 96, 40), 4, 9, False),
+    (64, (96, 64, 9), 2, 4, True), (32, (72, 40, 14), 3, 5, True), (64, (70, 200, 11), 5, 3, False)])
+def test_ring_kernel_tall_tiles_bitwise(height, dims, sweeps, chunk, prefetch):
+    """Ring tiles of 48 and 64 rows (8 waves of 6 or 8 rows), with and
+    without the one-plane prefetch, against the oracle bit for bit."""
+    g, o = _ring_pair(dims, (11, 9), sweeps, chunk=chunk, height=height, prefetch=prefetch)
+    info = g.info["jacobi"]
+    assert info["tile_height"] == height and info["prefetch"] == int(prefetch) and info["sweeps_per_launch"] == sweeps
+    o.step(DT, 2)
+    g.step(DT, 2)
+    assert_fields_equal(g, o, f"ring H={height} {dims} K={sweeps} pf={prefetch}",
+                        names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    np.testing.assert_array_equal(_bits(g.read_field("correction")), _bits(o.read(orc.FIELD_CORRECTION)))
+    g.close()
 
 
 @pytest.mark.parametrize("sweeps", [1, 2, 3, 4, 5, 6, 7, 8])
