@@ -415,6 +415,9 @@ FxRender make_render(const float u[FXU_COUNT], const int32_t dims[3], int32_t wi
   r.zoom = u[FXU_VIEW_ZOOM];
   r.canvas_height = u[FXU_CANVAS_H];
   r.nozzle = u[FXU_NOZZLE];
+  r.bnx = (dims[0] + kFxBrick - 1) / kFxBrick;
+  r.bny = (dims[1] + kFxBrick - 1) / kFxBrick;
+  r.bnz = (dims[2] + kFxBrick - 1) / kFxBrick;
   return r;
 }
 

@@ -57,7 +57,12 @@ void launch_checksum(const void* data, uint64_t bytes, uint64_t* partial, uint64
 // The volume ray marcher: packed = the three render volumes (4 x f16 per
 // cell, flow / chemistry / media), detail = 64^3 rgba8 texels, out = width x
 // height rgba16f pixels, row 0 at the top.
-void launch_render_volume(const void* packed, const uint32_t* detail, void* out, const FxRender& r, hipStream_t s);
+void launch_render_volume(const void* packed, const uint32_t* detail, const float* bricks, void* out,
+                          const FxRender& r, hipStream_t s);
+// Occupancy bricks of the media volume (bnx x bny x bnz floats): the largest
+// media component over each brick's texels and one texel beyond, read by
+// the beauty view's empty-space skipping.
+void launch_render_bricks(const void* packed, float* bricks, const FxRender& r, hipStream_t s);
 // The presentation after the ray march: with cinematic, the quarter-resolution
 // bloom (bloom: ceil(w/4) x ceil(h/4) rgba16f) and the tone-map composite;
 // without, the scene image stored as 8 bits.  frame: w x h rgba8 pixels.

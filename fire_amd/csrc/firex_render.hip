@@ -72,6 +72,7 @@ __device__ inline float4 texel(const Half4* __restrict__ vol, uint32_t i) {
 struct Site {
   uint32_t i000, dx, dy, dz;
   float fx, fy, fz;
+  uint32_t brick;  // the occupancy brick holding texel (x0, y0, z0)
 };
 
 __device__ inline Site sample_site(V3 position, const FxRender& r) {
@@ -93,7 +94,38 @@ __device__ inline Site sample_site(V3 position, const FxRender& r) {
   s.fx = tx - flx;
   s.fy = ty - fly;
   s.fz = tz - flz;
+  s.brick = ((uint32_t)(z0 / kFxBrick) * (uint32_t)r.bny + (uint32_t)(y0 / kFxBrick)) * (uint32_t)r.bnx +
+            (uint32_t)(x0 / kFxBrick);
   return s;
+}
+
+// Empty-space skipping by bricks.  bricks[b] is the largest of the four
+// media components over the texels of brick b plus one texel beyond its
+// upper faces, which holds all eight texels of any sample whose lower corner
+// (x0, y0, z0) lies in the brick.  Each f32 mix a (1 - t) + b t of values
+// <= M stays <= M (1 + u)^3 (u = 2^-24), so three levels of mixing stay
+// below M (1 + 6e-7): a brick whose maximum is below kFxEmptyBrick
+// guarantees an interpolated occupancy below 0.0005, which is when the
+// beauty view's loop skips the sample, and the loop then runs exactly as if
+// it had sampled.
+#define kFxEmptyBrick 0.00049995f
+
+__device__ inline float max4(float4 v) { return fxm_max(fxm_max(v.x, v.y), fxm_max(v.z, v.w)); }
+
+__global__ __launch_bounds__(64) void k_render_bricks(const Half4* __restrict__ media, int nx, int ny, int nz,
+                                                      int bnx, int bny, float* __restrict__ bricks) {
+  const int b = (int)blockIdx.x;
+  const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
+  const int x0 = bx * kFxBrick, y0 = by * kFxBrick, z0 = bz * kFxBrick;
+  const int ex = min(x0 + kFxBrick, nx - 1) - x0 + 1, ey = min(y0 + kFxBrick, ny - 1) - y0 + 1;
+  const int ez = min(z0 + kFxBrick, nz - 1) - z0 + 1;
+  float m = -INFINITY;
+  for (int t = (int)threadIdx.x; t < ex * ey * ez; t += 64) {
+    const int x = x0 + t % ex, y = y0 + (t / ex) % ey, z = z0 + t / (ex * ey);
+    m = fxm_max(m, max4(texel(media, (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z))));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fxm_max(m, __shfl_xor(m, o));
+  if (threadIdx.x == 0) bricks[b] = m;
 }
 
 // The eight texels mixed along x, then y, then z.
@@ -247,7 +279,8 @@ template <uint32_t VIEW>
 __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__ flow_vol,
                                                        const Half4* __restrict__ chem_vol,
                                                        const Half4* __restrict__ media_vol,
-                                                       const uint32_t* __restrict__ detail, Half4* __restrict__ out,
+                                                       const uint32_t* __restrict__ detail,
+                                                       const float* __restrict__ bricks, Half4* __restrict__ out,
                                                        FxRender r) {
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
   const int px = (int)blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
@@ -298,20 +331,24 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
       if (step >= step_count || distance >= far_t || transmittance < 0.006f) break;
       const V3 position = add(eye, mul(dir, distance));
       const Site site = sample_site(position, r);
-      const float4 raw_media = sample_volume(media_vol, site);
-      if (beauty && cinematic >= 0.5f) {
-        const float occupancy = fxm_max(fxm_max(raw_media.x, raw_media.y), fxm_max(raw_media.z, raw_media.w));
-        if (occupancy < 0.0005f) {
-          const float next_distance = distance + step_length;
-          if (next_distance < far_t) {
-            const float4 nm = sample_volume(media_vol, sample_site(add(eye, mul(dir, next_distance)), r));
-            const float next_occupancy = fxm_max(fxm_max(nm.x, nm.y), fxm_max(nm.z, nm.w));
-            distance += next_occupancy < 0.0005f ? step_length * 2.0f : step_length;
-          } else {
-            distance = far_t;
-          }
-          continue;
+      const bool skipping = beauty && cinematic >= 0.5f;
+      float4 raw_media = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      bool empty = skipping && bricks[site.brick] < kFxEmptyBrick;
+      if (!empty) {
+        raw_media = sample_volume(media_vol, site);
+        empty = skipping && max4(raw_media) < 0.0005f;
+      }
+      if (empty) {
+        const float next_distance = distance + step_length;
+        if (next_distance < far_t) {
+          const Site next = sample_site(add(eye, mul(dir, next_distance)), r);
+          bool next_empty = bricks[next.brick] < kFxEmptyBrick;
+          if (!next_empty) next_empty = max4(sample_volume(media_vol, next)) < 0.0005f;
+          distance += next_empty ? step_length * 2.0f : step_length;
+        } else {
+          distance = far_t;
         }
+        continue;
       }
       const float4 flow = sample_volume(flow_vol, site);
       float4 chem = sample_volume(chem_vol, site);
@@ -811,7 +848,14 @@ __global__ __launch_bounds__(256) void k_composite(const Half4* __restrict__ sce
 
 namespace fx {
 
-void launch_render_volume(const void* packed, const uint32_t* detail, void* out, const FxRender& r, hipStream_t s) {
+void launch_render_bricks(const void* packed, float* bricks, const FxRender& r, hipStream_t s) {
+  const uint64_t cells = (uint64_t)r.nx * r.ny * r.nz;
+  const Half4* media = (const Half4*)packed + 2 * cells;
+  k_render_bricks<<<(uint32_t)(r.bnx * r.bny * r.bnz), 64, 0, s>>>(media, r.nx, r.ny, r.nz, r.bnx, r.bny, bricks);
+}
+
+void launch_render_volume(const void* packed, const uint32_t* detail, const float* bricks, void* out,
+                          const FxRender& r, hipStream_t s) {
   const Half4* vol = (const Half4*)packed;
   const uint64_t cells = (uint64_t)r.nx * r.ny * r.nz;
   dim3 grid((uint32_t)((r.width + 15) / 16), (uint32_t)((r.height + 15) / 16));
@@ -819,14 +863,14 @@ void launch_render_volume(const void* packed, const uint32_t* detail, void* out,
   Half4* o = (Half4*)out;
   // u32(clamp(round(controls.x), 0.0, 7.0)), qualityShaders.ts:436
   switch ((uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f)) {
-    case 0: k_render_volume<kRuntimeView><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    case 1: k_render_volume<1><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    case 2: k_render_volume<2><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    case 3: k_render_volume<3><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    case 4: k_render_volume<4><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    case 5: k_render_volume<5><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    case 6: k_render_volume<6><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
-    default: k_render_volume<7><<<grid, 256, 0, s>>>(f, c, m, detail, o, r); break;
+    case 0: k_render_volume<kRuntimeView><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    case 1: k_render_volume<1><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    case 2: k_render_volume<2><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    case 3: k_render_volume<3><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    case 4: k_render_volume<4><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    case 5: k_render_volume<5><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    case 6: k_render_volume<6><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
+    default: k_render_volume<7><<<grid, 256, 0, s>>>(f, c, m, detail, bricks, o, r); break;
   }
 }
 

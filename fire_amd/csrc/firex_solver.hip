@@ -67,6 +67,7 @@ struct firex_solver {
   // render constants of the last firex_render_volume
   void* sprite_work = nullptr;
   uint64_t sprite_work_size = 0;
+  float* bricks = nullptr;  // occupancy bricks of the media volume
   FxRender last_render;
   std::vector<void*> allocations;
   hipStream_t stream = nullptr;
@@ -1125,7 +1126,18 @@ int firex_render_volume(firex_solver* s, const firex_render_config* rc) {
                  s->cfg.cinematic != 0, u, s->cfg.particle_count, rc->width, rc->height, steps);
   const FxRender r = make_render(u, s->dims, rc->width, rc->height);
   s->last_render = r;
-  timed(s, FIREX_PASS_RENDER, [&] { launch_render_volume(s->packed, s->detail, s->scene, r, s->stream); });
+  if (!s->bricks) {
+    const uint64_t bytes = (uint64_t)r.bnx * r.bny * r.bnz * 4;
+    rc1 = alloc(s, (void**)&s->bricks, bytes, "occupancy bricks");
+    if (rc1) return rc1;
+    s->bytes += bytes;
+  }
+  // the beauty view of the cinematic path skips empty space
+  const bool skipping = s->cfg.cinematic && (uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f) == 0u;
+  timed(s, FIREX_PASS_RENDER, [&] {
+    if (skipping) launch_render_bricks(s->packed, s->bricks, r, s->stream);
+    launch_render_volume(s->packed, s->detail, s->bricks, s->scene, r, s->stream);
+  });
   rc1 = check_launch(s, "volume render");
   if (rc1) return rc1;
   // the cinematic scene pass draws the droplets over the volume; the

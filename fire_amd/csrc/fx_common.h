@@ -102,7 +102,11 @@ struct FxRender {
   float zoom;                  // params.fire.w
   float canvas_height;         // params.render.y
   float nozzle;                // params.controls.z
+  int32_t bnx, bny, bnz;       // occupancy bricks per axis (kFxBrick texels each)
 };
+
+// Edge of an occupancy brick of the media volume, in texels.
+#define kFxBrick 8
 
 // Droplet streak of one droplet in NDC (dropletVertex): quad corner (a, b)
 // sits at (cx, cy) + normal * a * radius + tangent * b * half, normal =
