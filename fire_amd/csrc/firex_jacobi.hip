@@ -4,6 +4,8 @@
 // Jacobi sweep (solvePressureGlobal, src/sim/firex/shaders.ts:375-403):
 //   p'(c) = (wx (p[x-1] + p[x+1]) + wy (p[y-1] + p[y+1]) + wz (p[z-1] + p[z+1]) - div(c))
 //           * (1 / (2 (wx + wy + wz)))
+// with the weighted sum contracted to fma(wz, sz, fma(wy, sy, wx sx))
+// (jacobi_sum)
 // with clamped (Neumann) neighbours on the side walls and floor, zero
 // (Dirichlet) above the open top, and p' = 0 on the top row y = ny - 1.
 //
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_tb(const typename PIO<P>::Wor
         if (gy + 1 >= ny) up = 0.0f;
         const float zl = z_lo ? ctr : A[s - 1][r];
         const float zh = z_hi ? ctr : C[s - 1][r];
-        const float nb = wx * (left + right) + wy * (down + up) + wz * (zl + zh);
+        const float nb = jacobi_sum(wx, left + right, wy, down + up, wz, zl + zh);
         float v = (nb - D[s][r]) * inv;
         if (gy == ny - 1) v = 0.0f;
         res[r] = PIO<P>::round(v);
@@ -274,6 +276,7 @@ struct RingDims {
   static constexpr int M0 = 3 + PF;
   static constexpr int D = PF ? (K + 2 <= 4 ? 4 : K + 2 <= 6 ? 6 : 12) : (K + 1 <= 3 ? 3 : 6);
   static constexpr int U = PF ? 12 : 6;
+  static constexpr int NF = (2 * K + PF + U - 1) / U;  // unrolled blocks holding the fill iterations
   static_assert(K >= 1 && K + 1 + PF <= D, "sweeps per launch out of range");
   static_assert(U % 2 == 0 && U % 3 == 0 && U % M0 == 0 && U % D == 0, "ring sizes");
 };
@@ -338,6 +341,14 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
     y_out[r] = y >= ylo && y < yhi;
   }
   const bool x_out = gx >= xlo && gx < xhi;
+  // Store offsets: the lane's column where (x, y) is one of the tile's output
+  // cells, else 2 GiB, past the end of every field the ring kernel runs on
+  // (< 2 GiB), so the buffer store is discarded by the hardware range check
+  // (with or without the plane offset, which stays below 2 GiB, the sum
+  // neither wraps nor lands in range).  Stores then need no branches.
+  uint32_t vst[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) vst[r] = x_out && y_out[r] ? cx * PIO<P>::kBytes : 0x80000000u;
   const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.jacobi_inv;
   // wave-edge neighbours; wave 0's "down" and wave W-1's "up" are halo rows
   // or the zeroed top row, so any in-bounds slot will do
@@ -362,82 +373,105 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
 
   // Iteration t loads plane t and level s computes plane t - s - PF, so the
   // chunk [z0, z1) takes planes z0 - K .. z1 + K - 1 and (z1 - z0) + 2K + PF
-  // iterations.
+  // iterations.  Level s is needed on planes z0 - (K - s) .. z1 - 1 + (K - s)
+  // only, which it reaches at iteration 2s + PF: the fill iterations before
+  // that feed nothing and skip the level (their ring slots and edge rows hold
+  // values no needed plane reads).  The fill iterations lie in the first NF
+  // unrolled blocks, peeled so that the skips are resolved at compile time,
+  // and level K then computes exactly the chunk's own planes z0 .. z1 - 1.
+  // The blocks after them run branch-free (the DPP shifts of a level stay in
+  // the basic block of the add that consumes them and fold into it); the
+  // last, partial block checks the stream's end before each iteration.
   const int t0 = z0 - K;
   const int n_it = (z1 - z0) + 2 * K + PF;
-  for (int tb = 0;; tb += U) {
+  constexpr int NF = RingDims<K, PF>::NF;
+  auto iteration = [&](const int tb, const int ph, const bool fill) __attribute__((always_inline)) {
+    const int t = t0 + tb + ph;
+    const int par = ph & 1;  // U is even: the parity of the iteration
+    {
+      // Planes outside the grid load their mirror image.  (With PF = 1
+      // the last iteration's load is not used.  It stays unconditional: a
+      // load that may or may not have been issued makes the compiler's
+      // memory counter waits stall on the newest loads.)
+      const uint32_t pbase = (uint32_t)min(max(mirror(zr + zsign * t, nz), 0), nz - 1) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        P0[ph % M0][r] = PIO<P>::bload(rin, cx * PIO<P>::kBytes, (pbase + rowoff[r]) * PIO<P>::kBytes);
+        Dv[ph % D][r] = __builtin_bit_cast(
+            float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)(cx * 4u), (int)((pbase + rowoff[r]) * 4u), 0));
+      }
+    }
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+      if (fill && 2 * s + PF > tb + ph) continue;
+      const int z = t - s - PF;
+      // ring slots (plane - t0 mod ring size) of planes z - 1, z, z + 1 of
+      // level s - 1, and of z's div
+      const int m0 = s == 1 ? M0 : 3;
+      const int sa = ((ph - s - PF - 1) % m0 + m0) % m0;
+      const int sc = ((ph - s - PF) % m0 + m0) % m0;
+      const int sb = ((ph - s - PF + 1) % m0 + m0) % m0;
+      const int sd = ((ph - s - PF) % D + D) % D;
+      const int lv = s >= 2 ? s - 2 : 0;
+      const float* below = s == 1 ? P0[sa] : Pr[lv][sa];
+      const float* centre = s == 1 ? P0[sc] : Pr[lv][sc];
+      const float* above = s == 1 ? P0[sb] : Pr[lv][sb];
+      float res[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float ctr = centre[r];
+        const float left = shl_from_left(ctr);
+        const float right = shr_from_right(ctr);
+        const float down = r > 0 ? centre[r - 1] : edge[par][s - 1][wdn][1][lane];
+        const float up = r < R - 1 ? centre[r + 1] : edge[par][s - 1][wup][0][lane];
+        const float nb = jacobi_sum(wx, left + right, wy, down + up, wz, below[r] + above[r]);
+        res[r] = PIO<P>::round((nb - Dv[sd][r]) * inv);
+      }
+      if (fix_top) res[R - 1] = 0.0f;
+      if (s < K) {
+        // level s keeps planes in slots (plane - t0) % 3
+        const int qc = ((ph - s - PF) % 3 + 3) % 3;
+#pragma unroll
+        for (int r = 0; r < R; ++r) Pr[s - 1][qc][r] = res[r];
+      } else {
+        // z0 <= z < z1 here (see above)
+        const uint32_t pbase = (uint32_t)(zr + zsign * z) * plane * PIO<P>::kBytes;
+#pragma unroll
+        for (int r = 0; r < R; ++r) PIO<P>::bstore(rout, vst[r], pbase + rowoff[r] * PIO<P>::kBytes, res[r]);
+      }
+    }
+    // Publish rows 0 and R-1 of the plane each level's neighbours use next
+    // iteration (level q: plane t - q - PF), then swap parity.
+    {
+      const int q0 = ((ph - PF) % M0 + M0) % M0;
+      edge[par ^ 1][0][wave][0][lane] = P0[q0][0];
+      edge[par ^ 1][0][wave][1][lane] = P0[q0][R - 1];
+    }
+#pragma unroll
+    for (int s = 1; s < K; ++s) {
+      const int q = ((ph - s - PF) % 3 + 3) % 3;
+      edge[par ^ 1][s][wave][0][lane] = Pr[s - 1][q][0];
+      edge[par ^ 1][s][wave][1][lane] = Pr[s - 1][q][R - 1];
+    }
+    __syncthreads();
+  };
+  // Every exit below is block-uniform: all waves leave together.
+#pragma unroll
+  for (int fb = 0; fb < NF; ++fb)
 #pragma unroll
     for (int ph = 0; ph < U; ++ph) {
-      if (tb + ph >= n_it) return;  // block-uniform: every wave leaves here
-      const int t = t0 + tb + ph;
-      const int par = ph & 1;  // U is even: the parity of the iteration
-      {
-        // Planes outside the grid load their mirror image.  (With PF = 1
-        // the last iteration's load is not used.  It stays unconditional: a
-        // load that may or may not have been issued makes the compiler's
-        // memory counter waits stall on the newest loads.)
-        const uint32_t pbase = (uint32_t)min(max(mirror(zr + zsign * t, nz), 0), nz - 1) * plane;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          P0[ph % M0][r] = PIO<P>::bload(rin, cx * PIO<P>::kBytes, (pbase + rowoff[r]) * PIO<P>::kBytes);
-          Dv[ph % D][r] = __builtin_bit_cast(
-              float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)(cx * 4u), (int)((pbase + rowoff[r]) * 4u), 0));
-        }
-      }
-#pragma unroll
-      for (int s = 1; s <= K; ++s) {
-        const int z = t - s - PF;
-        // ring slots (plane - t0 mod ring size) of planes z - 1, z, z + 1 of
-        // level s - 1, and of z's div
-        const int m0 = s == 1 ? M0 : 3;
-        const int sa = ((ph - s - PF - 1) % m0 + m0) % m0;
-        const int sc = ((ph - s - PF) % m0 + m0) % m0;
-        const int sb = ((ph - s - PF + 1) % m0 + m0) % m0;
-        const int sd = ((ph - s - PF) % D + D) % D;
-        const int lv = s >= 2 ? s - 2 : 0;
-        const float* below = s == 1 ? P0[sa] : Pr[lv][sa];
-        const float* centre = s == 1 ? P0[sc] : Pr[lv][sc];
-        const float* above = s == 1 ? P0[sb] : Pr[lv][sb];
-        float res[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const float ctr = centre[r];
-          const float left = shl_from_left(ctr);
-          const float right = shr_from_right(ctr);
-          const float down = r > 0 ? centre[r - 1] : edge[par][s - 1][wdn][1][lane];
-          const float up = r < R - 1 ? centre[r + 1] : edge[par][s - 1][wup][0][lane];
-          const float nb = wx * (left + right) + wy * (down + up) + wz * (below[r] + above[r]);
-          res[r] = PIO<P>::round((nb - Dv[sd][r]) * inv);
-        }
-        if (fix_top) res[R - 1] = 0.0f;
-        if (s < K) {
-          // level s keeps planes in slots (plane - t0) % 3
-          const int qc = ((ph - s - PF) % 3 + 3) % 3;
-#pragma unroll
-          for (int r = 0; r < R; ++r) Pr[s - 1][qc][r] = res[r];
-        } else if (z >= z0 && z < z1) {
-          const uint32_t pbase = (uint32_t)(zr + zsign * z) * plane * PIO<P>::kBytes;
-#pragma unroll
-          for (int r = 0; r < R; ++r)
-            if (y_out[r] && x_out)
-              PIO<P>::bstore(rout, cx * PIO<P>::kBytes, pbase + rowoff[r] * PIO<P>::kBytes, res[r]);
-        }
-      }
-      // Publish rows 0 and R-1 of the plane each level's neighbours use next
-      // iteration (level q: plane t - q - PF), then swap parity.
-      {
-        const int q0 = ((ph - PF) % M0 + M0) % M0;
-        edge[par ^ 1][0][wave][0][lane] = P0[q0][0];
-        edge[par ^ 1][0][wave][1][lane] = P0[q0][R - 1];
-      }
-#pragma unroll
-      for (int s = 1; s < K; ++s) {
-        const int q = ((ph - s - PF) % 3 + 3) % 3;
-        edge[par ^ 1][s][wave][0][lane] = Pr[s - 1][q][0];
-        edge[par ^ 1][s][wave][1][lane] = Pr[s - 1][q][R - 1];
-      }
-      __syncthreads();
+      if (fb * U + ph >= n_it) return;
+      iteration(fb * U, ph, true);
     }
+  int tb = NF * U;
+  for (; tb + U <= n_it; tb += U) {
+#pragma unroll
+    for (int ph = 0; ph < U; ++ph) iteration(tb, ph, false);
+  }
+#pragma unroll
+  for (int ph = 0; ph < U; ++ph) {
+    if (tb + ph >= n_it) return;
+    iteration(tb, ph, false);
   }
 }
 
@@ -460,8 +494,15 @@ void launch_ring_rw(int k, const JacobiPlan& plan, const void* pin, void* pout, 
     case 1: launch_ring<P, 1, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
     case 2: launch_ring<P, 2, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
     case 3: launch_ring<P, 3, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
-    case 4: launch_ring<P, 4, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
-    default: launch_ring<P, 5, R, W, PF>(plan, pin, pout, div, st, zero_start, s); break;
+    default:
+      // the prefetch variant runs K <= 3 only (jacobi_plan_valid)
+      if constexpr (PF == 0) {
+        if (k == 4)
+          launch_ring<P, 4, R, W, PF>(plan, pin, pout, div, st, zero_start, s);
+        else
+          launch_ring<P, 5, R, W, PF>(plan, pin, pout, div, st, zero_start, s);
+      }
+      break;
   }
 }
 
@@ -525,7 +566,7 @@ bool jacobi_plan_valid(const JacobiPlan& plan) {
   const int max_k = plan.kernel == JACOBI_KERNEL_RING ? kRingMaxSweeps : 8;
   const bool shape = plan.kernel == JACOBI_KERNEL_RING
       ? (plan.rows == 4 || plan.rows == 6 || plan.rows == 8) && plan.waves == 8 &&
-            (plan.prefetch == 0 || plan.prefetch == 1)
+            (plan.prefetch == 0 || (plan.prefetch == 1 && plan.sweeps_per_launch <= 3))
       : plan.rows == 4 && plan.waves == 8;
   return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= max_k && shape && plan.chunk_z >= 1 &&
          (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);

@@ -519,7 +519,8 @@ __device__ inline float load_p(const typename PressureIO<P>::Word* p, const FxSt
 }
 
 // solvePressureGlobal, shaders.ts:375-403 (one Jacobi sweep).  The division
-// by 2 (wx + wy + wz) is a multiply by its f32 reciprocal.
+// by 2 (wx + wy + wz) is a multiply by its f32 reciprocal, and the weighted
+// neighbour sum is contracted: fma(wz, sz, fma(wy, sy, wx sx)).
 template <int P>
 __global__ __launch_bounds__(256) void k_jacobi(const typename PressureIO<P>::Word* __restrict__ pin,
                                                 typename PressureIO<P>::Word* __restrict__ pout,
@@ -531,9 +532,9 @@ __global__ __launch_bounds__(256) void k_jacobi(const typename PressureIO<P>::Wo
     PressureIO<P>::store(pout, id, 0.0f);
     return;
   }
-  float nb = st.wx * (load_p<P>(pin, st, x - 1, y, z) + load_p<P>(pin, st, x + 1, y, z))
-           + st.wy * (load_p<P>(pin, st, x, y - 1, z) + load_p<P>(pin, st, x, y + 1, z))
-           + st.wz * (load_p<P>(pin, st, x, y, z - 1) + load_p<P>(pin, st, x, y, z + 1));
+  const float nb = jacobi_sum(st.wx, load_p<P>(pin, st, x - 1, y, z) + load_p<P>(pin, st, x + 1, y, z),
+                              st.wy, load_p<P>(pin, st, x, y - 1, z) + load_p<P>(pin, st, x, y + 1, z),
+                              st.wz, load_p<P>(pin, st, x, y, z - 1) + load_p<P>(pin, st, x, y, z + 1));
   PressureIO<P>::store(pout, id, (nb - div[id]) * st.jacobi_inv);
 }
 

@@ -136,6 +136,14 @@ __device__ static inline float fx_keep_f32(float v) {
   asm volatile("" : "+v"(v));
   return v;
 }
+
+// Weighted neighbour sum of a Jacobi update, wx sx + wy sy + wz sz with each
+// multiply-add contracted into one fused multiply-add (v_fma_f32, rounded
+// once): fma(wz, sz, fma(wy, sy, wx sx)).  Every Jacobi kernel and the f32
+// oracle use this form.
+__device__ static inline float jacobi_sum(float wx, float sx, float wy, float sy, float wz, float sz) {
+  return __builtin_fmaf(wz, sz, __builtin_fmaf(wy, sy, wx * sx));
+}
 #endif
 
 // Floats of simulate's per-step coordinate tables for an nx x ny x nz grid

@@ -1451,7 +1451,10 @@ template <typename T> struct State {
 
   // solvePressureGlobal / solvePressureTiled / solvePressureF16Tiled (one sweep).
   // The f32 form multiplies by the f32 reciprocal of 2 (wx + wy + wz), one of
-  // the results WGSL's f32 division (2.5 ULP) admits; FP64 divides.
+  // the results WGSL's f32 division (2.5 ULP) admits, and contracts the
+  // weighted neighbour sum wx sx + wy sy + wz sz into fma(wz, sz, fma(wy, sy,
+  // wx sx)), the form a contracting shader compiler emits for it; FP64
+  // divides and rounds every operation.
   void jacobi_sweep(const std::vector<T>& pin, std::vector<T>& pout, bool f16Store) {
     T wx = mx * mx, wy = my * my, wz = mz * mz;
     T denom = T(2) * (wx + wy + wz);
@@ -1461,13 +1464,14 @@ template <typename T> struct State {
         for (int x = 0; x < nx; ++x) {
           size_t id = idx(x, y, z);
           if (y == ny - 1) { pout[id] = T(0); continue; }
-          T nb = wx * (load_p(pin, x - 1, y, z) + load_p(pin, x + 1, y, z))
-               + wy * (load_p(pin, x, y - 1, z) + load_p(pin, x, y + 1, z))
-               + wz * (load_p(pin, x, y, z - 1) + load_p(pin, x, y, z + 1));
+          T nb = jacobi_sum(wx, load_p(pin, x - 1, y, z) + load_p(pin, x + 1, y, z),
+                            wy, load_p(pin, x, y - 1, z) + load_p(pin, x, y + 1, z),
+                            wz, load_p(pin, x, y, z - 1) + load_p(pin, x, y, z + 1));
           T v = jacobi_divide(nb - divergence[id], denom);
           pout[id] = f16Store ? store_pressure_f16(v) : v;
         }
   }
+  T jacobi_sum(T wx, T sx, T wy, T sy, T wz, T sz) const;
   T jacobi_divide(T num, T denom) const;
   T store_pressure_f16(T v) const;
 
@@ -1595,6 +1599,13 @@ template <> float State<float>::jacobi_divide(float num, float denom) const {
   return num * (1.0f / denom);
 }
 template <> double State<double>::jacobi_divide(double num, double denom) const { return num / denom; }
+template <> float State<float>::jacobi_sum(float wx, float sx, float wy, float sy, float wz, float sz) const {
+  return std::fma(wz, sz, std::fma(wy, sy, wx * sx));
+}
+template <> double State<double>::jacobi_sum(double wx, double sx, double wy, double sy, double wz,
+                                             double sz) const {
+  return wx * sx + wy * sy + wz * sz;
+}
 template <> float State<float>::store_pressure_f16(float v) const { return round_to_f16(v); }
 template <> double State<double>::store_pressure_f16(double v) const { return v; }
 

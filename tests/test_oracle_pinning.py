@@ -293,6 +293,57 @@ def test_zero_is_a_fixed_point_and_f16_storage_rounds_like_numpy():
     np.testing.assert_array_equal(p16, p32.astype(np.float16).astype(np.float32))
 
 
+def _fma32(a, b, c):
+    """f32 fused multiply-add from float64: a*b of two f32 values is exact in
+    float64 (48 significant bits), the add rounds once to float64 and the cast
+    once more to f32.  The second rounding can only differ from a single one
+    when the float64 sum lands on an f32 rounding midpoint (about 2^-29 per
+    value); the seeded data below has no such case."""
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(np.float32)
+
+
+def test_jacobi_f32_sweep_is_the_contracted_form():
+    """One f32 sweep equals an independent numpy restatement of solvePressureGlobal
+    (shaders.ts:375-403): replicated walls and floor, zero above the open top, top
+    row forced to 0, the weighted sum contracted as fma(wz, sz, fma(wy, sy, wx sx)),
+    then (nb - div) times the f32 reciprocal of 2 (wx + wy + wz).  The uncontracted
+    sum and the other contraction order both differ from it, so either mistake fails."""
+    nx, ny, nz = 8, 10, 6
+    o = orc.Oracle((nx, ny, nz), precision=32)
+    rng = np.random.default_rng(11)
+    p = rng.normal(size=(nz, ny, nx)).astype(np.float32)
+    div = rng.normal(scale=50, size=(nz, ny, nx)).astype(np.float32)
+    o.write(orc.FIELD_DIVERGENCE, div.astype(np.float64))
+    o.write(orc.FIELD_PRESSURE, p.astype(np.float64))
+    o.kernel_jacobi(DT, f16=False)
+    got = o.read(orc.FIELD_PRESSURE).astype(np.float32)
+
+    f = np.float32
+    mx, my, mz = f(nx) / f(1.15), f(ny) / f(1.35), f(nz) / f(1.15)
+    wx, wy, wz = mx * mx, my * my, mz * mz
+    inv = f(1) / (f(2) * ((wx + wy) + wz))
+    pe = np.pad(p, ((1, 1), (1, 1), (1, 1)), mode="edge")  # clamped reads
+    pe[:, -1, :] = 0  # y = ny reads 0 (open top)
+    c = (slice(1, -1),) * 3
+    sx = pe[1:-1, 1:-1, :-2] + pe[1:-1, 1:-1, 2:]
+    sy = pe[1:-1, :-2, 1:-1] + pe[1:-1, 2:, 1:-1]
+    sz = pe[:-2, 1:-1, 1:-1] + pe[2:, 1:-1, 1:-1]
+    assert sx.dtype == np.float32 and pe[c].shape == p.shape
+
+    def finish(nb):
+        v = (nb - div) * inv
+        v[:, ny - 1, :] = 0
+        return v.astype(np.float32)
+
+    full = np.full_like(sx, 1)
+    want = finish(_fma32(wz * full, sz, _fma32(wy * full, sy, wx * sx)))
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    plain = finish(wx * sx + wy * sy + wz * sz)
+    other = finish(_fma32(wz * full, sz, _fma32(wx * full, sx, wy * sy)))
+    assert (plain.view(np.uint32) != want.view(np.uint32)).sum() > 0
+    assert (other.view(np.uint32) != want.view(np.uint32)).sum() > 0
+
+
 # ---------------------------------------------------------------- step graph
 def test_pass_order_and_ping_pong_indices():
     o = orc.Oracle((12, 16, 12), precision=32, pressure_iterations=5, correction_iterations=3)
