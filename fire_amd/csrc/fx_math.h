@@ -22,9 +22,13 @@
 // Results below 2^-125 flush to +0 so that no denormal is ever produced by
 // these functions (the device and the host then agree whatever their
 // denormal mode).  Every function uses only IEEE-754 single-precision add,
-// subtract, multiply, divide and sqrt, rounding to nearest, and integer/bit
-// operations; no fused multiply-add may be formed (build with
-// -ffp-contract=off on both sides).
+// subtract, multiply, divide, sqrt and fused multiply-add (fxm_fma: one
+// rounding, as v_fma_f32 on the device and the FMA instruction or fmaf on
+// the host), rounding to nearest, and integer/bit operations.  The fused
+// multiply-adds are written out explicitly -- the Horner steps of the
+// polynomials, the Cody-Waite reductions, mix and smoothstep, each the
+// contraction a shader compiler may form for that expression -- and no other
+// may be formed (build with -ffp-contract=off on both sides).
 //
 // min/max/clamp are written as compare-and-select so that their treatment
 // of signed zeros is identical on every target.
@@ -51,18 +55,22 @@ FXM_FN uint32_t fxm_float_to_bits(float f) {
   return u;
 }
 
+// a * b + c rounded once.
+FXM_FN float fxm_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 FXM_FN float fxm_min(float a, float b) { return (b < a) ? b : a; }
 FXM_FN float fxm_max(float a, float b) { return (a < b) ? b : a; }
 FXM_FN float fxm_clamp(float x, float lo, float hi) { return fxm_min(fxm_max(x, lo), hi); }
 
-// WGSL mix(e1, e2, e3) = e1 * (1 - e3) + e2 * e3.
-FXM_FN float fxm_mix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+// WGSL mix(e1, e2, e3) = e1 * (1 - e3) + e2 * e3, contracted as
+// fma(e2, e3, e1 * (1 - e3)).
+FXM_FN float fxm_mix(float a, float b, float t) { return fxm_fma(b, t, a * (1.0f - t)); }
 
 // WGSL smoothstep(low, high, x): t = clamp((x - low) / (high - low), 0, 1);
-// t * t * (3 - 2 t).
+// t * t * (3 - 2 t), with 3 - 2 t as fma(-2, t, 3).
 FXM_FN float fxm_smoothstep(float lo, float hi, float x) {
   float t = fxm_clamp((x - lo) / (hi - lo), 0.0f, 1.0f);
-  return t * t * (3.0f - 2.0f * t);
+  return t * t * fxm_fma(-2.0f, t, 3.0f);
 }
 
 FXM_FN float fxm_floor(float x) { return __builtin_floorf(x); }
@@ -89,9 +97,14 @@ FXM_FN float fxm_exp(float x) {
   const float kLn2Hi = 0.693115234375f;   // 0x1.62ep-1, 12 significant bits
   const float kLn2Lo = 3.19461833e-05f;   // ln 2 - kLn2Hi rounded to f32
   float k = fxm_rint(x * kLog2e);
-  float r = (x - k * kLn2Hi) - k * kLn2Lo;
-  float p = 1.0f + r * (1.0f + r * (0.5f + r * (1.66666672e-01f + r * (4.16666679e-02f
-            + r * (8.33333377e-03f + r * (1.38888892e-03f + r * 1.98412701e-04f))))));
+  float r = fxm_fma(-k, kLn2Lo, fxm_fma(-k, kLn2Hi, x));  // k kLn2Hi is exact
+  float p = fxm_fma(r, 1.98412701e-04f, 1.38888892e-03f);
+  p = fxm_fma(r, p, 8.33333377e-03f);
+  p = fxm_fma(r, p, 4.16666679e-02f);
+  p = fxm_fma(r, p, 1.66666672e-01f);
+  p = fxm_fma(r, p, 0.5f);
+  p = fxm_fma(r, p, 1.0f);
+  p = fxm_fma(r, p, 1.0f);
   return fxm_ldexp(p, k);
 }
 
@@ -103,9 +116,13 @@ FXM_FN float fxm_exp2(float x) {
   float k = fxm_rint(x);
   float r = x - k;  // exact
   // c_n = (ln 2)^n / n!
-  float p = 1.0f + r * (6.93147182e-01f + r * (2.40226507e-01f + r * (5.55041097e-02f
-            + r * (9.61812865e-03f + r * (1.33335579e-03f + r * (1.54035297e-04f
-            + r * 1.52527336e-05f))))));
+  float p = fxm_fma(r, 1.52527336e-05f, 1.54035297e-04f);
+  p = fxm_fma(r, p, 1.33335579e-03f);
+  p = fxm_fma(r, p, 9.61812865e-03f);
+  p = fxm_fma(r, p, 5.55041097e-02f);
+  p = fxm_fma(r, p, 2.40226507e-01f);
+  p = fxm_fma(r, p, 6.93147182e-01f);
+  p = fxm_fma(r, p, 1.0f);
   return fxm_ldexp(p, k);
 }
 
@@ -131,10 +148,13 @@ FXM_FN float fxm_log2(float x) {
   float f = m - 1.0f;          // exact
   float s = f / (2.0f + f);
   float z = s * s;
-  float poly = z * (3.33333343e-01f + z * (2.00000003e-01f + z * (1.42857149e-01f + z * 1.11111112e-01f)));
-  float t = s + s * poly;      // atanh(s)
+  float poly = fxm_fma(z, 1.11111112e-01f, 1.42857149e-01f);
+  poly = fxm_fma(z, poly, 2.00000003e-01f);
+  poly = fxm_fma(z, poly, 3.33333343e-01f);
+  poly = z * poly;
+  float t = fxm_fma(s, poly, s);  // atanh(s)
   float ln_m = t + t;
-  return (float)e + bias + ln_m * 1.44269502f;
+  return fxm_fma(ln_m, 1.44269502f, (float)e + bias);
 }
 
 // WGSL pow(x, y) = exp2(y * log2(x)).
@@ -147,7 +167,7 @@ FXM_FN float fxm_reduce_pio2(float x, int* quadrant) {
   const float kP2 = 4.83751297e-04f;        // 0x1.fb4p-12
   const float kP3 = 7.54978995e-08f;        // 0x1.4442d2p-24
   float k = fxm_rint(x * kTwoOverPi);
-  float r = ((x - k * kP1) - k * kP2) - k * kP3;
+  float r = fxm_fma(-k, kP3, fxm_fma(-k, kP2, fxm_fma(-k, kP1, x)));
   float q = k - 4.0f * fxm_floor(k * 0.25f);  // k mod 4 in {0,1,2,3}
   *quadrant = (int)q;
   return r;
@@ -155,12 +175,19 @@ FXM_FN float fxm_reduce_pio2(float x, int* quadrant) {
 
 FXM_FN float fxm_sin_poly(float r) {
   float z = r * r;
-  return r + r * z * (-1.66666672e-01f + z * (8.33333377e-03f + z * (-1.98412701e-04f + z * 2.75573188e-06f)));
+  float p = fxm_fma(z, 2.75573188e-06f, -1.98412701e-04f);
+  p = fxm_fma(z, p, 8.33333377e-03f);
+  p = fxm_fma(z, p, -1.66666672e-01f);
+  return fxm_fma(r * z, p, r);
 }
 
 FXM_FN float fxm_cos_poly(float r) {
   float z = r * r;
-  return 1.0f + z * (-0.5f + z * (4.16666679e-02f + z * (-1.38888892e-03f + z * (2.48015876e-05f + z * -2.75573188e-07f))));
+  float p = fxm_fma(z, -2.75573188e-07f, 2.48015876e-05f);
+  p = fxm_fma(z, p, -1.38888892e-03f);
+  p = fxm_fma(z, p, 4.16666679e-02f);
+  p = fxm_fma(z, p, -0.5f);
+  return fxm_fma(z, p, 1.0f);
 }
 
 FXM_FN float fxm_sin(float x) {

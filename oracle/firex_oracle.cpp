@@ -76,14 +76,18 @@ template <> struct Math<double> {
   static double floor(double x) { return std::floor(x); }
 };
 
-// WGSL min/max/clamp/mix/smoothstep, written as in fx_math.h.
+// WGSL min/max/clamp/mix/smoothstep, written as in fx_math.h: the f32 forms
+// contract mix to fma(b, t, a (1 - t)) and smoothstep's 3 - 2t to
+// fma(-2, t, 3) (fxm_fma); FP64 rounds every operation.
 template <typename T> inline T wmin(T a, T b) { return (b < a) ? b : a; }
 template <typename T> inline T wmax(T a, T b) { return (a < b) ? b : a; }
 template <typename T> inline T wclamp(T x, T lo, T hi) { return wmin(wmax(x, lo), hi); }
-template <typename T> inline T wmix(T a, T b, T t) { return a * (T(1) - t) + b * t; }
+inline float wfma(float a, float b, float c) { return fxm_fma(a, b, c); }
+inline double wfma(double a, double b, double c) { return a * b + c; }
+template <typename T> inline T wmix(T a, T b, T t) { return wfma(b, t, a * (T(1) - t)); }
 template <typename T> inline T wsmooth(T lo, T hi, T x) {
   T t = wclamp((x - lo) / (hi - lo), T(0), T(1));
-  return t * t * (T(3) - T(2) * t);
+  return t * t * wfma(T(-2), t, T(3));
 }
 
 template <typename T> struct V3 { T x, y, z; };
