@@ -1,6 +1,7 @@
-"""Full-size parity: the UHD mixed-precision workload (192x288x192) and the
-508^3 plume, stepped on the GPU with the autotuned fused plan and by the
-FP32 oracle on the host, compared bit for bit (+0 and -0 differ).
+"""Full-size parity: the CPU-runnable 64x96x64 configuration run in full, the
+plain FP32 128x192x128 configuration, the UHD mixed-precision workload
+(192x288x192) and the 508^3 plume, stepped on the GPU and by the FP32 oracle
+on the host, compared bit for bit (+0 and -0 differ).
 
 The FP32 oracle restates the reference step (src/sim/firex/shaders.ts,
 FireXEngine.ts:2115-2276); at these sizes the ring Jacobi kernel runs with
@@ -42,22 +43,51 @@ def _compare(g, o, label, names):
         assert bad == 0, f"{label}: field {name} differs in {bad} of {a.size} values"
 
 
-def _run_pair(dims, storage, heavy, emitters, noise, steps):
+def _run_pair(dims, storage, heavy, emitters, noise, steps, iters=96, plan="fused"):
     em = emitters
-    o = orc.Oracle(dims, precision=32, pressure_iterations=96, correction_iterations=96)
+    o = orc.Oracle(dims, precision=32, pressure_iterations=iters, correction_iterations=iters)
     o.set_bool("cinematic", True)
     o.set_heavy_mode({"off": orc.HEAVY_OFF, "auto": orc.HEAVY_AUTO}[heavy])
-    o.set_species_storage({"f16": orc.SPECIES_F16, "bf16": orc.SPECIES_BF16}[storage])
-    o.reset()
+    if storage != "f32":
+        o.set_species_storage({"f16": orc.SPECIES_F16, "bf16": orc.SPECIES_BF16}[storage])
+        o.reset()
     o.set_emitters(em, 0.1, noise, 2026)
-    g = FireXHipSolver(dims, tier="uhd", pressure_iterations=96, correction_iterations=96, plan="fused",
-                       species_storage=storage, heavy_mode=heavy, jacobi_autotune=True)
+    g = FireXHipSolver(dims, tier="uhd", pressure_iterations=iters, correction_iterations=iters, plan=plan,
+                       species_storage=storage, heavy_mode=heavy, jacobi_autotune=plan == "fused")
     g.set_emitters(em, 0.1, noise, 2026)
     print(f"{dims}: GPU plan {g.info['jacobi']}, heavy {g.info['heavy']}, f16 pressure {g.info['f16_pressure']}",
           flush=True)
     g.step(DT, steps)
     o.step(DT, steps)
     return g, o
+
+
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_cpu_plain_config_full_run_bitwise(plan):
+    """The CPU-runnable configuration in full: 64x96x64, f32 fields, one seeded
+    Gaussian emitter (sigma 4 cells) with +-10 % jitter, 24 + 24 sweeps,
+    dt 1/60, 60 steps."""
+    dims = (64, 96, 64)
+    em = seeded_emitters(dims, 1, radius_range=(4 * 2 ** 0.5, 4 * 2 ** 0.5), seed=2026)
+    g, o = _run_pair(dims, "f32", "off", em, 0.0, 60, iters=24, plan=plan)
+    _compare(g, o, f"64x96x64 {plan} after 60 steps",
+             {"flow", "species", "pressure", "correction", "divergence", "reaction"})
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_hd_plain_config_bitwise(plan):
+    """The plain FP32 GPU configuration: 128x192x128, f32 fields, one seeded
+    Gaussian emitter (sigma 8 cells), 96 + 96 sweeps, three steps, on the
+    reference-structured plan (one launch per reference pass) and the fused plan."""
+    dims = (128, 192, 128)
+    em = seeded_emitters(dims, 1, radius_range=(8 * 2 ** 0.5, 8 * 2 ** 0.5), seed=2026)
+    g, o = _run_pair(dims, "f32", "off", em, 0.0, 3, iters=96, plan=plan)
+    _compare(g, o, f"128x192x128 {plan} after 3 steps",
+             {"flow", "species", "pressure", "correction", "divergence", "reaction"})
+    g.close()
+    o.close()
 
 
 def test_uhd_mixed_workload_bitwise():
