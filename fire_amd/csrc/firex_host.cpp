@@ -193,8 +193,10 @@ FxStep make_step(const float u[FXU_COUNT], const int32_t dims[3], const EmitterC
   // shaders.ts:220-221, 253, 256, 270, 284, 288
   st.entrainment = fxm_clamp(u[FXU_ENTRAINMENT], 0.0f, 3.0f);
   st.ambient_oxygen = fxm_max(st.chem_y, 0.001f);
+  st.inv_ambient_oxygen = 1.0f / st.ambient_oxygen;
   st.heat_release = 2150.0f + 1450.0f * st.chem_z;
   st.max_dt = fxm_max(dt, 0.0001f);
+  st.inv_max_dt = 1.0f / st.max_dt;
   st.vortex_k = 0.00135f * st.chem_w * dt;
   st.turbulence_base = 0.075f + 0.16f * st.chem_w;
   st.tt31 = time * 3.1f;
@@ -217,6 +219,7 @@ FxStep make_step(const float u[FXU_COUNT], const int32_t dims[3], const EmitterC
   st.persistence = fxm_clamp(u[FXU_PERSISTENCE], 0.4f, 3.0f);
   st.ambient4 = fxm_pow(st.ambient, 4.0f);
   st.radiation_denominator = 1005.0f * st.persistence;
+  st.inv_radiation_denominator = 1.0f / st.radiation_denominator;
   st.decay_products = fxm_exp(-dt * 0.075f / st.persistence);
   st.decay_vapor = fxm_exp(-dt * 0.14f / st.persistence);
   // emitter, shaders.ts:101-133

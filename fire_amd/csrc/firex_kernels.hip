@@ -293,13 +293,13 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
   const float4 L = flowIn[cidx(st, x - 1, y, z)], R = flowIn[cidx(st, x + 1, y, z)];
   const float4 B = flowIn[cidx(st, x, y - 1, z)], Tp = flowIn[cidx(st, x, y + 1, z)];
   const float4 K = flowIn[cidx(st, x, y, z - 1)], F = flowIn[cidx(st, x, y, z + 1)];
-  float vax = (L.x + R.x + B.x + Tp.x + K.x + F.x) / 6.0f;
-  float vay = (L.y + R.y + B.y + Tp.y + K.y + F.y) / 6.0f;
-  float vaz = (L.z + R.z + B.z + Tp.z + K.z + F.z) / 6.0f;
+  float vax = (L.x + R.x + B.x + Tp.x + K.x + F.x) * (1.0f / 6.0f);
+  float vay = (L.y + R.y + B.y + Tp.y + K.y + F.y) * (1.0f / 6.0f);
+  float vaz = (L.z + R.z + B.z + Tp.z + K.z + F.z) * (1.0f / 6.0f);
   fl.x = fxm_mix(fl.x, vax, st.mix_velocity);
   fl.y = fxm_mix(fl.y, vay, st.mix_velocity);
   fl.z = fxm_mix(fl.z, vaz, st.mix_velocity);
-  float ta = (L.w + R.w + B.w + Tp.w + K.w + F.w) / 6.0f;
+  float ta = (L.w + R.w + B.w + Tp.w + K.w + F.w) * (1.0f / 6.0f);
   fl.w = fxm_mix(fl.w, ta, st.mix_temperature);
 
   {
@@ -309,10 +309,10 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     const float4 sT = SpeciesIO<S>::load(specIn, cidx(st, x, y + 1, z));
     const float4 sK = SpeciesIO<S>::load(specIn, cidx(st, x, y, z - 1));
     const float4 sF = SpeciesIO<S>::load(specIn, cidx(st, x, y, z + 1));
-    float4 sa = make_float4((sL.x + sR.x + sB.x + sT.x + sK.x + sF.x) / 6.0f,
-                            (sL.y + sR.y + sB.y + sT.y + sK.y + sF.y) / 6.0f,
-                            (sL.z + sR.z + sB.z + sT.z + sK.z + sF.z) / 6.0f,
-                            (sL.w + sR.w + sB.w + sT.w + sK.w + sF.w) / 6.0f);
+    float4 sa = make_float4((sL.x + sR.x + sB.x + sT.x + sK.x + sF.x) * (1.0f / 6.0f),
+                            (sL.y + sR.y + sB.y + sT.y + sK.y + sF.y) * (1.0f / 6.0f),
+                            (sL.z + sR.z + sB.z + sT.z + sK.z + sF.z) * (1.0f / 6.0f),
+                            (sL.w + sR.w + sB.w + sT.w + sK.w + sF.w) * (1.0f / 6.0f));
     sp = mix4(sp, sa, st.mix_species);
   }
 
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     sp.w = fxm_clamp(sp.w + evaporation * 2.15f, 0.0f, 3.0f);
   }
   {
-    float oxygenDeficit = fxm_clamp((st.ambient_oxygen - sp.y) / st.ambient_oxygen, 0.0f, 1.0f);
+    float oxygenDeficit = fxm_clamp((st.ambient_oxygen - sp.y) * st.inv_ambient_oxygen, 0.0f, 1.0f);
     float fuelCore = fxm_smoothstep(0.30f, 0.78f, sp.x);
     float interfaceWeight = oxygenDeficit * fxm_mix(1.0f, 0.18f, fuelCore);
     float speed = fxm_sqrt(fl.x * fl.x + fl.y * fl.y + fl.z * fl.z);
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     sp.z = fxm_clamp(sp.z + burn * (1.0f + richness * 1.8f), 0.0f, 3.0f);
     sp.w = fxm_clamp(sp.w + burn * 0.62f, 0.0f, 3.0f);
     fl.w += burn * st.heat_release;
-    react = fxm_clamp(burn / st.max_dt * 0.22f, 0.0f, 4.0f);
+    react = fxm_clamp(burn * st.inv_max_dt * 0.22f, 0.0f, 4.0f);
   }
   {
     float curlx = 0.5f * ((Tp.z - B.z) * st.my - (F.y - K.y) * st.mz);
@@ -395,7 +395,8 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     float gdy = 0.5f * ((Tp.w - B.w) * st.my) + 0.00001f;
     float gdz = 0.5f * ((F.w - K.w) * st.mz) + 0.00001f;
     float glen = fxm_sqrt(gdx * gdx + gdy * gdy + gdz * gdz);
-    float dirx = gdx / glen, diry = gdy / glen, dirz = gdz / glen;
+    const float ilen = 1.0f / glen;
+    float dirx = gdx * ilen, diry = gdy * ilen, dirz = gdz * ilen;
     fl.x = fl.x + (diry * curlz - dirz * curly) * st.vortex_k;
     fl.y = fl.y + (dirz * curlx - dirx * curlz) * st.vortex_k;
     fl.z = fl.z + (dirx * curly - diry * curlx) * st.vortex_k;
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     fl.z = fl.z + ltz * tk;
   }
   {
-    float buoyancy = fxm_clamp((fl.w - st.ambient) / 1500.0f, -0.1f, 1.2f);
+    float buoyancy = fxm_clamp((fl.w - st.ambient) * (1.0f / 1500.0f), -0.1f, 1.2f);
     fl.y += buoyancy * dt * 0.72f * st.buoyancy_clamp;
   }
   sp.z *= fxm_exp(-water * dt * 4.8f);
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
   fl.y = fl.y + st.spray_y * water * dt * 0.82f;
   fl.z = fl.z + st.spray_z * water * dt * 0.82f;
   {
-    float radiation = 5.67e-8f * (st.ambient4 - fxm_pow(fxm_max(fl.w, 0.0f), 4.0f)) / st.radiation_denominator;
+    float radiation = 5.67e-8f * (st.ambient4 - fxm_pow(fxm_max(fl.w, 0.0f), 4.0f)) * st.inv_radiation_denominator;
     fl.w = fxm_max(st.ambient, fl.w + dt * radiation);
     sp.z *= st.decay_products;
     sp.w *= st.decay_vapor;

@@ -45,8 +45,10 @@ struct FxStep {
   float swirl_tz, swirl_tx;    // time * 2.3, time * 1.9
   float lift_clamp;            // clamp(fire.x, 0, 4)
   float entrainment, ambient_oxygen;
+  float inv_ambient_oxygen;    // 1 / ambient_oxygen (f32)
   float heat_release;          // 2150 + 1450 * chemistry.z
   float max_dt;                // max(dt, 0.0001)
+  float inv_max_dt;            // 1 / max_dt (f32)
   float vortex_k;              // 0.00135 * chemistry.w * dt
   float turbulence_base;       // 0.075 + 0.16 * chemistry.w
   float tt31, tt53, tt41, tt27, tt47;  // time * {3.1, 5.3, 4.1, 2.7, 4.7}
@@ -55,6 +57,7 @@ struct FxStep {
   float persistence;           // clamp(fire.z, 0.4, 3)
   float ambient4;              // pow(water.w, 4)
   float radiation_denominator; // 1005 * persistence
+  float inv_radiation_denominator;  // 1 / radiation_denominator (f32)
   float decay_products, decay_vapor;   // exp(-dt * 0.075 / q), exp(-dt * 0.14 / q)
   // reference emitter (shaders.ts:101-133)
   float em_cx, em_cy, em_cz;

@@ -67,9 +67,12 @@ FXM_FN float fxm_clamp(float x, float lo, float hi) { return fxm_min(fxm_max(x, 
 FXM_FN float fxm_mix(float a, float b, float t) { return fxm_fma(b, t, a * (1.0f - t)); }
 
 // WGSL smoothstep(low, high, x): t = clamp((x - low) / (high - low), 0, 1);
-// t * t * (3 - 2 t), with 3 - 2 t as fma(-2, t, 3).
+// t * t * (3 - 2 t), with 3 - 2 t as fma(-2, t, 3).  The division is a
+// multiply by the f32 reciprocal of high - low (one of the results WGSL's
+// 2.5 ULP f32 division admits), which folds to a constant where the edges
+// are constants.
 FXM_FN float fxm_smoothstep(float lo, float hi, float x) {
-  float t = fxm_clamp((x - lo) / (hi - lo), 0.0f, 1.0f);
+  float t = fxm_clamp((x - lo) * (1.0f / (hi - lo)), 0.0f, 1.0f);
   return t * t * fxm_fma(-2.0f, t, 3.0f);
 }
 

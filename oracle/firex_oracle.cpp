@@ -85,8 +85,14 @@ template <typename T> inline T wclamp(T x, T lo, T hi) { return wmin(wmax(x, lo)
 inline float wfma(float a, float b, float c) { return fxm_fma(a, b, c); }
 inline double wfma(double a, double b, double c) { return a * b + c; }
 template <typename T> inline T wmix(T a, T b, T t) { return wfma(b, t, a * (T(1) - t)); }
+// x / c for a divisor c that is a constant, a uniform or shared by several
+// quotients: the f32 form multiplies by the f32 reciprocal of c (one of the
+// results WGSL's 2.5 ULP f32 division admits), as the kernels do; FP64
+// divides.
+inline float wdivr(float x, float c) { return x * (1.0f / c); }
+inline double wdivr(double x, double c) { return x / c; }
 template <typename T> inline T wsmooth(T lo, T hi, T x) {
-  T t = wclamp((x - lo) / (hi - lo), T(0), T(1));
+  T t = wclamp(wdivr(x - lo, hi - lo), T(0), T(1));
   return t * t * wfma(T(-2), t, T(3));
 }
 
@@ -402,21 +408,21 @@ template <typename T> struct State {
           const V4<T> L = flowIn[idx(x - 1, y, z)], R = flowIn[idx(x + 1, y, z)];
           const V4<T> B = flowIn[idx(x, y - 1, z)], Tp = flowIn[idx(x, y + 1, z)];
           const V4<T> K = flowIn[idx(x, y, z - 1)], F = flowIn[idx(x, y, z + 1)];
-          T vax = (L.x + R.x + B.x + Tp.x + K.x + F.x) / T(6);
-          T vay = (L.y + R.y + B.y + Tp.y + K.y + F.y) / T(6);
-          T vaz = (L.z + R.z + B.z + Tp.z + K.z + F.z) / T(6);
+          T vax = wdivr(L.x + R.x + B.x + Tp.x + K.x + F.x, T(6));
+          T vay = wdivr(L.y + R.y + B.y + Tp.y + K.y + F.y, T(6));
+          T vaz = wdivr(L.z + R.z + B.z + Tp.z + K.z + F.z, T(6));
           T mv = wclamp(dt * T(0.32), T(0), T(0.12));
           fl.x = wmix(fl.x, vax, mv); fl.y = wmix(fl.y, vay, mv); fl.z = wmix(fl.z, vaz, mv);
-          T ta = (L.w + R.w + B.w + Tp.w + K.w + F.w) / T(6);
+          T ta = wdivr(L.w + R.w + B.w + Tp.w + K.w + F.w, T(6));
           fl.w = wmix(fl.w, ta, wclamp(dt * T(0.18), T(0), T(0.08)));
 
           const V4<T> sL = specIn[idx(x - 1, y, z)], sR = specIn[idx(x + 1, y, z)];
           const V4<T> sB = specIn[idx(x, y - 1, z)], sT = specIn[idx(x, y + 1, z)];
           const V4<T> sK = specIn[idx(x, y, z - 1)], sF = specIn[idx(x, y, z + 1)];
-          V4<T> sa{(sL.x + sR.x + sB.x + sT.x + sK.x + sF.x) / T(6),
-                   (sL.y + sR.y + sB.y + sT.y + sK.y + sF.y) / T(6),
-                   (sL.z + sR.z + sB.z + sT.z + sK.z + sF.z) / T(6),
-                   (sL.w + sR.w + sB.w + sT.w + sK.w + sF.w) / T(6)};
+          V4<T> sa{wdivr(sL.x + sR.x + sB.x + sT.x + sK.x + sF.x, T(6)),
+                   wdivr(sL.y + sR.y + sB.y + sT.y + sK.y + sF.y, T(6)),
+                   wdivr(sL.z + sR.z + sB.z + sT.z + sK.z + sF.z, T(6)),
+                   wdivr(sL.w + sR.w + sB.w + sT.w + sK.w + sF.w, T(6))};
           sp = mix4(sp, sa, wclamp(dt * T(0.12), T(0), T(0.06)));
 
           T source = seeded ? emitter_seeded(pcx, pcy, pcz, id) : emitter_reference(npx, npy, npz);
@@ -457,7 +463,7 @@ template <typename T> struct State {
 
           T entrainment = wclamp(mixingX, T(0), T(3));
           T ambientOxygen = wmax(chemY, T(0.001));
-          T oxygenDeficit = wclamp((ambientOxygen - sp.y) / ambientOxygen, T(0), T(1));
+          T oxygenDeficit = wclamp(wdivr(ambientOxygen - sp.y, ambientOxygen), T(0), T(1));
           T fuelCore = wsmooth(T(0.30), T(0.78), sp.x);
           T interfaceWeight = oxygenDeficit * wmix(T(1), T(0.18), fuelCore);
           T speed = Math<T>::sqrt(fl.x * fl.x + fl.y * fl.y + fl.z * fl.z);
@@ -483,7 +489,7 @@ template <typename T> struct State {
           sp.z = wclamp(sp.z + burn * (T(1) + richness * T(1.8)), T(0), T(3));
           sp.w = wclamp(sp.w + burn * T(0.62), T(0), T(3));
           fl.w += burn * (T(2150) + T(1450) * chemZ);
-          T react = wclamp(burn / wmax(dt, T(0.0001)) * T(0.22), T(0), T(4));
+          T react = wclamp(wdivr(burn, wmax(dt, T(0.0001))) * T(0.22), T(0), T(4));
 
           T curlx = T(0.5) * ((Tp.z - B.z) * my - (F.y - K.y) * mz);
           T curly = T(0.5) * ((F.x - K.x) * mz - (R.z - L.z) * mx);
@@ -492,7 +498,7 @@ template <typename T> struct State {
           T gdy = T(0.5) * ((Tp.w - B.w) * my) + T(0.00001);
           T gdz = T(0.5) * ((F.w - K.w) * mz) + T(0.00001);
           T glen = Math<T>::sqrt(gdx * gdx + gdy * gdy + gdz * gdz);
-          T dirx = gdx / glen, diry = gdy / glen, dirz = gdz / glen;
+          T dirx = wdivr(gdx, glen), diry = wdivr(gdy, glen), dirz = wdivr(gdz, glen);
           T vk = T(0.00135) * chemW * dt;
           fl.x = fl.x + (diry * curlz - dirz * curly) * vk;
           fl.y = fl.y + (dirz * curlx - dirx * curlz) * vk;
@@ -509,7 +515,7 @@ template <typename T> struct State {
           fl.y = fl.y + lty * tk;
           fl.z = fl.z + ltz * tk;
 
-          T buoyancy = wclamp((fl.w - ambient) / T(1500), T(-0.1), T(1.2));
+          T buoyancy = wclamp(wdivr(fl.w - ambient, T(1500)), T(-0.1), T(1.2));
           fl.y += buoyancy * dt * T(0.72) * wclamp(fireY, T(0), T(4));
 
           sp.z *= Math<T>::exp(-water * dt * T(4.8));
@@ -523,8 +529,8 @@ template <typename T> struct State {
           fl.z = fl.z + smz * water * dt * T(0.82);
 
           T persistence = wclamp(fireZ, T(0.4), T(3));
-          T radiation = T(5.67e-8) * (Math<T>::pow(ambient, T(4)) - Math<T>::pow(wmax(fl.w, T(0)), T(4)))
-                        / (T(1005) * persistence);
+          T radiation = wdivr(T(5.67e-8) * (Math<T>::pow(ambient, T(4)) - Math<T>::pow(wmax(fl.w, T(0)), T(4))),
+                              T(1005) * persistence);
           fl.w = wmax(ambient, fl.w + dt * radiation);
           sp.z *= Math<T>::exp(-dt * T(0.075) / persistence);
           sp.w *= Math<T>::exp(-dt * T(0.14) / persistence);
