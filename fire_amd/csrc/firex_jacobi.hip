@@ -213,9 +213,10 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_tb(const typename PIO<P>::Wor
 // ------------------------------------------------------------- ring kernel
 // The same K-level pipeline, arranged to spend as few instructions as
 // possible per cell update:
-//   * every level runs every iteration and the z loop is unrolled by U, so
-//     each level's planes live in register rings addressed at compile time
-//     (level 0: 3 + PF planes, levels 1..K-1: 3 planes, div: D planes);
+//   * the z loop is unrolled by U, so each level's planes live in register
+//     rings addressed at compile time (level 0: 3 + PF planes, levels
+//     1..K-1: 3 planes, div: D planes), and the unrolled blocks between the
+//     pipeline's fill and its end have no branch at all;
 //   * with PF = 1, level s works on plane t - s - 1 while plane t is being
 //     loaded, so the loads of a plane have a whole iteration to arrive (the
 //     barrier that ends an iteration waits for LDS only); with PF = 0 it
