@@ -161,6 +161,76 @@ def test_reference_emitter_vertical_separability_and_bounds():
     assert min(vals) >= 0 and max(vals) <= 1.18 * 1.0 + 1e-12
 
 
+def _emitter_gaussian(X, Y, Z, W, H, D):
+    """exp(-(radial + vertical)) of emitter() (shaders.ts:101-117) at time 0,
+    Inferno tuning (f32 uniforms), FP64 shader constants."""
+    rb = np.sqrt(0.02024)
+    rx = min(rb * min(max(F32(3.1), 0.35), 3.2), W * 0.42)
+    rz = min(rb * min(max(F32(2.7), 0.35), 3.2), D * 0.42)
+    th = min(max(F32(1.35), 0.35), 2.5)
+    ox, oy, oz = X - W * 0.5, Y - 0.075, Z - (D * 0.5 + 0.011)
+    return np.exp(-(ox * ox / (rx * rx) + oz * oz / (rz * rz) + oy * oy / (0.0022 * th * th))), ox / rx, oz / rz
+
+
+def test_reference_emitter_horizontal_texture_on_nodal_lines():
+    """The horizontal factors of emitter() at time 0 (sin 0 = 0, cos 0 = 1):
+    cellular = 0.76 + 0.24 sin(49 X) sin(43 Z) and pulse = 0.84 + 0.16 sin(27 X + 21 Z)
+    (X, Z in metres).  On the pulse's nodal lines 27 X + 21 Z = m pi the pulse is
+    exactly 0.84, so emitter / (0.84 * Gaussian) must be the cellular closed form
+    along the line; where sin(49 X) = 0 as well it is exactly 0.76.  With the
+    cinematic flag, on the burner's centre line (offset.x = 0) the lobe argument
+    is 0 and the lobe factor is the closed form 0.48 + 0.70 * smoothstep(-0.42,
+    0.48, 0).  A wrong frequency, phase, amplitude or radius fails these."""
+    W, H, D = F32(1.15), F32(1.35), F32(1.15)
+    o = orc.Oracle((32, 48, 32), precision=64)
+    o.set_bool("cinematic", False)
+    rng = np.random.default_rng(12)
+    checked = 0
+    for m in range(3, 12):
+        for X in rng.uniform(0.25, 0.9, size=6):
+            Z = (m * np.pi - 27 * X) / 21
+            if not 0.25 < Z < 0.9:
+                continue
+            Y = rng.uniform(0.04, 0.11)
+            g, _, _ = _emitter_gaussian(X, Y, Z, W, H, D)
+            e = o.emitter(0.0, (X / W, Y / H, Z / D))
+            cellular = 0.76 + 0.24 * np.sin(49 * X) * np.sin(43 * Z)
+            assert e == pytest.approx(0.84 * cellular * g, rel=1e-9, abs=1e-300)
+            checked += 1
+    assert checked >= 10
+    for k in range(13, 15):  # sin(49 X) = 0 at X = k pi / 49 (0.83..0.90 m)
+        X = k * np.pi / 49
+        Z = (4 * np.pi - 27 * X) / 21  # on the pulse's nodal line m = 4
+        Z = Z if 0 < Z < D else (5 * np.pi - 27 * X) / 21
+        g, _, _ = _emitter_gaussian(X, 0.075, Z, W, H, D)
+        assert o.emitter(0.0, (X / W, 0.075 / H, Z / D)) == pytest.approx(0.76 * 0.84 * g, rel=1e-9)
+    o.set_bool("cinematic", True)
+    s = (0.42 / 0.90) ** 2 * (3 - 2 * (0.42 / 0.90))
+    lobes = 0.48 + (1.18 - 0.48) * s
+    for m in range(5, 9):
+        X = W * 0.5
+        Z = (m * np.pi - 27 * X) / 21
+        if not 0.2 < Z < 0.95:
+            continue
+        g, ax, _ = _emitter_gaussian(X, 0.08, Z, W, H, D)
+        assert abs(ax) < 1e-12
+        cellular = 0.76 + 0.24 * np.sin(49 * X) * np.sin(43 * Z)
+        assert o.emitter(0.0, (X / W, 0.08 / H, Z / D)) == pytest.approx(0.84 * cellular * lobes * g, rel=1e-9)
+    # Anywhere, the cinematic / plain ratio cancels the Gaussian, cellular and
+    # pulse factors and leaves the lobe factor alone.
+    plain = orc.Oracle((32, 48, 32), precision=64)
+    plain.set_bool("cinematic", False)
+    for _ in range(40):
+        X, Y, Z = rng.uniform(0.3, 0.85), rng.uniform(0.04, 0.11), rng.uniform(0.3, 0.85)
+        _, ax, az = _emitter_gaussian(X, Y, Z, W, H, D)
+        a = np.sin(ax * 6.2) * np.sin(az * 5.4)
+        t = min(max((a + 0.42) / 0.90, 0.0), 1.0)
+        want = 0.48 + (1.18 - 0.48) * t * t * (3 - 2 * t)
+        pos = (X / W, Y / H, Z / D)
+        assert o.emitter(0.0, pos) / plain.emitter(0.0, pos) == pytest.approx(want, rel=1e-9)
+    plain.close()
+
+
 def test_clamps_and_boundary_values_after_simulate():
     dims = (12, 16, 10)
     o = orc.Oracle(dims, precision=64)
