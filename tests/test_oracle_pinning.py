@@ -228,6 +228,23 @@ def test_reference_emitter_horizontal_texture_on_nodal_lines():
         want = 0.48 + (1.18 - 0.48) * t * t * (3 - 2 * t)
         pos = (X / W, Y / H, Z / D)
         assert o.emitter(0.0, pos) / plain.emitter(0.0, pos) == pytest.approx(want, rel=1e-9)
+    # After one step the emitter reads time t = dt: the burner centre moves by
+    # (0.013 sin 1.73t, 0, 0.011 cos 1.37t) and the lobes by +1.7t and -1.3t.
+    o.step(DT, 1)
+    plain.step(DT, 1)
+    t = F32(DT)
+    rb = np.sqrt(0.02024)
+    rx = min(rb * F32(3.1), W * 0.42)
+    rz = min(rb * F32(2.7), D * 0.42)
+    for _ in range(40):
+        X, Y, Z = rng.uniform(0.3, 0.85), rng.uniform(0.04, 0.11), rng.uniform(0.3, 0.85)
+        ax = (X - (W * 0.5 + np.sin(t * 1.73) * 0.013)) / rx
+        az = (Z - (D * 0.5 + np.cos(t * 1.37) * 0.011)) / rz
+        a = np.sin(ax * 6.2 + t * 1.7) * np.sin(az * 5.4 - t * 1.3)
+        u = min(max((a + 0.42) / 0.90, 0.0), 1.0)
+        want = 0.48 + (1.18 - 0.48) * u * u * (3 - 2 * u)
+        pos = (X / W, Y / H, Z / D)
+        assert o.emitter(0.0, pos) / plain.emitter(0.0, pos) == pytest.approx(want, rel=1e-9)
     plain.close()
 
 
