@@ -161,14 +161,15 @@ def test_reference_emitter_vertical_separability_and_bounds():
     assert min(vals) >= 0 and max(vals) <= 1.18 * 1.0 + 1e-12
 
 
-def _emitter_gaussian(X, Y, Z, W, H, D):
-    """exp(-(radial + vertical)) of emitter() (shaders.ts:101-117) at time 0,
+def _emitter_gaussian(X, Y, Z, W, H, D, t=0.0):
+    """exp(-(radial + vertical)) of emitter() (shaders.ts:101-117) at time t,
     Inferno tuning (f32 uniforms), FP64 shader constants."""
     rb = np.sqrt(0.02024)
     rx = min(rb * min(max(F32(3.1), 0.35), 3.2), W * 0.42)
     rz = min(rb * min(max(F32(2.7), 0.35), 3.2), D * 0.42)
     th = min(max(F32(1.35), 0.35), 2.5)
-    ox, oy, oz = X - W * 0.5, Y - 0.075, Z - (D * 0.5 + 0.011)
+    ox = X - (W * 0.5 + np.sin(t * 1.73) * 0.013)
+    oy, oz = Y - 0.075, Z - (D * 0.5 + np.cos(t * 1.37) * 0.011)
     return np.exp(-(ox * ox / (rx * rx) + oz * oz / (rz * rz) + oy * oy / (0.0022 * th * th))), ox / rx, oz / rz
 
 
@@ -245,6 +246,16 @@ def test_reference_emitter_horizontal_texture_on_nodal_lines():
         want = 0.48 + (1.18 - 0.48) * u * u * (3 - 2 * u)
         pos = (X / W, Y / H, Z / D)
         assert o.emitter(0.0, pos) / plain.emitter(0.0, pos) == pytest.approx(want, rel=1e-9)
+    # ... and on the pulse's nodal lines 3.7t + 27X + 21Z = m pi the plain
+    # emitter is 0.84 * cellular(t) * Gaussian(t).
+    for m in range(4, 10):
+        X = rng.uniform(0.3, 0.85)
+        Z = (m * np.pi - 27 * X - 3.7 * t) / 21
+        if not 0.25 < Z < 0.9:
+            continue
+        g, _, _ = _emitter_gaussian(X, 0.07, Z, W, H, D, t)
+        cellular = 0.76 + 0.24 * np.sin(49 * X + 5.2 * t) * np.sin(43 * Z - 4.4 * t)
+        assert plain.emitter(0.0, (X / W, 0.07 / H, Z / D)) == pytest.approx(0.84 * cellular * g, rel=1e-9)
     plain.close()
 
 
