@@ -1776,6 +1776,12 @@ double orc_get_parameter(void* p, const char* name) {
   return NAN;
 }
 
+// Simulation clock as the uniform block sees it (time.y, time.z): lets a
+// test evaluate the step at a chosen time and frame.
+void orc_set_time(void* p, double simulationTime, double frameIndex) {
+  Handle* h = static_cast<Handle*>(p);
+  ORC_DISPATCH(h, (s.simulationTime = simulationTime, s.frameIndex = frameIndex, 0));
+}
 void orc_set_time_step(void* p, double ts) { Handle* h = static_cast<Handle*>(p); h->f.timeStep = ts; h->d.timeStep = ts; }
 void orc_set_species_storage(void* p, int mode) { static_cast<Handle*>(p)->f.speciesStorage = mode; }
 void orc_set_heavy_mode(void* p, int mode) { Handle* h = static_cast<Handle*>(p); h->f.heavyMode = mode; h->d.heavyMode = mode; }

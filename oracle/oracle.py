@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_firex.so")
+# FIREX_ORACLE_LIB points the tests at another build of the oracle (used to
+# check that planted mistakes in the restatement fail the pinning tests).
+LIB_PATH = os.environ.get("FIREX_ORACLE_LIB") or os.path.join(_HERE, "liboracle_firex.so")
 
 FIELD_FLOW, FIELD_SPECIES, FIELD_PRESSURE, FIELD_CORRECTION = 0, 1, 2, 3
 FIELD_DIVERGENCE, FIELD_REACTION, FIELD_LIQUID, FIELD_FLOW_PREDICTED = 4, 5, 6, 7
@@ -47,6 +49,7 @@ def load() -> ctypes.CDLL:
         "orc_set_bool": (i, [vp, ctypes.c_char_p, i]),
         "orc_get_parameter": (d, [vp, ctypes.c_char_p]),
         "orc_set_time_step": (None, [vp, d]),
+        "orc_set_time": (None, [vp, d, d]),
         "orc_set_species_storage": (None, [vp, i]),
         "orc_set_heavy_mode": (None, [vp, i]),
         "orc_set_emitters": (i, [vp, i, fp, f, f, u64]),
@@ -183,6 +186,10 @@ class Oracle:
 
     def get_parameter(self, name: str) -> float:
         return self.lib.orc_get_parameter(self.h, name.encode())
+
+    def set_time(self, simulation_time: float, frame_index: int = 0) -> None:
+        """Set the clock the next step's uniforms carry (time.y, time.z)."""
+        self.lib.orc_set_time(self.h, float(simulation_time), float(frame_index))
 
     def set_species_storage(self, mode: int) -> None:
         self.lib.orc_set_species_storage(self.h, mode)
