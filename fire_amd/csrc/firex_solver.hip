@@ -76,6 +76,11 @@ struct firex_solver {
   JacobiPlan jacobi;
   float jacobi_tuned_us = 0;  // autotuner's time per sweep (0: not tuned)
   int corr_index = 0;      // buffer holding the last residual solution
+  // The divergence field holds the residual solve's right-hand side (the
+  // divergence of the primary-projected flow, FireXEngine.ts:2257) until the
+  // divergence view's refresh in firex_pack_fields replaces it with the final
+  // flow's divergence; the residual norm is defined only while this holds.
+  bool divergence_is_rhs = true;
   // HIP-event timing of launches (firex_set_pass_timing)
   struct TimedLaunch {
     int pass;
@@ -201,6 +206,7 @@ int initialize_fields(firex_solver* s) {
   s->liquid_may_contain_water = false;
   s->flow_a = 0;
   s->corr_index = 0;
+  s->divergence_is_rhs = true;  // zero correction against a zero divergence
   launch_initialize(s->cfg.species_storage, s->flow[0], s->species[0], s->cells, 300.0f, (float)s->params.oxygenRate,
                     s->stream);
   return check_launch(s, "field initialization");
@@ -384,7 +390,10 @@ bool launch_rest_graph(firex_solver* s, const FxStep& st) {
   }
   for (const auto& g : s->graphs) {
     if (g.flow_a == s->flow_a && g.pressure_index == s->pressure_index) {
-      if (hipGraphLaunch(g.exec, s->stream) != hipSuccess) return false;
+      if (hipGraphLaunch(g.exec, s->stream) != hipSuccess) {
+        (void)hipGetLastError();  // state untouched: the caller runs the step eagerly
+        return false;
+      }
       s->flow_a = g.post_flow_a;
       s->pressure_index = g.post_pressure_index;
       s->corr_index = g.post_corr_index;
@@ -415,7 +424,16 @@ bool launch_rest_graph(firex_solver* s, const FxStep& st) {
   g.post_pressure_index = s->pressure_index;
   g.post_corr_index = s->corr_index;
   s->graphs.push_back(g);
-  return hipGraphLaunch(g.exec, s->stream) == hipSuccess;
+  if (hipGraphLaunch(g.exec, s->stream) != hipSuccess) {
+    // The capture advanced the buffer indices but nothing ran: put them back
+    // so the caller's eager fallback starts from this step's real state.
+    (void)hipGetLastError();
+    s->flow_a = g.flow_a;
+    s->pressure_index = g.pressure_index;
+    s->corr_index = corr_before;
+    return false;
+  }
+  return true;
 }
 
 // The water branch at the head of encodeSimulationStep
@@ -468,6 +486,7 @@ int encode_step(firex_solver* s, const float u[FXU_COUNT]) {
   } else {
     encode_rest(s, st);
   }
+  s->divergence_is_rhs = true;
   if (s->timing) {
     s->totals.steps += 1;
     if (s->pending.size() > 8192) {
@@ -991,7 +1010,7 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
     case FIREX_FIELD_FLOW_PREDICTED: dst = flow1(s); comps = 4; break;
     case FIREX_FIELD_SPECIES: dst = s->species[s->species_index]; comps = 4; kind = s->cfg.species_storage; break;
     case FIREX_FIELD_PRESSURE: dst = s->pressure[s->pressure_index]; kind = s->f16p ? 1 : 0; break;
-    case FIREX_FIELD_DIVERGENCE: dst = s->divergence; break;
+    case FIREX_FIELD_DIVERGENCE: dst = s->divergence; s->divergence_is_rhs = true; break;
     case FIREX_FIELD_REACTION: dst = s->reaction; break;
     case FIREX_FIELD_LIQUID:
       kind = 3;
@@ -1050,8 +1069,10 @@ int firex_pack_fields(firex_solver* s) {
   // u32(clamp(round(controls.x), 0, 7)) == 7u, qualityShaders.ts:55
   const float view = fxm_clamp(nearbyintf(u[FXU_VIEW_MODE]), 0.0f, 7.0f);
   const int divergence_view = (uint32_t)view == 7u ? 1 : 0;
-  if (divergence_view)
+  if (divergence_view) {
     timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow0(s), s->divergence, st, s->stream); });
+    s->divergence_is_rhs = false;
+  }
   timed(s, FIREX_PASS_PACK, [&] {
     launch_pack_fields(s->cfg.species_storage, flow0(s), s->species[s->species_index], s->reaction, s->liquid,
                        s->divergence, s->packed, st, divergence_view, s->stream);
@@ -1182,17 +1203,19 @@ int firex_render_frame(firex_solver* s, const firex_render_config* rc) {
   return FIREX_OK;
 }
 
-int firex_reduce(firex_solver* s, int32_t which, double* out) {
-  int rc = check_usable(s);
-  if (rc) return rc;
-  if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
-  hipSetDevice(s->cfg.device);
+// Enqueues reduction `which` on the solver's stream: block partials into
+// `partial` (at most kReducePartials doubles), the FP64 sum into *result.
+static int enqueue_reduction(firex_solver* s, int32_t which, double* partial, double* result) {
+  if (which < FIREX_REDUCE_SUM_FUEL || which > FIREX_REDUCE_FLOW_DIVERGENCE_L2)
+    return set_error(FIREX_ERR_INVALID, "unknown reduction");
+  if (which == FIREX_REDUCE_RESIDUAL_L2 && !s->divergence_is_rhs)
+    return set_error(FIREX_ERR_INVALID,
+                     "the residual's right-hand side is gone: the divergence view (firex_pack_fields with viewMode 7) "
+                     "replaced the divergence field with the final flow's divergence; step again first");
   float u[FXU_COUNT];
   write_uniforms(s->params, s->dims, 0.0, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
                  s->cfg.cinematic != 0, u, s->cfg.particle_count);
   const FxStep st = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
-  double* partial = s->scratch;
-  double* result = s->scratch + kReducePartials;
   switch (which) {
     case FIREX_REDUCE_SUM_FUEL: case FIREX_REDUCE_SUM_OXYGEN: case FIREX_REDUCE_SUM_PRODUCTS: case FIREX_REDUCE_SUM_VAPOR:
       launch_reduce_field(s->cfg.species_storage, 0, which - FIREX_REDUCE_SUM_FUEL, s->species[s->species_index], s->cells,
@@ -1210,12 +1233,20 @@ int firex_reduce(firex_solver* s, int32_t which, double* out) {
       launch_reduce_residual(s->f16p ? 1 : 0, p, s->divergence, st, partial, result, s->stream);
       break;
     }
-    case FIREX_REDUCE_FLOW_DIVERGENCE_L2:
+    default:  // FIREX_REDUCE_FLOW_DIVERGENCE_L2
       launch_reduce_flow_divergence(flow0(s), st, partial, result, s->stream);
       break;
-    default: return set_error(FIREX_ERR_INVALID, "unknown reduction");
   }
-  rc = check_launch(s, "reduction");
+  return check_launch(s, "reduction");
+}
+
+int firex_reduce(firex_solver* s, int32_t which, double* out) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
+  hipSetDevice(s->cfg.device);
+  double* result = s->scratch + kReducePartials;
+  rc = enqueue_reduction(s, which, s->scratch, result);
   if (rc) return rc;
   double v = 0;
   FX_HIP(s, hipMemcpyAsync(&v, result, sizeof v, hipMemcpyDeviceToHost, s->stream), "reduction readback");

@@ -339,7 +339,12 @@ int firex_render_volume(firex_solver* solver, const firex_render_config* config)
  * rounding moves c * 255 across a half step. */
 int firex_render_frame(firex_solver* solver, const firex_render_config* config);
 
-/* FP64 reduction on the device (FIREX_REDUCE_*). */
+/* FP64 reduction on the device (FIREX_REDUCE_*).  FIREX_REDUCE_RESIDUAL_L2
+ * fails with FIREX_ERR_INVALID after firex_pack_fields ran the divergence view
+ * (viewMode 7): that view replaces the residual solve's right-hand side in the
+ * divergence field with the final flow's divergence, as the reference's
+ * presentation divergence pass does (FireXEngine.ts:2588-2596); the next step
+ * restores it. */
 int firex_reduce(firex_solver* solver, int32_t which, double* out);
 int firex_get_info(firex_solver* solver, firex_info* info);
 
