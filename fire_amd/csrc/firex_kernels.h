@@ -79,5 +79,10 @@ void launch_droplet_streaks(const void* particles, int count, const FxRender& r,
                             void* work, hipStream_t s);
 
 constexpr uint32_t kReducePartials = 1024;
+// Blocks (= partials) an FP64 reduction over `cells` cells launches: at most
+// kReducePartials for any cells < 2^32.
+uint32_t reduce_block_count(uint32_t cells);
+// In-place sqrt of one double (the L2 norms of an on-device reduction).
+void launch_sqrt_f64(double* value, hipStream_t s);
 
 }  // namespace fx

@@ -777,6 +777,17 @@ static uint32_t reduce_blocks(uint32_t cells, uint32_t& per_block) {
   return (cells + per_block - 1) / per_block;
 }
 
+uint32_t reduce_block_count(uint32_t cells) {
+  uint32_t per_block;
+  return reduce_blocks(cells, per_block);
+}
+
+namespace {
+__global__ void k_sqrt_f64(double* v) { v[0] = sqrt(v[0]); }
+}
+
+void launch_sqrt_f64(double* value, hipStream_t s) { k_sqrt_f64<<<1, 1, 0, s>>>(value); }
+
 void launch_reduce_field(int storage, int kind, int component, const void* data, uint32_t cells, double* partial,
                          double* out, hipStream_t s) {
   uint32_t per_block;

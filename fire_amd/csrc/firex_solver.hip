@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1254,6 +1255,57 @@ int firex_reduce(firex_solver* s, int32_t which, double* out) {
   if (which == FIREX_REDUCE_RESIDUAL_L2 || which == FIREX_REDUCE_FLOW_DIVERGENCE_L2) v = sqrt(v);
   *out = v;
   return FIREX_OK;
+}
+
+// ------------------------------------------------ allocation-free reduction
+// firex_reduce_noalloc's scratch: the block partials of an FP64 reduction (at
+// most kReducePartials doubles, 8 KiB), a file-scope device array that each
+// GPU the library runs on holds until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for
+// the symbol): its address on each device comes from hipGetSymbolAddress,
+// once per device, and that pointer is what the kernels receive.
+namespace fx_noalloc {
+__device__ __align__(256) double scratch[kReducePartials];
+constexpr int kMaxDevices = 64;
+std::mutex lock;
+double* device_scratch[kMaxDevices] = {};
+}  // namespace fx_noalloc
+
+// The scratch array's address on device `dev`, which must be current.
+static int noalloc_scratch(int dev, double** out) {
+  if (dev < 0 || dev >= fx_noalloc::kMaxDevices)
+    return set_error(FIREX_ERR_INVALID, "device index beyond the allocation-free scratch table");
+  std::lock_guard<std::mutex> hold(fx_noalloc::lock);
+  if (!fx_noalloc::device_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(fx_noalloc::scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return set_error(FIREX_ERR_DEVICE, std::string("hipGetSymbolAddress of the reduction scratch failed: ") +
+                                             hipGetErrorString(e));
+    }
+    fx_noalloc::device_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = fx_noalloc::device_scratch[dev];
+  return FIREX_OK;
+}
+
+int firex_reduce_noalloc(firex_solver* s, int32_t which, double* device_out) {
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!device_out) return set_error(FIREX_ERR_INVALID, "device_out is null");
+  if (reduce_block_count(s->cells) > kReducePartials)
+    return set_error(FIREX_ERR_INVALID, "grid of " + std::to_string(s->cells) + " cells needs more than " +
+                                            std::to_string(kReducePartials) + " reduction partials");
+  hipSetDevice(s->cfg.device);
+  double* partial = nullptr;
+  rc = noalloc_scratch(s->cfg.device, &partial);
+  if (rc) return rc;
+  rc = enqueue_reduction(s, which, partial, device_out);
+  if (rc) return rc;
+  if (which == FIREX_REDUCE_RESIDUAL_L2 || which == FIREX_REDUCE_FLOW_DIVERGENCE_L2)
+    launch_sqrt_f64(device_out, s->stream);
+  return check_launch(s, "allocation-free reduction");
 }
 
 int firex_set_pass_timing(firex_solver* s, int32_t enabled) {

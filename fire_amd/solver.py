@@ -195,6 +195,7 @@ _SIGNATURES = {
     "firex_read_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
     "firex_write_field": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int32]),
     "firex_reduce": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "firex_reduce_noalloc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_void_p]),
     "firex_pack_fields": (ctypes.c_int, [_P]),
     "firex_get_field_layout": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(FieldLayout)]),
     "firex_render_volume": (ctypes.c_int, [_P, ctypes.POINTER(RenderConfig)]),

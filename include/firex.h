@@ -346,6 +346,18 @@ int firex_render_frame(firex_solver* solver, const firex_render_config* config);
  * presentation divergence pass does (FireXEngine.ts:2588-2596); the next step
  * restores it. */
 int firex_reduce(firex_solver* solver, int32_t which, double* out);
+/* firex_reduce without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference (it computes no
+ * reductions).  The result (the same FP64 value firex_reduce returns) is
+ * written to device_out, device-accessible memory the caller owns, on the
+ * solver's stream and without waiting: read it after firex_sync or order
+ * later work on that stream.  The block partials live in one 8 KiB array per
+ * GPU inside the library (grids up to 2^32 - 1 cells; larger are rejected).
+ * Every call on a GPU shares that array, so calls that overlap on one GPU
+ * (two solvers on different streams) race; calls on one solver are ordered
+ * by its stream. */
+int firex_reduce_noalloc(firex_solver* solver, int32_t which, double* device_out);
 int firex_get_info(firex_solver* solver, firex_info* info);
 
 /* ---- instrumentation (additions: the reference has no GPU timestamps) -- */
