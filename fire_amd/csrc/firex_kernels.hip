@@ -224,13 +224,18 @@ __global__ __launch_bounds__(256) void k_sim_tables(float* __restrict__ tab, FxS
   }
 }
 
+// A term whose exponent is below -86 adds exactly nothing: fxm_exp returns +0
+// there, amplitude * +0 is a zero and s + that zero is s (s starts at +0).
+// d2 >= 87 r^2 (the product rounded) guarantees d2 / r^2 rounds above 86, so
+// those cells skip the division and the exponential without changing a bit.
 __device__ inline float emitter_seeded(const FxStep& st, float pcx, float pcy, float pcz, uint32_t cell) {
   float s = 0.0f;
   for (int i = 0; i < st.seeded_count; ++i) {
     float dx = pcx - st.seeded[i][0], dy = pcy - st.seeded[i][1], dz = pcz - st.seeded[i][2];
     float d2 = dx * dx + dy * dy + dz * dz;
     float r = st.seeded[i][3];
-    s = s + st.seeded[i][4] * fxm_exp(-(d2 / (r * r)));
+    const float rr = r * r;
+    if (d2 < rr * 87.0f) s = s + st.seeded[i][4] * fxm_exp(-(d2 / rr));
   }
   if (s > 0.0f && st.jitter > 0.0f) {
     float uj = fx_seeded_uniform(st.seed, st.frame, cell, 0);
@@ -266,7 +271,14 @@ __global__ __launch_bounds__(256) void k_copy_u128(uint4* __restrict__ dst, cons
 }
 
 // simulate, shaders.ts:151-342 (advection, sources, reaction, forces, BCs).
-template <int S, bool kSeeded>
+// kLiquid = false (no liquid field, as when the hose never ran) fixes the
+// water amount at +0; the terms that then reduce to exact identities are
+// skipped: exp(-0) = 1 factors and the zero evaporation.  Per cell, the
+// reaction's Arrhenius rate is evaluated only where the ignition weight is
+// nonzero, and the radiation only where T is not exactly ambient; the other
+// cells take the exact values those terms reduce to (see below), so every
+// result keeps its bits, signed zeros included.
+template <int S, bool kSeeded, bool kLiquid>
 __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flowIn,
                                                   const typename SpeciesIO<S>::Word* __restrict__ specIn,
                                                   float4* __restrict__ flowOut,
@@ -350,12 +362,17 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     fl.w = fxm_mix(fl.w, st.ambient, 0.035f);
   }
 
-  const float liq = st.has_liquid ? (float)liquid[id] * (1.0f / 4096.0f) : 0.0f;
-  const float water = fxm_clamp(liq * 3.4f, 0.0f, 5.0f);
-  {
+  const float liq = kLiquid ? (float)liquid[id] * (1.0f / 4096.0f) : 0.0f;
+  const float water = kLiquid ? fxm_clamp(liq * 3.4f, 0.0f, 5.0f) : 0.0f;
+  if (kLiquid) {
     float evaporation = water * fxm_smoothstep(335.0f, 1040.0f, fl.w) * dt * 13.5f;
     fl.w = fxm_max(st.ambient, fl.w - evaporation * 2080.0f - water * dt * 260.0f);
     sp.w = fxm_clamp(sp.w + evaporation * 2.15f, 0.0f, 3.0f);
+  } else {
+    // water = +0: evaporation = +0 (smoothstep >= +0), so T - 0 - 0 = T and
+    // vapor + 0 (which turns a -0 into +0)
+    fl.w = fxm_max(st.ambient, fl.w);
+    sp.w = fxm_clamp(sp.w + 0.0f, 0.0f, 3.0f);
   }
   {
     float oxygenDeficit = fxm_clamp((st.ambient_oxygen - sp.y) * st.inv_ambient_oxygen, 0.0f, 1.0f);
@@ -372,20 +389,37 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     float fuelBefore = fxm_max(sp.x, 0.0f);
     float oxygenBefore = fxm_max(sp.y, 0.0f);
     float availableFuel = fxm_min(fuelBefore, oxygenBefore * 0.25f);
-    float safeFuel = fxm_max(fuelBefore, 0.00001f);
-    float safeOxygen = fxm_max(oxygenBefore, 0.00001f);
-    float activation = fxm_exp(-15000.0f / fxm_max(fl.w, st.ambient));
-    float arrhenius = fxm_clamp(2.5e6f * activation * fxm_pow(safeFuel, -0.30f) * fxm_pow(safeOxygen, 1.30f), 0.0f, 80.0f);
     float ignition = fxm_smoothstep(690.0f, 930.0f, fl.w);
-    float requestedBurn = availableFuel * (1.0f - fxm_exp(-arrhenius * ignition * dt)) * fxm_exp(-water * 18.0f);
-    float burn = fxm_clamp(requestedBurn, 0.0f, availableFuel);
-    float richness = fxm_clamp(1.0f - oxygenBefore / fxm_max(fuelBefore * 4.0f, 0.00001f), 0.0f, 1.0f);
-    sp.x = fxm_max(0.0f, fuelBefore - burn);
-    sp.y = fxm_max(0.0f, oxygenBefore - burn * 4.0f);
-    sp.z = fxm_clamp(sp.z + burn * (1.0f + richness * 1.8f), 0.0f, 3.0f);
-    sp.w = fxm_clamp(sp.w + burn * 0.62f, 0.0f, 3.0f);
-    fl.w += burn * st.heat_release;
-    react = fxm_clamp(burn * st.inv_max_dt * 0.22f, 0.0f, 4.0f);
+    if (ignition != 0.0f) {
+      float safeFuel = fxm_max(fuelBefore, 0.00001f);
+      float safeOxygen = fxm_max(oxygenBefore, 0.00001f);
+      float activation = fxm_exp(-15000.0f / fxm_max(fl.w, st.ambient));
+      float arrhenius =
+          fxm_clamp(2.5e6f * activation * fxm_pow(safeFuel, -0.30f) * fxm_pow(safeOxygen, 1.30f), 0.0f, 80.0f);
+      float requestedBurn = availableFuel * (1.0f - fxm_exp(-arrhenius * ignition * dt));
+      if (kLiquid) requestedBurn = requestedBurn * fxm_exp(-water * 18.0f);  // else * exp(-0) = * 1
+      float burn = fxm_clamp(requestedBurn, 0.0f, availableFuel);
+      float richness = fxm_clamp(1.0f - oxygenBefore / fxm_max(fuelBefore * 4.0f, 0.00001f), 0.0f, 1.0f);
+      sp.x = fxm_max(0.0f, fuelBefore - burn);
+      sp.y = fxm_max(0.0f, oxygenBefore - burn * 4.0f);
+      sp.z = fxm_clamp(sp.z + burn * (1.0f + richness * 1.8f), 0.0f, 3.0f);
+      sp.w = fxm_clamp(sp.w + burn * 0.62f, 0.0f, 3.0f);
+      fl.w += burn * st.heat_release;
+      react = fxm_clamp(burn * st.inv_max_dt * 0.22f, 0.0f, 4.0f);
+    } else {
+      // Below 690 K the ignition weight is a zero: the finite rate times it
+      // is a zero, exp of a zero is exactly 1, so the request is
+      // availableFuel * (+0) (times exp(-18 w) > 0), a zero with
+      // availableFuel's sign, and every burn-weighted term adds that zero:
+      // burn * c = burn for the positive factors c below, and T (far from
+      // zero) + burn = T.
+      float burn = fxm_clamp(availableFuel * 0.0f, 0.0f, availableFuel);
+      sp.x = fxm_max(0.0f, fuelBefore - burn);
+      sp.y = fxm_max(0.0f, oxygenBefore - burn);
+      sp.z = fxm_clamp(sp.z + burn, 0.0f, 3.0f);
+      sp.w = fxm_clamp(sp.w + burn, 0.0f, 3.0f);
+      react = fxm_clamp(burn, 0.0f, 4.0f);
+    }
   }
   {
     float curlx = 0.5f * ((Tp.z - B.z) * st.my - (F.y - K.y) * st.mz);
@@ -417,13 +451,18 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
     float buoyancy = fxm_clamp((fl.w - st.ambient) * (1.0f / 1500.0f), -0.1f, 1.2f);
     fl.y += buoyancy * dt * 0.72f * st.buoyancy_clamp;
   }
-  sp.z *= fxm_exp(-water * dt * 4.8f);
+  if (kLiquid) sp.z *= fxm_exp(-water * dt * 4.8f);  // else * exp(-0) = * 1
   fl.x = fl.x + st.spray_x * water * dt * 0.82f;
   fl.y = fl.y + st.spray_y * water * dt * 0.82f;
   fl.z = fl.z + st.spray_z * water * dt * 0.82f;
   {
-    float radiation = 5.67e-8f * (st.ambient4 - fxm_pow(fxm_max(fl.w, 0.0f), 4.0f)) * st.inv_radiation_denominator;
-    fl.w = fxm_max(st.ambient, fl.w + dt * radiation);
+    // At exactly ambient T the radiation is 5.67e-8 (ambient4 - ambient4) ... = +0
+    // (ambient4 is this same fxm_pow of the ambient), and max(ambient,
+    // ambient + dt * 0) = ambient: nothing to evaluate.
+    if (fl.w != st.ambient) {
+      float radiation = 5.67e-8f * (st.ambient4 - fxm_pow(fxm_max(fl.w, 0.0f), 4.0f)) * st.inv_radiation_denominator;
+      fl.w = fxm_max(st.ambient, fl.w + dt * radiation);
+    }
     sp.z *= st.decay_products;
     sp.w *= st.decay_vapor;
   }
@@ -711,12 +750,21 @@ static void simulate_s(const SimulateArgs& a, hipStream_t s) {
   uint32_t blocks = (entries + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   k_sim_tables<<<blocks, 256, 0, s>>>(a.tables, st, st.seeded_count > 0 ? 0 : 1);
-  if (st.seeded_count > 0)
-    k_simulate<S, true><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(a.flow_in, (const W*)a.species_in, a.flow_out,
-                                                                  (W*)a.species_out, a.reaction, a.liquid, a.tables, st);
-  else
-    k_simulate<S, false><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(a.flow_in, (const W*)a.species_in, a.flow_out,
-                                                                   (W*)a.species_out, a.reaction, a.liquid, a.tables, st);
+  const W* sin_ = (const W*)a.species_in;
+  W* sout = (W*)a.species_out;
+  const dim3 g = cell_grid(st), b(kBX, kBY);
+  const bool liquid = st.has_liquid != 0;
+  if (st.seeded_count > 0) {
+    if (liquid)
+      k_simulate<S, true, true><<<g, b, 0, s>>>(a.flow_in, sin_, a.flow_out, sout, a.reaction, a.liquid, a.tables, st);
+    else
+      k_simulate<S, true, false><<<g, b, 0, s>>>(a.flow_in, sin_, a.flow_out, sout, a.reaction, a.liquid, a.tables, st);
+  } else {
+    if (liquid)
+      k_simulate<S, false, true><<<g, b, 0, s>>>(a.flow_in, sin_, a.flow_out, sout, a.reaction, a.liquid, a.tables, st);
+    else
+      k_simulate<S, false, false><<<g, b, 0, s>>>(a.flow_in, sin_, a.flow_out, sout, a.reaction, a.liquid, a.tables, st);
+  }
 }
 
 size_t sim_table_bytes(const int32_t dims[3]) { return (size_t)fx_sim_table_floats(dims) * 4u; }
