@@ -21,19 +21,40 @@ namespace fx {
 // ------------------------------------------------------------------ storage
 template <int S> struct SpeciesIO;
 
+// 16 bytes at an 8-byte aligned address (two consecutive 8-byte words).
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef u32x4_t u32x4_a8_t __attribute__((aligned(8)));
+
+// Each storage also loads two consecutive words at once (load_pair: the x
+// pair of a trilinear corner, one 16-byte load for the 8-byte storages).
 template <> struct SpeciesIO<FX_STORAGE_F32> {
   using Word = float4;
+  using Bits = float4;
   __device__ static inline float4 load(const Word* p, uint32_t i) { return p[i]; }
   __device__ static inline void store(Word* p, uint32_t i, float4 v) { p[i] = v; }
+  __device__ static inline Bits load_bits(const Word* p, uint32_t i) { return p[i]; }
+  __device__ static inline float4 unpack(Bits b) { return b; }
+  __device__ static inline void load_pair(const Word* p, uint32_t i, float4& a, float4& b) {
+    a = p[i];
+    b = p[i + 1];
+  }
 };
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 
 template <> struct SpeciesIO<FX_STORAGE_F16> {
   using Word = half4_t;
-  __device__ static inline float4 load(const Word* p, uint32_t i) {
-    half4_t h = p[i];
+  using Bits = uint2;
+  __device__ static inline float4 unpack(Bits b) {
+    half4_t h = __builtin_bit_cast(half4_t, b);
     return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+  }
+  __device__ static inline Bits load_bits(const Word* p, uint32_t i) { return __builtin_bit_cast(uint2, p[i]); }
+  __device__ static inline float4 load(const Word* p, uint32_t i) { return unpack(load_bits(p, i)); }
+  __device__ static inline void load_pair(const Word* p, uint32_t i, float4& a, float4& b) {
+    const u32x4_t v = *reinterpret_cast<const u32x4_a8_t*>(p + i);
+    a = unpack(make_uint2(v.x, v.y));
+    b = unpack(make_uint2(v.z, v.w));
   }
   __device__ static inline void store(Word* p, uint32_t i, float4 v) {
     half4_t h;
@@ -47,10 +68,17 @@ template <> struct SpeciesIO<FX_STORAGE_F16> {
 
 template <> struct SpeciesIO<FX_STORAGE_BF16> {
   using Word = ushort4;
-  __device__ static inline float4 load(const Word* p, uint32_t i) {
-    ushort4 b = p[i];
-    return make_float4(fx_bf16_bits_to_float(b.x), fx_bf16_bits_to_float(b.y),
-                       fx_bf16_bits_to_float(b.z), fx_bf16_bits_to_float(b.w));
+  using Bits = uint2;
+  __device__ static inline float4 unpack(Bits w) {
+    return make_float4(fxm_bits_to_float(w.x << 16), fxm_bits_to_float(w.x & 0xffff0000u),
+                       fxm_bits_to_float(w.y << 16), fxm_bits_to_float(w.y & 0xffff0000u));
+  }
+  __device__ static inline Bits load_bits(const Word* p, uint32_t i) { return __builtin_bit_cast(uint2, p[i]); }
+  __device__ static inline float4 load(const Word* p, uint32_t i) { return unpack(load_bits(p, i)); }
+  __device__ static inline void load_pair(const Word* p, uint32_t i, float4& a, float4& b) {
+    const u32x4_t v = *reinterpret_cast<const u32x4_a8_t*>(p + i);
+    a = unpack(make_uint2(v.x, v.y));
+    b = unpack(make_uint2(v.z, v.w));
   }
   __device__ static inline void store(Word* p, uint32_t i, float4 v) {
     ushort4 b;
@@ -108,6 +136,42 @@ __device__ inline uint32_t cidx(const FxStep& st, int x, int y, int z) {
 
 __device__ inline float4 mix4(float4 a, float4 b, float t) {
   return make_float4(fxm_mix(a.x, b.x, t), fxm_mix(a.y, b.y, t), fxm_mix(a.z, b.z, t), fxm_mix(a.w, b.w, t));
+}
+
+// The sample point of trilinearFlow / trilinearSpecies (shaders.ts:61-95) is
+// clamped to [0, dims - 1.001] per axis, so its floor lies in [0, dims - 2]
+// (every grid has at least two cells per axis) and no corner index needs the
+// edge clamp: the corners are base, base + 1 (x), + nx (y), + nx ny (z).
+// 24-bit multiplies suffice (coordinates and grid sizes are at most 1024).
+struct TriCorners {
+  uint32_t i000, sy, sz;
+  float fx, fy, fz;
+};
+__device__ inline TriCorners tri_corners(const FxStep& st, float px, float py, float pz) {
+  float sx = fxm_clamp(px - 0.5f, 0.0f, st.gx - 1.001f);
+  float sy = fxm_clamp(py - 0.5f, 0.0f, st.gy - 1.001f);
+  float sz = fxm_clamp(pz - 0.5f, 0.0f, st.gz - 1.001f);
+  float flx = fxm_floor(sx), fly = fxm_floor(sy), flz = fxm_floor(sz);
+  TriCorners c;
+  const uint32_t bx = (uint32_t)(int)flx, by = (uint32_t)(int)fly, bz = (uint32_t)(int)flz;
+  c.sy = (uint32_t)st.nx;
+  c.sz = __umul24((uint32_t)st.nx, (uint32_t)st.ny);
+  c.i000 = bx + __umul24(by, c.sy) + __umul24(bz, c.sz);
+  c.fx = sx - flx;
+  c.fy = sy - fly;
+  c.fz = sz - flz;
+  return c;
+}
+template <typename Pair>
+__device__ inline float4 trilinear_pairs(const TriCorners& c, Pair pair) {
+  float4 c000, c100, c010, c110, c001, c101, c011, c111;
+  pair(c.i000, c000, c100);
+  pair(c.i000 + c.sy, c010, c110);
+  pair(c.i000 + c.sz, c001, c101);
+  pair(c.i000 + c.sz + c.sy, c011, c111);
+  float4 z0 = mix4(mix4(c000, c100, c.fx), mix4(c010, c110, c.fx), c.fy);
+  float4 z1 = mix4(mix4(c001, c101, c.fx), mix4(c011, c111, c.fx), c.fy);
+  return mix4(z0, z1, c.fz);
 }
 
 template <typename Load>
@@ -298,13 +362,19 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
   float bxp = pcx - orig.x * dt * st.mx;
   float byp = pcy - orig.y * dt * st.my;
   float bzp = pcz - orig.z * dt * st.mz;
-  float4 fl = trilinear(st, bxp, byp, bzp, [&](uint32_t i) { return flowIn[i]; });
-  float4 sp = trilinear(st, bxp, byp, bzp, [&](uint32_t i) { return SpeciesIO<S>::load(specIn, i); });
+  const TriCorners tc = tri_corners(st, bxp, byp, bzp);
+  float4 fl = trilinear_pairs(tc, [&](uint32_t i, float4& a, float4& b) { a = flowIn[i]; b = flowIn[i + 1]; });
+  float4 sp = trilinear_pairs(tc, [&](uint32_t i, float4& a, float4& b) { SpeciesIO<S>::load_pair(specIn, i, a, b); });
   sp = make_float4(fxm_max(sp.x, 0.0f), fxm_max(sp.y, 0.0f), fxm_max(sp.z, 0.0f), fxm_max(sp.w, 0.0f));
 
-  const float4 L = flowIn[cidx(st, x - 1, y, z)], R = flowIn[cidx(st, x + 1, y, z)];
-  const float4 B = flowIn[cidx(st, x, y - 1, z)], Tp = flowIn[cidx(st, x, y + 1, z)];
-  const float4 K = flowIn[cidx(st, x, y, z - 1)], F = flowIn[cidx(st, x, y, z + 1)];
+  // Six neighbours, clamped at the walls (loadFlow, shaders.ts:170-175).
+  const uint32_t sy = (uint32_t)st.nx, sz = __umul24((uint32_t)st.nx, (uint32_t)st.ny);
+  const uint32_t iL = x > 0 ? id - 1 : id, iR = x < st.nx - 1 ? id + 1 : id;
+  const float4 L = flowIn[iL], R = flowIn[iR];
+  const uint32_t iB = y > 0 ? id - sy : id, iT = y < st.ny - 1 ? id + sy : id;
+  const uint32_t iK = z > 0 ? id - sz : id, iF = z < st.nz - 1 ? id + sz : id;
+  const float4 B = flowIn[iB], Tp = flowIn[iT];
+  const float4 K = flowIn[iK], F = flowIn[iF];
   float vax = (L.x + R.x + B.x + Tp.x + K.x + F.x) * (1.0f / 6.0f);
   float vay = (L.y + R.y + B.y + Tp.y + K.y + F.y) * (1.0f / 6.0f);
   float vaz = (L.z + R.z + B.z + Tp.z + K.z + F.z) * (1.0f / 6.0f);
@@ -315,12 +385,10 @@ __global__ __launch_bounds__(256) void k_simulate(const float4* __restrict__ flo
   fl.w = fxm_mix(fl.w, ta, st.mix_temperature);
 
   {
-    const float4 sL = SpeciesIO<S>::load(specIn, cidx(st, x - 1, y, z));
-    const float4 sR = SpeciesIO<S>::load(specIn, cidx(st, x + 1, y, z));
-    const float4 sB = SpeciesIO<S>::load(specIn, cidx(st, x, y - 1, z));
-    const float4 sT = SpeciesIO<S>::load(specIn, cidx(st, x, y + 1, z));
-    const float4 sK = SpeciesIO<S>::load(specIn, cidx(st, x, y, z - 1));
-    const float4 sF = SpeciesIO<S>::load(specIn, cidx(st, x, y, z + 1));
+    using SIO = SpeciesIO<S>;
+    const float4 sL = SIO::load(specIn, iL), sR = SIO::load(specIn, iR);
+    const float4 sB = SIO::load(specIn, iB), sT = SIO::load(specIn, iT);
+    const float4 sK = SIO::load(specIn, iK), sF = SIO::load(specIn, iF);
     float4 sa = make_float4((sL.x + sR.x + sB.x + sT.x + sK.x + sF.x) * (1.0f / 6.0f),
                             (sL.y + sR.y + sB.y + sT.y + sK.y + sF.y) * (1.0f / 6.0f),
                             (sL.z + sR.z + sB.z + sT.z + sK.z + sF.z) * (1.0f / 6.0f),
