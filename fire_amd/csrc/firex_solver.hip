@@ -787,6 +787,129 @@ void firex_destroy(firex_solver* s) {
   delete s;
 }
 
+namespace {
+
+// formatFireXBytes (src/sim/firex/gridConfiguration.ts:386-399): KiB below
+// 1 MiB, else MiB, GiB or TiB with two decimals (one from 10 up), trailing
+// zeros dropped (formatDecimal, :372-375).
+std::string format_bytes(double bytes) {
+  auto dec = [](double v, int digits) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.*f", digits, v);
+    std::string t = buf;
+    const size_t dot = t.find('.');
+    if (dot != std::string::npos) {
+      size_t end = t.size();
+      while (end > dot + 1 && t[end - 1] == '0') --end;
+      if (end == dot + 1) end = dot;
+      t.resize(end);
+    }
+    return t;
+  };
+  if (!(bytes >= 0)) return "\u2014";
+  const double mib = bytes / (1024.0 * 1024.0);
+  if (mib < 1) return dec(bytes / 1024.0, 1) + " KiB";
+  const double gib = mib / 1024.0;
+  if (gib >= 1024) {
+    const double tib = gib / 1024.0;
+    return dec(tib, tib >= 10 ? 1 : 2) + " TiB";
+  }
+  if (gib >= 1) return dec(gib, gib >= 10 ? 1 : 2) + " GiB";
+  return dec(mib, mib >= 10 ? 1 : 2) + " MiB";
+}
+
+uint64_t estimated_bytes(const int32_t dims[3], int32_t particles) {
+  firex_memory_estimate e;
+  firex_estimate_memory(dims, particles, &e);
+  return (uint64_t)e.total_bytes;
+}
+
+bool same_buffer_set(const firex_solver* s, const firex_config& c, bool heavy, bool f16p, bool compact) {
+  const firex_config& o = s->cfg;
+  return c.dims[0] == o.dims[0] && c.dims[1] == o.dims[1] && c.dims[2] == o.dims[2] &&
+         c.particle_count == o.particle_count && c.species_storage == o.species_storage && heavy == s->heavy &&
+         f16p == s->f16p && compact == s->compact && c.plan == o.plan &&
+         c.jacobi_sweeps_per_launch == o.jacobi_sweeps_per_launch && c.jacobi_chunk_z == o.jacobi_chunk_z &&
+         c.jacobi_kernel == o.jacobi_kernel && c.jacobi_autotune == o.jacobi_autotune &&
+         c.jacobi_tile_height == o.jacobi_tile_height && c.jacobi_prefetch == o.jacobi_prefetch;
+}
+
+}  // namespace
+
+int firex_reconfigure(firex_solver* s, const firex_config* config, uint64_t memory_guard_bytes, int32_t* rebuilt) {
+  clear_error();
+  int rc = check_usable(s);
+  if (rc) return rc;
+  if (!config) return set_error(FIREX_ERR_INVALID, "config is null");
+  if (!s->own_stream) return set_error(FIREX_ERR_INVALID, "a plume on a serving stream cannot be reconfigured");
+  if (config->device != s->cfg.device) return set_error(FIREX_ERR_INVALID, "the device of a solver cannot change");
+  firex_config next = *config;
+  next.memory_budget_bytes = 0;  // the transactional guard below replaces the creation guard
+  bool heavy = false, f16p = false, compact = false;
+  uint64_t need = 0;
+  rc = solver_plan(&next, &heavy, &f16p, &compact, &need);
+  if (rc) return rc;
+  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipStreamSynchronize(s->stream), "reconfigure");
+  rc = collect_timing(s);
+  if (rc) return rc;
+  if (same_buffer_set(s, next, heavy, f16p, compact)) {
+    // The buffer set stays (FireXEngine.ts:814-820, 870-880 keep matching
+    // resources): only the per-step settings change, the fields are kept.
+    if (next.pressure_iterations != s->cfg.pressure_iterations ||
+        next.correction_iterations != s->cfg.correction_iterations || next.capture_graphs != s->cfg.capture_graphs)
+      drop_graphs(s);
+    s->cfg.pressure_iterations = next.pressure_iterations;
+    s->cfg.correction_iterations = next.correction_iterations;
+    s->cfg.time_step = next.time_step;
+    s->cfg.maximum_substeps = next.maximum_substeps;
+    s->cfg.cinematic = next.cinematic;
+    s->cfg.ray_steps = next.ray_steps;
+    s->cfg.capture_graphs = next.capture_graphs;
+    s->cfg.memory_budget_bytes = config->memory_budget_bytes;
+    s->clock = FixedStepClock();  // a new FixedStepClock for the configuration
+    s->clock.step_seconds = next.time_step;
+    s->clock.maximum_steps = next.maximum_substeps;
+    if (rebuilt) *rebuilt = 0;
+    return FIREX_OK;
+  }
+  // assertGridSupported (FireXEngine.ts:2823-2853): the estimated peak of the
+  // current and the replacement set must fit the guard.
+  const uint64_t current = estimated_bytes(s->dims, s->cfg.particle_count);
+  const uint64_t replacement = estimated_bytes(next.dims, next.particle_count);
+  if (memory_guard_bytes && current + replacement > memory_guard_bytes)
+    return set_error(FIREX_ERR_BUDGET, "Transactional rebuild peak requires about " +
+                                           format_bytes((double)(current + replacement)) + " (" +
+                                           format_bytes((double)current) + " current + " +
+                                           format_bytes((double)replacement) + " replacement), above the selected " +
+                                           format_bytes((double)memory_guard_bytes) + " guard.");
+  // rebuildResourcesSafely (FireXEngine.ts:1891-1915): build the replacement
+  // while the current set stays live; any failure leaves the solver as it was.
+  firex_solver* fresh = nullptr;
+  rc = solver_create(&next, nullptr, &fresh);
+  if (rc) return rc;
+  fresh->cfg.memory_budget_bytes = config->memory_budget_bytes;
+  // installResourceSet (:1917-1931): the parameters carry over, the clock,
+  // time and indices start again and the fields are initialised with the
+  // current parameters.
+  fresh->params = s->params;
+  fresh->emitters = s->emitters;
+  rc = initialize_fields(fresh);
+  if (!rc) {
+    const hipError_t e = hipStreamSynchronize(fresh->stream);
+    if (e != hipSuccess) rc = hip_fail(fresh, e, "field initialization");
+  }
+  if (rc) {
+    const std::string msg = firex_last_error();
+    firex_destroy(fresh);
+    return set_error(rc, msg);
+  }
+  std::swap(*s, *fresh);  // the handle now holds the new set
+  firex_destroy(fresh);   // and the old one is freed
+  if (rebuilt) *rebuilt = 1;
+  return FIREX_OK;
+}
+
 int firex_reset(firex_solver* s) {
   int rc = check_usable(s);
   if (rc) return rc;

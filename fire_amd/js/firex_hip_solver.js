@@ -8,8 +8,13 @@
  * setParameter (FireXEngine.ts:937-1032), initialize (:506-547), reset
  * (:843-852), renderOfflineFrame(dt, present = false, signal) (:643-697),
  * frame(t, dt) (:629-641), getGridInfo (:478-490), dispose (:1034-1062),
- * plus field readback (new surface).  Presentation (present = true) is not
- * part of this solver: rendering stays on the reference path.
+ * the configuration setters setGridMemoryBudget (:929-935), setQuality
+ * (:863-893), setGridDimensions (:895-927), setOfflineRenderTier (:799-831)
+ * and setOfflineOpticalDetailTarget (:834-841), serialised like the
+ * reference's resourceMutationQueue (:1882-1889) and rebuilding the device
+ * buffers transactionally (firex_reconfigure), plus field readback (new
+ * surface).  Presentation (present = true) is not part of this solver:
+ * rendering stays on the reference path.
  *
  * CommonJS, no dependencies; runs on Node >= 12 (N-API 8).
  */
@@ -27,6 +32,35 @@ const REDUCTIONS = {
   sumFuel: 0, sumOxygen: 1, sumProducts: 2, sumVapor: 3, sumTemperature: 4, sumReaction: 5,
   residualL2: 6, flowDivergenceL2: 7,
 };
+
+// Tier names (FIREX_QUALITY and FIREX_OFFLINE_QUALITY, FireXEngine.ts:187-277)
+// and the offline tiers' minimum transition guards (minimumMemoryGuardBytes).
+const QUALITIES = ['low', 'balanced', 'high', 'maximum', 'cinematic'];
+const OFFLINE_TIERS = ['hd', 'qhd', 'uhd'];
+const MIB = 1024 * 1024;
+const OFFLINE_MINIMUM_GUARD = { hd: 512 * MIB, qhd: 1024 * MIB, uhd: 2048 * MIB };
+// gridConfiguration.ts:111, 118-128: the default budget and the largest guard.
+const DEFAULT_GRID_MEMORY_BUDGET_BYTES = 512 * MIB;
+const MAX_GRID_MEMORY_BUDGET_BYTES = 32 * 1024 * MIB;
+
+/** formatFireXBytes, gridConfiguration.ts:372-399. */
+function formatBytes(bytes) {
+  const decimal = (value, digits) => value.toFixed(digits).replace(/\.0+$|(\.[0-9]*?)0+$/, '$1');
+  if (!Number.isFinite(bytes) || bytes < 0) return '\u2014';
+  const mebibytes = bytes / MIB;
+  if (mebibytes < 1) return `${decimal(bytes / 1024, 1)} KiB`;
+  const gibibytes = mebibytes / 1024;
+  if (gibibytes >= 1024) {
+    const tebibytes = gibibytes / 1024;
+    return `${decimal(tebibytes, tebibytes >= 10 ? 1 : 2)} TiB`;
+  }
+  if (gibibytes >= 1) return `${decimal(gibibytes, gibibytes >= 10 ? 1 : 2)} GiB`;
+  return `${decimal(mebibytes, mebibytes >= 10 ? 1 : 2)} MiB`;
+}
+
+function sameDimensions(a, b) {
+  return Boolean(a && b) && a.length === 3 && b.length === 3 && a.every((value, index) => value === b[index]);
+}
 
 function abortError(message) {
   const error = new Error(message || 'The Fire-X offline render was aborted.');
@@ -49,6 +83,22 @@ class FireXHipSolver {
     this.handle = null;
     this.disposed = false;
     this.pending = [];
+    // The engine's configuration state (FireXEngine.ts:405-421).  A live
+    // quality tier takes options.dims as its grid override (exact); an
+    // offline tier takes them as an exact grid (an addition: the reference's
+    // offline override is a per-axis minimum, resolveFireXOfflineDimensions).
+    const tier = this.options.tier;
+    if (!QUALITIES.includes(tier) && !OFFLINE_TIERS.includes(tier)) {
+      throw new RangeError(`Unknown Fire-X tier "${tier}".`);
+    }
+    this.quality = QUALITIES.includes(tier) ? tier : 'cinematic';
+    this.offlineRenderTier = OFFLINE_TIERS.includes(tier) ? tier : null;
+    this.gridOverride = QUALITIES.includes(tier) && options.dims ? [...options.dims] : null;
+    this.exactDimensions = OFFLINE_TIERS.includes(tier) && options.dims ? [...options.dims] : null;
+    this.gridMemoryBudgetBytes = DEFAULT_GRID_MEMORY_BUDGET_BYTES;
+    this.offlineOpticalDetailTarget = null;
+    this.rebuilding = false;
+    this.resourceMutationQueue = Promise.resolve();
     if (options.scene) {
       for (const key of Object.keys(options.scene)) this.setParameter(key, options.scene[key]);
     }
@@ -57,10 +107,136 @@ class FireXHipSolver {
   async initialize() {
     if (this.handle) return;
     if (this.disposed) throw new Error('The Fire-X engine has already been disposed.');
-    this.handle = native.create(this.options);
+    this.handle = native.create(this.nativeOptions());
     for (const [name, value] of this.pending) native.setParameter(this.handle, name, value);
     this.pending = [];
     native.reset(this.handle);  // the reset state follows the scene's oxygenRate
+  }
+
+  // ---- configuration (FireXEngine.ts:799-935) ------------------------------
+
+  /** setGridMemoryBudget: the transactional-rebuild guard, clamped to [256 MiB, 32 GiB]. */
+  setGridMemoryBudget(bytes) {
+    if (!Number.isFinite(bytes) || bytes <= 0) return;
+    this.gridMemoryBudgetBytes = Math.max(256 * MIB, Math.min(MAX_GRID_MEMORY_BUDGET_BYTES, Math.floor(bytes)));
+  }
+
+  /** setQuality: a live quality tier; rebuilds when its grid or droplet count differs. */
+  setQuality(quality) {
+    return this.enqueueResourceMutation(async () => {
+      if (this.offlineRenderTier !== null) {
+        throw new Error('Compute quality cannot change while an exact offline solver bundle is active.');
+      }
+      if (!QUALITIES.includes(quality)) throw new RangeError(`Unknown Fire-X quality "${quality}".`);
+      if (quality === this.quality) return;
+      const previous = this.quality;
+      this.quality = quality;
+      let rebuilt = false;
+      this.rebuilding = true;
+      try {
+        if (this.handle) rebuilt = this.applyConfiguration(this.gridMemoryBudgetBytes);
+      } catch (error) {
+        this.quality = previous;
+        throw error;
+      } finally {
+        this.rebuilding = false;
+      }
+      if (!rebuilt && this.handle) this.reset();
+    });
+  }
+
+  /** setGridDimensions: the live grid override (null: the quality's grid). */
+  setGridDimensions(dimensions) {
+    const next = dimensions ? [dimensions[0], dimensions[1], dimensions[2]] : null;
+    return this.enqueueResourceMutation(async () => {
+      if (this.offlineRenderTier !== null) {
+        throw new Error('Grid dimensions cannot change while an exact offline solver bundle is active.');
+      }
+      const previous = this.gridOverride;
+      if (previous === next || sameDimensions(previous, next)) return;
+      const target = next || this.tierConfiguration(this.quality).dimensions;
+      if (!this.handle || sameDimensions(native.getInfo(this.handle).dimensions, target)) {
+        this.gridOverride = next;
+        return;
+      }
+      this.rebuilding = true;
+      try {
+        this.gridOverride = next;
+        this.applyConfiguration(this.gridMemoryBudgetBytes);
+      } catch (error) {
+        this.gridOverride = previous;
+        throw error;
+      } finally {
+        this.rebuilding = false;
+      }
+    });
+  }
+
+  /** setOfflineRenderTier: an exact offline solver bundle (null: back to the live quality). */
+  setOfflineRenderTier(tier) {
+    return this.enqueueResourceMutation(async () => {
+      if (tier === this.offlineRenderTier) return;
+      if (tier !== null && !OFFLINE_TIERS.includes(tier)) throw new RangeError(`Unknown offline tier "${tier}".`);
+      const previousTier = this.offlineRenderTier;
+      const previousExact = this.exactDimensions;
+      const transitionGuard = Math.max(
+        this.gridMemoryBudgetBytes,
+        previousTier ? OFFLINE_MINIMUM_GUARD[previousTier] : 0,
+        tier ? OFFLINE_MINIMUM_GUARD[tier] : 0,
+      );
+      this.rebuilding = true;
+      try {
+        this.offlineRenderTier = tier;
+        this.exactDimensions = null;
+        if (this.handle) this.applyConfiguration(transitionGuard);
+      } catch (error) {
+        this.offlineRenderTier = previousTier;
+        this.exactDimensions = previousExact;
+        throw error;
+      } finally {
+        this.rebuilding = false;
+      }
+    });
+  }
+
+  /** setOfflineOpticalDetailTarget: the offline render-frequency target, snapped to 0 | 256 | 512 | 1024. */
+  setOfflineOpticalDetailTarget(target) {
+    this.offlineOpticalDetailTarget = target === null
+      ? null
+      : target >= 768 ? 1024 : target >= 384 ? 512 : target >= 192 ? 256 : 0;
+    if (this.offlineOpticalDetailTarget !== null) this.setParameter('opticalDetailTarget', this.offlineOpticalDetailTarget);
+  }
+
+  /** The grid the next build uses (effectiveGridDimensions, FireXEngine.ts:2800-2805). */
+  effectiveGridDimensions() {
+    if (this.exactDimensions) return [...this.exactDimensions];
+    if (this.offlineRenderTier !== null) return native.resolveOfflineDims(this.offlineRenderTier, this.gridOverride);
+    return this.gridOverride ? [...this.gridOverride] : this.tierConfiguration(this.quality).dimensions;
+  }
+
+  tierConfiguration(tier) {
+    return native.configFromTier(tier);
+  }
+
+  /** Options for the native solver: the active tier (activeConfiguration, :2807-2811) on the effective grid. */
+  nativeOptions() {
+    const options = Object.assign({}, this.options, {
+      tier: this.offlineRenderTier !== null ? this.offlineRenderTier : this.quality,
+      dims: this.effectiveGridDimensions(),
+    });
+    delete options.scene;
+    return options;
+  }
+
+  /** Applies the configuration to the live solver; true when the device buffers were rebuilt. */
+  applyConfiguration(guardBytes) {
+    return native.reconfigure(this.handle, this.nativeOptions(), guardBytes);
+  }
+
+  enqueueResourceMutation(operation) {
+    const queued = this.resourceMutationQueue.then(operation);
+    this.resourceMutationQueue = queued.then(() => undefined, () => undefined);
+    return queued;
   }
 
   setParameter(name, value) {
@@ -83,7 +259,7 @@ class FireXHipSolver {
   /** Live frame: fixed-step clock, returns the host milliseconds spent. */
   frame(timeSeconds, deltaSeconds) {
     void timeSeconds;
-    if (!this.handle || this.disposed) return 0;
+    if (!this.handle || this.disposed || this.rebuilding) return 0;
     const start = Date.now();
     native.frame(this.handle, deltaSeconds);
     return Date.now() - start;
@@ -167,12 +343,13 @@ class FireXHipSolver {
   }
 
   getGridInfo() {
-    const info = native.getInfo(this.requireHandle());
-    const domain = [
-      native.getParameter(this.handle, 'domainWidth'),
-      native.getParameter(this.handle, 'domainHeight'),
-      native.getParameter(this.handle, 'domainDepth'),
-    ];
+    if (this.disposed) throw new Error('The Fire-X engine has already been disposed.');
+    const info = this.handle
+      ? native.getInfo(this.handle)
+      : { dimensions: this.effectiveGridDimensions(), cellCount: 0, deviceBytes: 0, simulationTime: 0, frameIndex: 0 };
+    if (!this.handle) info.cellCount = info.dimensions[0] * info.dimensions[1] * info.dimensions[2];
+    const parameter = (name, fallback) => (this.handle ? native.getParameter(this.handle, name) : fallback);
+    const domain = [parameter('domainWidth', 1.15), parameter('domainHeight', 1.35), parameter('domainDepth', 1.15)];
     const estimate = native.estimateMemory(info.dimensions, 0);
     return {
       dimensions: info.dimensions,
@@ -183,7 +360,7 @@ class FireXHipSolver {
       deviceBytes: info.deviceBytes,
       simulationTime: info.simulationTime,
       frameIndex: info.frameIndex,
-      custom: this.options.dims !== undefined,
+      custom: this.gridOverride !== null || this.exactDimensions !== null,
     };
   }
 
@@ -211,6 +388,12 @@ function splitOfflineSimulationDelta(deltaSeconds, maximumStepSeconds) {
   return native.splitOfflineDelta(deltaSeconds, maximumStepSeconds);
 }
 
+/** resolveFireXOfflineDimensions, FireXEngine.ts:279-290. */
+function resolveFireXOfflineDimensions(tier, override) {
+  return native.resolveOfflineDims(tier, override || null);
+}
+
 module.exports = {
-  FireXHipSolver, estimateFireXGridMemory, splitOfflineSimulationDelta, native, REDUCTIONS, FIELD_COMPONENTS,
+  FireXHipSolver, estimateFireXGridMemory, splitOfflineSimulationDelta, resolveFireXOfflineDimensions, formatBytes,
+  native, REDUCTIONS, FIELD_COMPONENTS,
 };

@@ -121,53 +121,69 @@ static int str_prop(napi_env env, napi_value obj, const char* key, char* buf, si
   return str_arg(env, v, buf, cap);
 }
 
-/* create({tier, dims:[x,y,z], pressureIterations, correctionIterations,
- *         timeStep, maximumSubsteps, speciesStorage, heavyMode, plan,
- *         cinematic, device, memoryBudgetBytes, jacobiSweepsPerLaunch,
- *         captureGraphs, jacobiAutotune, jacobiTileHeight, particleCount}) */
-static napi_value js_create(napi_env env, napi_callback_info info) {
-  napi_value argv[1];
-  if (!get_args(env, info, 1, argv)) return NULL;
+/* Options object -> firex_config: {tier, dims:[x,y,z], pressureIterations,
+ * correctionIterations, timeStep, maximumSubsteps, speciesStorage,
+ * heavyMode, plan, cinematic, device, memoryBudgetBytes,
+ * jacobiSweepsPerLaunch, captureGraphs, jacobiAutotune, jacobiTileHeight,
+ * jacobiPrefetch, particleCount, raySteps}.  Returns 0 after throwing. */
+static int parse_config(napi_env env, napi_value obj, firex_config* cfg) {
   napi_valuetype t;
-  napi_typeof(env, argv[0], &t);
-  firex_config cfg;
-  firex_config_init(&cfg);
+  napi_typeof(env, obj, &t);
+  firex_config_init(cfg);
   char buf[64];
-  if (t == napi_object) {
-    if (str_prop(env, argv[0], "tier", buf, sizeof buf)) FX_CHECK(env, firex_config_from_tier(&cfg, buf));
-    bool has = false;
-    napi_has_named_property(env, argv[0], "dims", &has);
-    if (has) {
-      napi_value dims;
-      napi_get_named_property(env, argv[0], "dims", &dims);
+  if (t != napi_object) return 1;
+  if (str_prop(env, obj, "tier", buf, sizeof buf)) {
+    int rc = firex_config_from_tier(cfg, buf);
+    if (rc != FIREX_OK) {
+      throw_status(env, rc);
+      return 0;
+    }
+  }
+  bool has = false;
+  napi_has_named_property(env, obj, "dims", &has);
+  if (has) {
+    napi_value dims;
+    napi_get_named_property(env, obj, "dims", &dims);
+    napi_valuetype dt;
+    napi_typeof(env, dims, &dt);
+    if (dt != napi_null && dt != napi_undefined) {
       for (uint32_t i = 0; i < 3; ++i) {
         napi_value e;
         double d = 0;
         if (napi_get_element(env, dims, i, &e) != napi_ok || napi_get_value_double(env, e, &d) != napi_ok) {
           napi_throw_type_error(env, NULL, "dims must be an array of three numbers");
-          return NULL;
+          return 0;
         }
-        cfg.dims[i] = (int32_t)d;
+        cfg->dims[i] = (int32_t)d;
       }
     }
-    cfg.pressure_iterations = (int32_t)num_or(env, argv[0], "pressureIterations", cfg.pressure_iterations);
-    cfg.correction_iterations = (int32_t)num_or(env, argv[0], "correctionIterations", cfg.correction_iterations);
-    cfg.time_step = num_or(env, argv[0], "timeStep", cfg.time_step);
-    cfg.maximum_substeps = (int32_t)num_or(env, argv[0], "maximumSubsteps", cfg.maximum_substeps);
-    cfg.cinematic = (int32_t)num_or(env, argv[0], "cinematic", cfg.cinematic);
-    cfg.device = (int32_t)num_or(env, argv[0], "device", cfg.device);
-    cfg.memory_budget_bytes = (uint64_t)num_or(env, argv[0], "memoryBudgetBytes", 0);
-    cfg.jacobi_sweeps_per_launch = (int32_t)num_or(env, argv[0], "jacobiSweepsPerLaunch", 0);
-    cfg.capture_graphs = (int32_t)num_or(env, argv[0], "captureGraphs", 0);
-    cfg.jacobi_autotune = (int32_t)num_or(env, argv[0], "jacobiAutotune", 0);
-    cfg.jacobi_tile_height = (int32_t)num_or(env, argv[0], "jacobiTileHeight", 0);
-    cfg.jacobi_prefetch = (int32_t)num_or(env, argv[0], "jacobiPrefetch", 0);
-    cfg.particle_count = (int32_t)num_or(env, argv[0], "particleCount", cfg.particle_count);
-    cfg.ray_steps = (int32_t)num_or(env, argv[0], "raySteps", cfg.ray_steps);
-    if (str_prop(env, argv[0], "speciesStorage", buf, sizeof buf)) cfg.species_storage = lookup(buf, storage_names, 3);
-    if (str_prop(env, argv[0], "heavyMode", buf, sizeof buf)) cfg.heavy_mode = lookup(buf, heavy_names, 4);
-    if (str_prop(env, argv[0], "plan", buf, sizeof buf)) cfg.plan = lookup(buf, plan_names, 2);
   }
+  cfg->pressure_iterations = (int32_t)num_or(env, obj, "pressureIterations", cfg->pressure_iterations);
+  cfg->correction_iterations = (int32_t)num_or(env, obj, "correctionIterations", cfg->correction_iterations);
+  cfg->time_step = num_or(env, obj, "timeStep", cfg->time_step);
+  cfg->maximum_substeps = (int32_t)num_or(env, obj, "maximumSubsteps", cfg->maximum_substeps);
+  cfg->cinematic = (int32_t)num_or(env, obj, "cinematic", cfg->cinematic);
+  cfg->device = (int32_t)num_or(env, obj, "device", cfg->device);
+  cfg->memory_budget_bytes = (uint64_t)num_or(env, obj, "memoryBudgetBytes", 0);
+  cfg->jacobi_sweeps_per_launch = (int32_t)num_or(env, obj, "jacobiSweepsPerLaunch", 0);
+  cfg->capture_graphs = (int32_t)num_or(env, obj, "captureGraphs", 0);
+  cfg->jacobi_autotune = (int32_t)num_or(env, obj, "jacobiAutotune", 0);
+  cfg->jacobi_tile_height = (int32_t)num_or(env, obj, "jacobiTileHeight", 0);
+  cfg->jacobi_prefetch = (int32_t)num_or(env, obj, "jacobiPrefetch", 0);
+  cfg->particle_count = (int32_t)num_or(env, obj, "particleCount", cfg->particle_count);
+  cfg->ray_steps = (int32_t)num_or(env, obj, "raySteps", cfg->ray_steps);
+  if (str_prop(env, obj, "speciesStorage", buf, sizeof buf)) cfg->species_storage = lookup(buf, storage_names, 3);
+  if (str_prop(env, obj, "heavyMode", buf, sizeof buf)) cfg->heavy_mode = lookup(buf, heavy_names, 4);
+  if (str_prop(env, obj, "plan", buf, sizeof buf)) cfg->plan = lookup(buf, plan_names, 2);
+  return 1;
+}
+
+/* create(options): see parse_config. */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  firex_config cfg;
+  if (!parse_config(env, argv[0], &cfg)) return NULL;
   firex_solver* s = NULL;
   FX_CHECK(env, firex_create(&cfg, &s));
   handle_t* h = (handle_t*)malloc(sizeof *h);
@@ -175,6 +191,55 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   napi_value ext;
   NAPI_CALL(env, napi_create_external(env, h, finalize_handle, NULL, &ext));
   return ext;
+}
+
+/* reconfigure(h, options, guardBytes) -> true when the buffer set was rebuilt */
+static napi_value js_reconfigure(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  firex_solver* s = get_solver(env, argv[0]);
+  if (!s) return NULL;
+  firex_config cfg;
+  if (!parse_config(env, argv[1], &cfg)) return NULL;
+  double guard = 0;
+  napi_valuetype t;
+  napi_typeof(env, argv[2], &t);
+  if (t == napi_number) napi_get_value_double(env, argv[2], &guard);
+  int32_t rebuilt = 0;
+  FX_CHECK(env, firex_reconfigure(s, &cfg, guard > 0 ? (uint64_t)guard : 0, &rebuilt));
+  napi_value out;
+  NAPI_CALL(env, napi_get_boolean(env, rebuilt != 0, &out));
+  return out;
+}
+
+static napi_value dims_array(napi_env env, const int32_t* d);
+
+/* configFromTier(name) -> the tier's solver configuration (FIREX_QUALITY /
+ * FIREX_OFFLINE_QUALITY, FireXEngine.ts:187-277) */
+static napi_value js_config_from_tier(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  char tier[32];
+  if (!str_arg(env, argv[0], tier, sizeof tier)) {
+    napi_throw_type_error(env, NULL, "tier must be a string");
+    return NULL;
+  }
+  firex_config cfg;
+  firex_config_init(&cfg);
+  FX_CHECK(env, firex_config_from_tier(&cfg, tier));
+  napi_value out, v;
+  NAPI_CALL(env, napi_create_object(env, &out));
+  NAPI_CALL(env, napi_set_named_property(env, out, "dimensions", dims_array(env, cfg.dims)));
+  const struct { const char* key; double value; } fields[] = {
+      {"pressureIterations", cfg.pressure_iterations}, {"correctionIterations", cfg.correction_iterations},
+      {"timeStep", cfg.time_step}, {"maximumSubsteps", cfg.maximum_substeps},
+      {"particleCount", cfg.particle_count}, {"cinematic", cfg.cinematic}, {"raySteps", cfg.ray_steps},
+  };
+  for (size_t i = 0; i < sizeof fields / sizeof fields[0]; ++i) {
+    NAPI_CALL(env, napi_create_double(env, fields[i].value, &v));
+    NAPI_CALL(env, napi_set_named_property(env, out, fields[i].key, v));
+  }
+  return out;
 }
 
 static napi_value js_destroy(napi_env env, napi_callback_info info) {
@@ -678,6 +743,8 @@ static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
       {"destroy", NULL, js_destroy, NULL, NULL, NULL, napi_default, NULL},
+      {"reconfigure", NULL, js_reconfigure, NULL, NULL, NULL, napi_default, NULL},
+      {"configFromTier", NULL, js_config_from_tier, NULL, NULL, NULL, napi_default, NULL},
       {"reset", NULL, js_reset, NULL, NULL, NULL, napi_default, NULL},
       {"setParameter", NULL, js_set_parameter, NULL, NULL, NULL, napi_default, NULL},
       {"getParameter", NULL, js_get_parameter, NULL, NULL, NULL, napi_default, NULL},

@@ -182,6 +182,7 @@ _SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_int32)]),
     "firex_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
     "firex_destroy": (None, [_P]),
+    "firex_reconfigure": (ctypes.c_int, [_P, ctypes.POINTER(Config), ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)]),
     "firex_reset": (ctypes.c_int, [_P]),
     "firex_set_parameter": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_double]),
     "firex_set_parameter_bool": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
@@ -390,6 +391,32 @@ class FireXHipSolver:
         handle = ctypes.c_void_p()
         _check(self._lib.firex_create(ctypes.byref(cfg), ctypes.byref(handle)))
         self._h = handle
+
+    def reconfigure(self, memory_guard_bytes: int = 0, **changes) -> bool:
+        """Transactional reconfiguration (firex_reconfigure): `changes` are
+        firex_config fields (dims, pressure_iterations, correction_iterations,
+        time_step, maximum_substeps, particle_count, cinematic, ray_steps,
+        species_storage, ...).  Returns True when the buffer set was rebuilt
+        (fields reset), False when only per-step settings changed.  On any
+        error the solver is left as it was."""
+        cfg = Config.from_buffer_copy(self.config)
+        for key, value in changes.items():
+            if key == "dims":
+                for i in range(3):
+                    cfg.dims[i] = int(value[i])
+            elif key == "species_storage":
+                cfg.species_storage = STORAGE[value]
+            elif key == "heavy_mode":
+                cfg.heavy_mode = HEAVY[value]
+            else:
+                setattr(cfg, key, type(getattr(cfg, key))(value))
+        rebuilt = ctypes.c_int32()
+        _check(self._lib.firex_reconfigure(self._h, ctypes.byref(cfg), int(memory_guard_bytes),
+                                           ctypes.byref(rebuilt)))
+        self.config = cfg
+        self.dims = tuple(int(v) for v in cfg.dims)
+        self.cells = self.dims[0] * self.dims[1] * self.dims[2]
+        return bool(rebuilt.value)
 
     # lifetime ---------------------------------------------------------------
     def close(self) -> None:

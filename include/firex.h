@@ -244,6 +244,23 @@ int firex_resolve_offline_dims(const char* tier, const int32_t* override_dims, i
  * FireXEngine.ts:1536-1974).  Fails without partial state. */
 int firex_create(const firex_config* config, firex_solver** out);
 void firex_destroy(firex_solver* solver);
+/* Transactional reconfiguration of a live solver: the resource half of
+ * setGridDimensions, setOfflineRenderTier and setQuality
+ * (FireXEngine.ts:799-831, 863-927) through rebuildResourcesSafely
+ * (:1891-1915).  If the new configuration keeps the buffer set (same grid,
+ * particle count, species storage, pressure path, plan and Jacobi settings),
+ * only the per-step settings change (iterations, time step and substeps,
+ * cinematic flag, ray steps, budget), the fields are kept and *rebuilt = 0.
+ * Otherwise the estimated bytes of the current and the replacement set
+ * (estimateFireXGridMemory of each) must fit memory_guard_bytes (0 = no
+ * guard; FIREX_ERR_BUDGET with the reference's message, assertGridSupported
+ * :2823-2853), the replacement is built while the current set stays live,
+ * installed with installResourceSet's reset (:1917-1931: parameters kept;
+ * clock, time, indices and fields started again) and only then is the old set
+ * freed; *rebuilt = 1.  On any failure the solver is left exactly as it was.
+ * The device cannot change; plumes on a serving stream cannot be rebuilt. */
+int firex_reconfigure(firex_solver* solver, const firex_config* config, uint64_t memory_guard_bytes,
+                      int32_t* rebuilt);
 /* reset(), FireXEngine.ts:843-852. */
 int firex_reset(firex_solver* solver);
 
