@@ -126,116 +126,107 @@ function parseList(value, allowed, flag) {
   return [...new Set(items)];
 }
 
-function parseArguments(argv) {
-  const options = {
+function defaultOptions() {
+  return {
     scene: 'inferno', field: 'beauty', solverTier: 'hd', grid: null, exactDims: null, durationSeconds: 1, warmupSeconds: 0,
     dumpEvery: 0, dumpFields: [...DUMP_FIELDS], auditFields: [...AUDIT_FIELDS], plan: 'fused',
     speciesStorage: 'f32', particleCount: null, gridMemoryBudgetBytes: 0, output: null, outputDir: 'captures',
     overwrite: false, help: false, frameEvery: null, outputPreset: 'hd', camera: {}, opticalDetailTarget: 1024,
   };
-  for (let index = 0; index < argv.length; index += 1) {
-    const raw = argv[index];
-    const equals = raw.indexOf('=');
-    const flag = equals >= 0 ? raw.slice(0, equals) : raw;
-    const inline = equals >= 0 ? raw.slice(equals + 1) : null;
-    const value = () => {
-      if (inline !== null) return inline;
-      const next = argv[index + 1];
-      if (next === undefined || next.startsWith('--')) throw new Error(`${flag} requires a value.`);
-      index += 1;
-      return next;
-    };
-    switch (flag) {
-      case '--help': case '-h': options.help = true; break;
-      case '--scene': {
-        const scene = value().toLowerCase();
-        if (!(scene in SCENES)) throw new RangeError(`Unknown scene "${scene}".`);
-        options.scene = scene;
-        break;
-      }
-      case '--field': {
-        const view = value().toLowerCase();
-        if (!(view in FIELD_VIEWS)) throw new RangeError(`Unknown field view "${view}".`);
-        options.field = view;
-        break;
-      }
-      case '--solver': case '--solver-tier': {
-        const tier = value().toLowerCase();
-        if (!SOLVER_TIERS.includes(tier)) throw new RangeError(`Unknown solver tier "${tier}".`);
-        options.solverTier = tier;
-        break;
-      }
-      case '--grid': case '--dims': {
-        const dims = value().toLowerCase().split('x').map(Number);
-        if (dims.length !== 3 || !dims.every((d) => Number.isInteger(d) && d >= 2)) {
-          throw new RangeError(`${flag} must be XxYxZ with integers of at least 2.`);
-        }
-        if (flag === '--grid') options.grid = dims; else options.exactDims = dims;
-        break;
-      }
-      case '--duration': options.durationSeconds = Number(value()); frameCount(options.durationSeconds, flag); break;
-      case '--warmup': options.warmupSeconds = Number(value()); frameCount(options.warmupSeconds, flag); break;
-      case '--dump-every': {
-        const n = Number(value());
-        if (!Number.isInteger(n) || n < 0) throw new RangeError('--dump-every must be a non-negative integer.');
-        options.dumpEvery = n;
-        break;
-      }
-      case '--dump-fields': options.dumpFields = parseList(value(), FIELD_IDS, flag); break;
-      case '--audit-fields': options.auditFields = parseList(value(), FIELD_IDS, flag); break;
-      case '--plan': {
-        const plan = value();
-        if (plan !== 'fused' && plan !== 'reference') throw new RangeError('--plan must be fused or reference.');
-        options.plan = plan;
-        break;
-      }
-      case '--species-storage': {
-        const storage = value();
-        if (!['f32', 'f16', 'bf16'].includes(storage)) throw new RangeError('--species-storage must be f32, f16 or bf16.');
-        options.speciesStorage = storage;
-        break;
-      }
-      case '--particles': {
-        const n = Number(value());
-        if (!Number.isInteger(n) || n < 0) throw new RangeError('--particles must be a non-negative integer.');
-        options.particleCount = n;
-        break;
-      }
-      case '--memory-budget': {
-        const n = Number(value());
-        if (!Number.isFinite(n) || n < 0) throw new RangeError('--memory-budget must be a byte count.');
-        options.gridMemoryBudgetBytes = n;
-        break;
-      }
-      case '--frame-every': {
-        const n = Number(value());
-        if (!Number.isInteger(n) || n < 0) throw new RangeError('--frame-every must be a non-negative integer.');
-        options.frameEvery = n;
-        break;
-      }
-      case '--resolution': case '--preset': {
-        const preset = OUTPUT_ALIASES[value().toLowerCase()];
-        if (!preset) throw new Error(`Unknown output resolution for ${flag}.`);
-        options.outputPreset = preset;
-        break;
-      }
-      case '--camera-yaw': case '--camera-pitch': case '--camera-distance': {
-        const v = Number(value());
-        if (!Number.isFinite(v)) throw new RangeError(`${flag} must be a finite number.`);
-        options.camera[{ '--camera-yaw': 'cameraYaw', '--camera-pitch': 'cameraPitch', '--camera-distance': 'cameraDistance' }[flag]] = v;
-        break;
-      }
-      case '--optical-detail': {
-        const n = Number(value());
-        if (![0, 256, 512, 1024].includes(n)) throw new Error('--optical-detail must be 0, 256, 512, or 1024.');
-        options.opticalDetailTarget = n;
-        break;
-      }
-      case '--output': options.output = value(); break;
-      case '--output-dir': options.outputDir = value(); break;
-      case '--overwrite': options.overwrite = true; break;
-      default: throw new Error(`Unknown option ${flag}.\n${usage()}`);
+}
+
+function nonNegativeInteger(text, name) {
+  const n = Number(text);
+  if (!Number.isInteger(n) || n < 0) throw new RangeError(`${name} must be a non-negative integer.`);
+  return n;
+}
+
+function oneOf(text, allowed, message) {
+  if (!allowed.includes(text)) throw new RangeError(message);
+  return text;
+}
+
+// The options: each entry names its spellings, whether it reads a value, and
+// how that value updates the options.
+const OPTION_SPECS = [
+  [['--help', '-h'], false, (o) => { o.help = true; }],
+  [['--scene'], true, (o, text) => {
+    o.scene = text.toLowerCase();
+    if (!(o.scene in SCENES)) throw new RangeError(`Unknown scene "${o.scene}".`);
+  }],
+  [['--field'], true, (o, text) => {
+    o.field = text.toLowerCase();
+    if (!(o.field in FIELD_VIEWS)) throw new RangeError(`Unknown field view "${o.field}".`);
+  }],
+  [['--solver', '--solver-tier'], true, (o, text) => {
+    o.solverTier = oneOf(text.toLowerCase(), SOLVER_TIERS, `Unknown solver tier "${text.toLowerCase()}".`);
+  }],
+  [['--grid', '--dims'], true, (o, text, name) => {
+    const dims = text.toLowerCase().split('x').map(Number);
+    if (dims.length !== 3 || dims.some((d) => !Number.isInteger(d) || d < 2)) {
+      throw new RangeError(`${name} must be XxYxZ with integers of at least 2.`);
     }
+    o[name === '--grid' ? 'grid' : 'exactDims'] = dims;
+  }],
+  [['--duration'], true, (o, text, name) => { o.durationSeconds = Number(text); frameCount(o.durationSeconds, name); }],
+  [['--warmup'], true, (o, text, name) => { o.warmupSeconds = Number(text); frameCount(o.warmupSeconds, name); }],
+  [['--dump-every'], true, (o, text, name) => { o.dumpEvery = nonNegativeInteger(text, name); }],
+  [['--dump-fields'], true, (o, text, name) => { o.dumpFields = parseList(text, FIELD_IDS, name); }],
+  [['--audit-fields'], true, (o, text, name) => { o.auditFields = parseList(text, FIELD_IDS, name); }],
+  [['--plan'], true, (o, text) => { o.plan = oneOf(text, ['fused', 'reference'], '--plan must be fused or reference.'); }],
+  [['--species-storage'], true, (o, text) => {
+    o.speciesStorage = oneOf(text, ['f32', 'f16', 'bf16'], '--species-storage must be f32, f16 or bf16.');
+  }],
+  [['--particles'], true, (o, text, name) => { o.particleCount = nonNegativeInteger(text, name); }],
+  [['--memory-budget'], true, (o, text) => {
+    const n = Number(text);
+    if (!(n >= 0) || !Number.isFinite(n)) throw new RangeError('--memory-budget must be a byte count.');
+    o.gridMemoryBudgetBytes = n;
+  }],
+  [['--frame-every'], true, (o, text, name) => { o.frameEvery = nonNegativeInteger(text, name); }],
+  [['--resolution', '--preset'], true, (o, text, name) => {
+    const preset = OUTPUT_ALIASES[text.toLowerCase()];
+    if (!preset) throw new Error(`Unknown output resolution for ${name}.`);
+    o.outputPreset = preset;
+  }],
+  [['--camera-yaw', '--camera-pitch', '--camera-distance'], true, (o, text, name) => {
+    const v = Number(text);
+    if (!Number.isFinite(v)) throw new RangeError(`${name} must be a finite number.`);
+    const key = `camera${name.slice('--camera-'.length)[0].toUpperCase()}${name.slice('--camera-'.length + 1)}`;
+    o.camera[key] = v;
+  }],
+  [['--optical-detail'], true, (o, text) => {
+    const n = Number(text);
+    if (![0, 256, 512, 1024].includes(n)) throw new Error('--optical-detail must be 0, 256, 512, or 1024.');
+    o.opticalDetailTarget = n;
+  }],
+  [['--output'], true, (o, text) => { o.output = text; }],
+  [['--output-dir'], true, (o, text) => { o.outputDir = text; }],
+  [['--overwrite'], false, (o) => { o.overwrite = true; }],
+];
+const OPTIONS = new Map();
+for (const [names, takesValue, apply] of OPTION_SPECS) for (const name of names) OPTIONS.set(name, { takesValue, apply });
+
+// Tokens are "--name", "--name=value" or a value following a name that takes
+// one; a value may not itself look like an option.
+function parseArguments(argv) {
+  const options = defaultOptions();
+  const tokens = [...argv];
+  while (tokens.length > 0) {
+    const token = tokens.shift();
+    const match = /^(--[^=]+)=([^]*)$/.exec(token);
+    const name = match ? match[1] : token;
+    const spec = OPTIONS.get(name);
+    if (!spec) throw new Error(`Unknown option ${name}.\n${usage()}`);
+    let text = null;
+    if (spec.takesValue) {
+      if (match) text = match[2];
+      else if (tokens.length > 0 && !tokens[0].startsWith('--')) text = tokens.shift();
+      else throw new Error(`Option ${name} needs a value.`);
+    } else if (match) {
+      throw new Error(`Option ${name} takes no value.`);
+    }
+    spec.apply(options, text, name);
   }
   return options;
 }
