@@ -469,13 +469,26 @@ def main() -> int:
                              "filmic composite into an rgba8 frame; the reference's also draws droplet sprites "
                              "(none in this dry plume); its hardware is unspecified"}
         ls.close()
+        # the configuration as stated: FP32 pressure (the compact heavy path,
+        # FireXEngine.ts:2214-2246: shared pressure fields, cold start)
+        print("bench: 508^3 plume with FP32 pressure", file=sys.stderr, flush=True)
+        fs32 = make_solver(lw, "fused", autotune=True, heavy="f32")
+        finfo = fs32.info
+        fel = time_steps(fs32, args.large_steps, 2)
+        fpasses = pass_profile(fs32, 2, lcells, lw["storage"], f16_pressure=False)
+        fpasses.pop("_raw")
+        fs32.close()
+        fp32_pressure = {"pressure": "f32 (compact heavy path)", "ms_per_step": fel / args.large_steps * 1e3,
+                         "cell_updates_per_s": lcells * args.large_steps / fel, "jacobi_plan": finfo["jacobi"],
+                         "passes": {k: dict(v, frac_of_peak=round(v["algorithmic_GBps"] / PEAK_HBM_GBS, 4))
+                                    if isinstance(v, dict) else v for k, v in fpasses.items()}}
         large = {"grid": list(lw["dims"]), "cells": lcells, "steps": args.large_steps, "jacobi": list(lw["iters"]),
                  "species_storage": lw["storage"], "pressure": "f16" if linfo["f16_pressure"] else "f32",
                  "velocity_noise": lw["velocity_noise"], "ms_per_step": lel / args.large_steps * 1e3,
                  "cell_updates_per_s": lcells * args.large_steps / lel, "jacobi_plan": linfo["jacobi"],
                  "passes": {k: dict(v, frac_of_peak=round(v["algorithmic_GBps"] / PEAK_HBM_GBS, 4))
                             if isinstance(v, dict) else v for k, v in lpasses.items()},
-                 "step_plus_4k_frame": frame}
+                 "step_plus_4k_frame": frame, "fp32_pressure": fp32_pressure}
 
     droplets = None
     if args.droplet_steps > 0:

@@ -18,13 +18,20 @@ struct JacobiPlan {
   int32_t kernel = JACOBI_KERNEL_SHIFT;  // SHIFT: any grid, K <= 8; RING: nx >= 64, ny >= W * R, K <= 5
   int32_t prefetch = 0;           // RING: load plane t + 1 one iteration before its first use
 };
+// RING block shapes: 8 waves of 4, 6 or 8 rows (32-, 48- or 64-row tiles,
+// K <= 5), or 16 waves of 2 rows (32-row tiles, K <= 4, no prefetch: fewer
+// registers per thread, twice the waves per CU).
+constexpr int kRingWideWaves = 16;
+constexpr int kRingWideMaxSweeps = 4;
 
 // kernel: JACOBI_KERNEL_* (AUTO picks RING where the grid allows it);
 // sweeps_per_launch, chunk_z and tile_height (RING: 32, 48 or 64 rows per
 // tile): 0 = default; prefetch (RING, K <= 3): 0 = default (off), 1 = off,
 // 2 = on.
+// ring_waves (RING): 0 or 8 = 8 waves per block, 16 = the 16-wave shape.
 JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z = 0,
-                            int32_t kernel = JACOBI_KERNEL_AUTO, int32_t tile_height = 0, int32_t prefetch = 0);
+                            int32_t kernel = JACOBI_KERNEL_AUTO, int32_t tile_height = 0, int32_t prefetch = 0,
+                            int32_t ring_waves = 0);
 
 // One launch of the temporally blocked kernel: `sweeps` sweeps from
 // buffers[src] (or from the zero field, without reading it) into
@@ -54,7 +61,7 @@ bool jacobi_plan_valid(const JacobiPlan& plan);
 // device, shape, storage and fixed parameters, so a second solver of the
 // same shape does not time again.  Returns a hipError_t (0 on success).
 int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
-                     int32_t tile_height, int32_t prefetch, void* const buffers[2], float* div, hipStream_t s,
-                     JacobiPlan* out, float* us_per_sweep);
+                     int32_t tile_height, int32_t prefetch, int32_t ring_waves, void* const buffers[2], float* div,
+                     hipStream_t s, JacobiPlan* out, float* us_per_sweep);
 
 }  // namespace fx

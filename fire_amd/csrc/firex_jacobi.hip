@@ -523,6 +523,16 @@ void launch_ring_pf(int k, const JacobiPlan& plan, const void* pin, void* pout, 
 template <int P>
 void launch_ring_k(int k, const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
                    int zero_start, hipStream_t s) {
+  if (plan.waves == kRingWideWaves) {
+    // 16 waves of 2 rows (K <= kRingWideMaxSweeps, jacobi_plan_valid)
+    switch (k) {
+      case 1: launch_ring<P, 1, 2, kRingWideWaves, 0>(plan, pin, pout, div, st, zero_start, s); break;
+      case 2: launch_ring<P, 2, 2, kRingWideWaves, 0>(plan, pin, pout, div, st, zero_start, s); break;
+      case 3: launch_ring<P, 3, 2, kRingWideWaves, 0>(plan, pin, pout, div, st, zero_start, s); break;
+      default: launch_ring<P, 4, 2, kRingWideWaves, 0>(plan, pin, pout, div, st, zero_start, s); break;
+    }
+    return;
+  }
   if (plan.rows == 8)
     launch_ring_pf<P, 8>(k, plan, pin, pout, div, st, zero_start, s);
   else if (plan.rows == 6)
@@ -565,9 +575,11 @@ void launch_tb_k(int k, const JacobiPlan& plan, const void* pin, void* pout, con
 
 bool jacobi_plan_valid(const JacobiPlan& plan) {
   const int max_k = plan.kernel == JACOBI_KERNEL_RING ? kRingMaxSweeps : 8;
+  const bool wide = plan.kernel == JACOBI_KERNEL_RING && plan.waves == kRingWideWaves && plan.rows == 2 &&
+                    plan.prefetch == 0 && plan.sweeps_per_launch <= kRingWideMaxSweeps;
   const bool shape = plan.kernel == JACOBI_KERNEL_RING
-      ? (plan.rows == 4 || plan.rows == 6 || plan.rows == 8) && plan.waves == 8 &&
-            (plan.prefetch == 0 || (plan.prefetch == 1 && plan.sweeps_per_launch <= 3))
+      ? wide || ((plan.rows == 4 || plan.rows == 6 || plan.rows == 8) && plan.waves == 8 &&
+                 (plan.prefetch == 0 || (plan.prefetch == 1 && plan.sweeps_per_launch <= 3)))
       : plan.rows == 4 && plan.waves == 8;
   return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= max_k && shape && plan.chunk_z >= 1 &&
          (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);
@@ -582,7 +594,7 @@ int jacobi_plan_blocks(const JacobiPlan& p, const int32_t dims[3]) {
 }
 
 JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
-                            int32_t tile_height, int32_t prefetch) {
+                            int32_t tile_height, int32_t prefetch, int32_t ring_waves) {
   JacobiPlan p;
   // the ring kernel's tile height (8 waves of 4, 6 or 8 rows); the shift
   // kernel keeps 32
@@ -596,12 +608,17 @@ JacobiPlan make_jacobi_plan(const int32_t dims[3], int32_t sweeps_per_launch, in
            : kernel == JACOBI_KERNEL_SHIFT ? JACOBI_KERNEL_SHIFT
            : ring_ok ? JACOBI_KERNEL_RING : JACOBI_KERNEL_SHIFT;
   if (p.kernel != JACOBI_KERNEL_RING) p.rows = 4;
+  if (p.kernel == JACOBI_KERNEL_RING && ring_waves == kRingWideWaves && p.rows == 4) {
+    p.rows = 2;  // the same 32-row tile as 16 waves of 2 rows
+    p.waves = kRingWideWaves;
+  }
   // the ring kernel's mirrored z halo needs nz >= K + 2
   const int max_k = p.kernel == JACOBI_KERNEL_RING ? (dims[2] - 2 < kRingMaxSweeps ? dims[2] - 2 : kRingMaxSweeps) : 8;
   const int dflt = p.kernel == JACOBI_KERNEL_RING ? 3 : 4;
   const int k = sweeps_per_launch > 0 ? sweeps_per_launch : dflt;
   p.sweeps_per_launch = k > max_k ? max_k : k;
-  p.prefetch = p.kernel == JACOBI_KERNEL_RING && prefetch == 2 && p.sweeps_per_launch <= 3 ? 1 : 0;
+  if (p.waves == kRingWideWaves && p.sweeps_per_launch > kRingWideMaxSweeps) p.sweeps_per_launch = kRingWideMaxSweeps;
+  p.prefetch = p.kernel == JACOBI_KERNEL_RING && prefetch == 2 && p.sweeps_per_launch <= 3 && p.waves == 8 ? 1 : 0;
   if (chunk_z > 0) {
     p.chunk_z = chunk_z;
     return p;
@@ -646,8 +663,8 @@ __global__ __launch_bounds__(256) void k_tune_pattern(float* __restrict__ div, u
   }
 }
 
-// device, dims, f16, K, chunk, kernel, tile height, prefetch
-using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int, int>;
+// device, dims, f16, K, chunk, kernel, tile height, prefetch, ring waves
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
 struct TuneEntry {
   JacobiPlan plan;
   float us_per_sweep;
@@ -673,12 +690,13 @@ std::vector<int> chunk_candidates(int nz, long tiles) {
 }  // namespace
 
 int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, int32_t chunk_z, int32_t kernel,
-                     int32_t tile_height, int32_t prefetch, void* const buffers[2], float* div, hipStream_t s,
-                     JacobiPlan* out, float* us_per_sweep) {
+                     int32_t tile_height, int32_t prefetch, int32_t ring_waves, void* const buffers[2], float* div,
+                     hipStream_t s, JacobiPlan* out, float* us_per_sweep) {
   int device = 0;
   hipError_t e = hipGetDevice(&device);
   if (e != hipSuccess) return e;
-  const TuneKey key{device, dims[0], dims[1], dims[2], f16, sweeps_per_launch, chunk_z, kernel, tile_height, prefetch};
+  const TuneKey key{device, dims[0], dims[1], dims[2], f16, sweeps_per_launch, chunk_z, kernel, tile_height,
+                    prefetch, ring_waves};
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     auto it = g_tune_cache.find(key);
@@ -691,7 +709,7 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
   // Candidates: the default plan first, then every kernel / depth / chunk /
   // prefetch combination the fixed parameters allow.
   std::vector<JacobiPlan> cands;
-  cands.push_back(make_jacobi_plan(dims, sweeps_per_launch, chunk_z, kernel, tile_height, prefetch));
+  cands.push_back(make_jacobi_plan(dims, sweeps_per_launch, chunk_z, kernel, tile_height, prefetch, ring_waves));
   const int kernels[2] = {JACOBI_KERNEL_RING, JACOBI_KERNEL_SHIFT};
   for (int kern : kernels) {
     if (kernel != JACOBI_KERNEL_AUTO && kernel != kern) continue;
@@ -708,18 +726,23 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
       // registers and give fewer blocks
       for (int height : {32, 48, 64}) {
         if (tile_height > 0 ? height != tile_height : height != 32) continue;
-        JacobiPlan base = make_jacobi_plan(dims, K, 1, kern, height);
-        if (base.sweeps_per_launch != K || base.rows * base.waves != height) continue;
-        base.chunk_z = 1;
-        const long tiles = jacobi_plan_blocks(base, dims) / dims[2];
-        std::vector<int> chunks = chunk_z > 0 ? std::vector<int>{chunk_z} : chunk_candidates(dims[2], tiles);
-        for (int c : chunks) {
-          for (int pf = 0; pf <= (kern == JACOBI_KERNEL_RING && K <= 3 ? 1 : 0); ++pf) {
-            if (prefetch > 0 && pf != prefetch - 1) continue;
-            JacobiPlan p = base;
-            p.chunk_z = c;
-            p.prefetch = pf;
-            if (jacobi_plan_valid(p)) cands.push_back(p);
+        // block shapes of a 32-row ring tile: 8 waves of 4 rows or 16 of 2
+        for (int waves : {8, kRingWideWaves}) {
+          if (waves == kRingWideWaves && (kern != JACOBI_KERNEL_RING || height != 32)) continue;
+          if (ring_waves > 0 && waves != ring_waves) continue;
+          JacobiPlan base = make_jacobi_plan(dims, K, 1, kern, height, 0, waves);
+          if (base.sweeps_per_launch != K || base.rows * base.waves != height || base.waves != waves) continue;
+          base.chunk_z = 1;
+          const long tiles = jacobi_plan_blocks(base, dims) / dims[2];
+          std::vector<int> chunks = chunk_z > 0 ? std::vector<int>{chunk_z} : chunk_candidates(dims[2], tiles);
+          for (int c : chunks) {
+            for (int pf = 0; pf <= (kern == JACOBI_KERNEL_RING && K <= 3 && waves == 8 ? 1 : 0); ++pf) {
+              if (prefetch > 0 && pf != prefetch - 1) continue;
+              JacobiPlan p = base;
+              p.chunk_z = c;
+              p.prefetch = pf;
+              if (jacobi_plan_valid(p)) cands.push_back(p);
+            }
           }
         }
       }
@@ -778,7 +801,7 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
         bool seen = false;
         for (const JacobiPlan& q : cands)
           seen = seen || (q.kernel == b.kernel && q.sweeps_per_launch == b.sweeps_per_launch &&
-                          q.prefetch == b.prefetch && q.rows == b.rows && q.chunk_z == c);
+                          q.prefetch == b.prefetch && q.rows == b.rows && q.waves == b.waves && q.chunk_z == c);
         if (seen) continue;
         JacobiPlan p = b;
         p.chunk_z = c;

@@ -632,6 +632,10 @@ int solver_plan(const firex_config* cfg, bool* heavy_out, bool* f16p_out, bool* 
   if (cfg->jacobi_tile_height != 0 && cfg->jacobi_tile_height != 32 && cfg->jacobi_tile_height != 48 &&
       cfg->jacobi_tile_height != 64)
     return set_error(FIREX_ERR_INVALID, "jacobi tile height must be 0, 32, 48 or 64");
+  if (cfg->jacobi_ring_waves != 0 && cfg->jacobi_ring_waves != 8 && cfg->jacobi_ring_waves != kRingWideWaves)
+    return set_error(FIREX_ERR_INVALID, "jacobi ring waves must be 0 (tuned), 8 or 16");
+  if (cfg->jacobi_ring_waves == kRingWideWaves && cfg->jacobi_tile_height != 0 && cfg->jacobi_tile_height != 32)
+    return set_error(FIREX_ERR_INVALID, "16 ring waves make 32-row tiles only");
   if (cfg->jacobi_prefetch < 0 || cfg->jacobi_prefetch > 2)
     return set_error(FIREX_ERR_INVALID, "jacobi prefetch must be 0 (tuned), 1 (off) or 2 (on)");
 
@@ -730,12 +734,13 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
       // fields, which initialize_fields() clears afterwards.
       void* bufs[2] = {s->pressure[0], s->pressure[1]};
       e = (hipError_t)tune_jacobi_plan(s->dims, f16p ? 1 : 0, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z,
-                                       cfg->jacobi_kernel, cfg->jacobi_tile_height, cfg->jacobi_prefetch, bufs,
+                                       cfg->jacobi_kernel, cfg->jacobi_tile_height, cfg->jacobi_prefetch,
+                                       cfg->jacobi_ring_waves, bufs,
                                        s->divergence, s->stream, &s->jacobi, &s->jacobi_tuned_us);
       if (e != hipSuccess) rc = hip_fail(s, e, "Jacobi plan autotuning");
     } else {
       s->jacobi = make_jacobi_plan(s->dims, cfg->jacobi_sweeps_per_launch, cfg->jacobi_chunk_z, cfg->jacobi_kernel,
-                                   cfg->jacobi_tile_height, cfg->jacobi_prefetch);
+                                   cfg->jacobi_tile_height, cfg->jacobi_prefetch, cfg->jacobi_ring_waves);
     }
     if (!rc) rc = initialize_fields(s);
   }
@@ -831,7 +836,8 @@ bool same_buffer_set(const firex_solver* s, const firex_config& c, bool heavy, b
          f16p == s->f16p && compact == s->compact && c.plan == o.plan &&
          c.jacobi_sweeps_per_launch == o.jacobi_sweeps_per_launch && c.jacobi_chunk_z == o.jacobi_chunk_z &&
          c.jacobi_kernel == o.jacobi_kernel && c.jacobi_autotune == o.jacobi_autotune &&
-         c.jacobi_tile_height == o.jacobi_tile_height && c.jacobi_prefetch == o.jacobi_prefetch;
+         c.jacobi_tile_height == o.jacobi_tile_height && c.jacobi_prefetch == o.jacobi_prefetch &&
+         c.jacobi_ring_waves == o.jacobi_ring_waves;
 }
 
 }  // namespace
@@ -1473,6 +1479,7 @@ int firex_get_info(firex_solver* s, firex_info* info) {
   info->jacobi_prefetch = s->jacobi.prefetch;
   info->jacobi_tuned_us_per_sweep = s->jacobi_tuned_us;
   info->jacobi_tile_height = s->jacobi.rows * s->jacobi.waves;
+  info->jacobi_ring_waves = s->jacobi.waves;
   info->particle_count = s->cfg.particle_count;
   return FIREX_OK;
 }

@@ -170,6 +170,7 @@ static int parse_config(napi_env env, napi_value obj, firex_config* cfg) {
   cfg->jacobi_autotune = (int32_t)num_or(env, obj, "jacobiAutotune", 0);
   cfg->jacobi_tile_height = (int32_t)num_or(env, obj, "jacobiTileHeight", 0);
   cfg->jacobi_prefetch = (int32_t)num_or(env, obj, "jacobiPrefetch", 0);
+  cfg->jacobi_ring_waves = (int32_t)num_or(env, obj, "jacobiRingWaves", 0);
   cfg->particle_count = (int32_t)num_or(env, obj, "particleCount", cfg->particle_count);
   cfg->ray_steps = (int32_t)num_or(env, obj, "raySteps", cfg->ray_steps);
   if (str_prop(env, obj, "speciesStorage", buf, sizeof buf)) cfg->species_storage = lookup(buf, storage_names, 3);

@@ -94,7 +94,7 @@ class Config(ctypes.Structure):
         ("ray_steps", ctypes.c_int32),
         ("jacobi_tile_height", ctypes.c_int32),
         ("jacobi_prefetch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("jacobi_ring_waves", ctypes.c_int32),
     ]
 
 
@@ -122,6 +122,7 @@ class Info(ctypes.Structure):
         ("jacobi_tuned_us_per_sweep", ctypes.c_float),
         ("particle_count", ctypes.c_int32),
         ("jacobi_tile_height", ctypes.c_int32),
+        ("jacobi_ring_waves", ctypes.c_int32),
     ]
 
 
@@ -353,7 +354,7 @@ class FireXHipSolver:
                  jacobi_sweeps_per_launch: int = 0, capture_graphs: bool = False, jacobi_chunk_z: int = 0,
                  jacobi_kernel: str = "auto", jacobi_autotune: bool = False,
                  particle_count: Optional[int] = None, ray_steps: Optional[int] = None,
-                 jacobi_tile_height: int = 0, jacobi_prefetch: Optional[bool] = None):
+                 jacobi_tile_height: int = 0, jacobi_prefetch: Optional[bool] = None, jacobi_ring_waves: int = 0):
         self._lib = load_library()
         cfg = config_from_tier(tier)
         if dims is not None:
@@ -381,6 +382,7 @@ class FireXHipSolver:
         cfg.jacobi_autotune = int(bool(jacobi_autotune))
         cfg.jacobi_tile_height = int(jacobi_tile_height)
         cfg.jacobi_prefetch = 0 if jacobi_prefetch is None else (2 if jacobi_prefetch else 1)
+        cfg.jacobi_ring_waves = int(jacobi_ring_waves)
         if particle_count is not None:
             cfg.particle_count = int(particle_count)
         if ray_steps is not None:
@@ -596,6 +598,7 @@ class FireXHipSolver:
             "jacobi": {"kernel": {1: "shift", 2: "ring"}.get(inf.jacobi_kernel, inf.jacobi_kernel),
                        "sweeps_per_launch": inf.jacobi_sweeps_per_launch, "chunk_z": inf.jacobi_chunk_z,
                        "prefetch": inf.jacobi_prefetch, "tile_height": inf.jacobi_tile_height,
+                       "waves": inf.jacobi_ring_waves,
                        "tuned_us_per_sweep": round(float(inf.jacobi_tuned_us_per_sweep), 3)},
             "particle_count": inf.particle_count,
         }

@@ -131,7 +131,9 @@ typedef struct firex_config {
   int32_t jacobi_prefetch;       /* FUSED plan, ring kernel: 0 = tuned default, 1 =
                                     off, 2 = load each plane one iteration early
                                     (K <= 3)                                       */
-  int32_t reserved;
+  int32_t jacobi_ring_waves;     /* FUSED plan, ring kernel: waves per block, 8 or
+                                    16 (16 waves of 2 rows: 32-row tiles, K <= 4);
+                                    0 = tuned default                              */
 } firex_config;
 
 /* Seeded Gaussian emitter (an addition: replaces the reference emitter()). */
@@ -160,6 +162,7 @@ typedef struct firex_info {
   float jacobi_tuned_us_per_sweep; /* autotuner's time per sweep; 0 = untuned */
   int32_t particle_count;       /* SPH droplets (firex_config.particle_count) */
   int32_t jacobi_tile_height;   /* FUSED plan in use: y rows per tile          */
+  int32_t jacobi_ring_waves;    /* FUSED plan in use: waves per ring block     */
 } firex_info;
 
 typedef struct firex_memory_estimate {

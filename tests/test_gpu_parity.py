@@ -213,14 +213,31 @@ def test_captured_graph_replay_bitwise(plan):
     g.close()
 
 
-def _ring_pair(dims, iters, sweeps, heavy="off", chunk=0, height=0, prefetch=None):
+def _ring_pair(dims, iters, sweeps, heavy="off", chunk=0, height=0, prefetch=None, waves=0):
     o = orc.Oracle(dims, precision=32, pressure_iterations=iters[0], correction_iterations=iters[1])
     o.set_bool("cinematic", True)
     o.set_heavy_mode({"off": orc.HEAVY_OFF, "f16": orc.HEAVY_F16}[heavy])
     g = FireXHipSolver(dims, tier="hd", pressure_iterations=iters[0], correction_iterations=iters[1], plan="fused",
                        heavy_mode=heavy, jacobi_sweeps_per_launch=sweeps, jacobi_kernel="ring",
-                       jacobi_chunk_z=chunk, jacobi_tile_height=height, jacobi_prefetch=prefetch)
+                       jacobi_chunk_z=chunk, jacobi_tile_height=height, jacobi_prefetch=prefetch,
+                       jacobi_ring_waves=waves)
     return g, o
+
+
+@pytest.mark.parametrize("dims,sweeps,chunk,heavy", [
+    ((80, 72, 30), 1, 7, "off"), ((80, 72, 30), 2, 5, "off"), ((80, 72, 30), 3, 0, "off"), ((80, 72, 30), 4, 7, "off"),
+    ((64, 32, 16), 4, 5, "off"), ((128, 96, 40), 4, 9, "off"), ((70, 100, 11), 3, 4, "off"), ((64, 48, 24), 4, 6, "f16")])
+def test_ring_kernel_sixteen_wave_blocks_bitwise(dims, sweeps, chunk, heavy):
+    """The 32-row ring tile as 16 waves of 2 rows (K <= 4), f32 and f16
+    pressure, against the oracle bit for bit."""
+    g, o = _ring_pair(dims, (11, 9), sweeps, heavy=heavy, chunk=chunk, waves=16)
+    info = g.info["jacobi"]
+    assert info["waves"] == 16 and info["tile_height"] == 32 and info["sweeps_per_launch"] == sweeps
+    o.step(DT, 2)
+    g.step(DT, 2)
+    assert_fields_equal(g, o, f"ring 16x2 {dims} K={sweeps} {heavy}", names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    g.close()
 
 
 @pytest.mark.parametrize("height,dims,sweeps,chunk,prefetch", [
