@@ -273,24 +273,33 @@ def cpu_baseline(wl: dict, sample_steps: int) -> dict:
 
 
 SERVE_GRIDS = [(64, 96, 64), (128, 192, 128), (160, 240, 160), (192, 288, 192)]
+SMALL_SERVE_GRIDS = [(64, 96, 64), (96, 64, 80), (64, 128, 64)]
 
 
-def serving_stream(requests: int, rate: float, seed: int) -> dict:
+def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batch: bool = False,
+                   batch_streams: int = 0, grids=None) -> dict:
     """Mixed-size plumes arriving as a seeded Poisson stream, served with
     continuous batching (admission by HBM budget and slots, captured steps,
     up to two steps in flight per plume).  Wall-clock arrivals: a request is
-    submitted when its arrival time has passed."""
+    submitted when its arrival time has passed.  batch: plumes of at most
+    4 Mi cells go to batch_streams streams and those on one stream step
+    together, each Jacobi launch one launch over all of them (16-wave ring
+    plan of depth 4, chunk height autotuned per grid); larger plumes step
+    alone on the other streams."""
     from fire_amd import FireXHipSolver, seeded_emitters
     from fire_amd.serve import FireXServer, make_request, poisson_stream
 
-    arrivals = poisson_stream(requests, rate, seed, grids=SERVE_GRIDS)
+    grids = grids or SERVE_GRIDS
+    arrivals = poisson_stream(requests, rate, seed, grids=grids)
     # Autotune every grid shape once up front (the plan cache is per
     # process), so the timed stream does not include tuning.
     for g in sorted({a[1] for a in arrivals}):
+        small = batch and g[0] * g[1] * g[2] <= 4 << 20
         FireXHipSolver(g, tier="uhd", pressure_iterations=96, correction_iterations=96, species_storage="f16",
-                       plan="fused", heavy_mode="off", jacobi_autotune=True).close()
-    srv = FireXServer(max_active=16, streams=4, plan="fused", capture_graphs=True, max_inflight_steps=2,
-                      jacobi_autotune=True)
+                       plan="fused", heavy_mode="off", jacobi_autotune=True,
+                       **({"jacobi_sweeps_per_launch": 4, "jacobi_ring_waves": 16} if small else {})).close()
+    srv = FireXServer(max_active=16, streams=streams, plan="fused", capture_graphs=True, max_inflight_steps=2,
+                      jacobi_autotune=True, batch_plumes=batch, batch_streams=batch_streams)
     ids = []
     t0 = time.perf_counter()
     pending = list(arrivals)
@@ -316,7 +325,7 @@ def serving_stream(requests: int, rate: float, seed: int) -> dict:
         return float(v[min(len(v) - 1, int(q * len(v)))]) if v else 0.0
 
     return {
-        "requests": len(ids), "arrival_rate_per_s": rate, "seed": seed, "grids": [list(g) for g in SERVE_GRIDS],
+        "requests": len(ids), "arrival_rate_per_s": rate, "seed": seed, "grids": [list(g) for g in grids],
         "steps_range": [60, 600], "jacobi": [96, 96], "species_storage": "f16",
         "completed": int(stats["done"]), "failed": int(stats["failed"]),
         "makespan_s": round(makespan, 3), "steps_completed": int(stats["steps_completed"]),
@@ -325,8 +334,10 @@ def serving_stream(requests: int, rate: float, seed: int) -> dict:
                             "max": round(stats["latency_ms_max"], 3)},
         "request_turnaround_s": {"p50": round(pct(turnaround, 0.5), 3), "p99": round(pct(turnaround, 0.99), 3)},
         "admission_wait_s": {"p50": round(pct(waits, 0.5), 3), "p99": round(pct(waits, 0.99), 3)},
-        "server": {"max_active": 16, "streams": 4, "capture_graphs": True, "max_inflight_steps": 2,
+        "server": {"max_active": 16, "streams": streams, "capture_graphs": True, "max_inflight_steps": 2,
+                   "batch_plumes": batch, "batch_streams": batch_streams if batch else 0,
                    "memory_budget_bytes": int(stats["memory_budget_bytes"])},
+        "group_steps": int(stats["group_steps"]), "grouped_plume_steps": int(stats["grouped_steps"]),
     }
 
 
@@ -501,6 +512,21 @@ def main() -> int:
         print(f"bench: serving stream ({args.serve_requests} requests at {args.serve_rate}/s)", file=sys.stderr,
               flush=True)
         serving = serving_stream(args.serve_requests, args.serve_rate, seed=2026)
+        # Small plumes batched: grids up to 4 Mi cells share two of the four
+        # streams and step in groups (one Jacobi launch per group), larger
+        # ones step alone on the other two.
+        print("bench: serving stream, small plumes batched", file=sys.stderr, flush=True)
+        serving["batched"] = serving_stream(args.serve_requests, args.serve_rate, seed=2026, streams=4, batch=True,
+                                            batch_streams=2)
+        # A stream of small grids only (where one plume's launches fill a
+        # fraction of the device): per-plume streams against batched groups.
+        small = {"grids": SMALL_SERVE_GRIDS, "requests": 2 * args.serve_requests, "rate": 10 * args.serve_rate}
+        print("bench: small-grid serving stream, per plume and batched", file=sys.stderr, flush=True)
+        serving["small_grids"] = {
+            "per_plume": serving_stream(small["requests"], small["rate"], seed=2026, grids=SMALL_SERVE_GRIDS),
+            "batched": serving_stream(small["requests"], small["rate"], seed=2026, streams=3, batch=True,
+                                      batch_streams=3, grids=SMALL_SERVE_GRIDS),
+        }
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -53,6 +53,37 @@ void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, vo
 // Host-side check of a plan against the kernel's assumptions.
 bool jacobi_plan_valid(const JacobiPlan& plan);
 
+// One launch of the RING kernel over several grids: each entry is one
+// grid's launch (its buffers, dims, stencil and chunk height), and the
+// launch's blocks are numbered entry after entry, so every grid gets exactly
+// the blocks, and the result, of its own launch.  The table travels in the
+// kernel arguments (no device allocation).
+constexpr int kRingBatchMax = 16;
+struct RingBatchEntry {
+  const void* pin;  // source pressure (not read when zero_start)
+  void* pout;
+  const float* div;
+  int32_t nx, ny, nz;
+  float wx, wy, wz, inv;
+  int32_t chunk_z;
+  int32_t zero_start;
+  int32_t tiles_x, tiles_y;  // set by launch_jacobi_ring_batch
+  uint32_t block_begin;      // set by launch_jacobi_ring_batch
+};
+struct RingBatch {
+  int32_t count;
+  uint32_t total_blocks;
+  RingBatchEntry entry[kRingBatchMax];
+};
+
+// Whether a plan runs in batched launches: the 16-wave RING shape.
+bool jacobi_plan_batchable(const JacobiPlan& plan);
+
+// One batched launch of `sweeps` (<= plan.sweeps_per_launch) sweeps; every
+// entry's grid must satisfy the plan (RING: nx >= 64, ny >= 32, nz >= K + 2,
+// fields under 2 GiB).  Fills the entries' tiles and block offsets.
+void launch_jacobi_ring_batch(const JacobiPlan& plan, int f16, int sweeps, RingBatch& batch, hipStream_t s);
+
 // Autotuner: times candidate plans for a grid of dims on the current device
 // and returns the fastest in *out (and its time per sweep in microseconds).
 // sweeps_per_launch / chunk_z / tile_height / prefetch > 0 and kernel !=

@@ -282,11 +282,18 @@ struct RingDims {
   static_assert(U % 2 == 0 && U % 3 == 0 && U % M0 == 0 && U % D == 0, "ring sizes");
 };
 
+// The grid and stencil one ring launch works on.
+struct RingGrid {
+  int32_t nx, ny, nz;
+  float wx, wy, wz, inv;
+};
+
+// One block of a ring launch: logical block b of tiles_x * tiles_y * chunks.
 template <int P, int K, int R, int W, int PF>
-__global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::Word* __restrict__ pin,
-                                                        typename PIO<P>::Word* __restrict__ pout,
-                                                        const float* __restrict__ div, FxStep st, int chunk_z,
-                                                        int tiles_x, int tiles_y, int zero_start) {
+__device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restrict__ pin,
+                                           typename PIO<P>::Word* __restrict__ pout, const float* __restrict__ div,
+                                           const RingGrid& st, int chunk_z, int tiles_x, int tiles_y, int zero_start,
+                                           uint32_t b) {
   constexpr int H = W * R;
   constexpr int M0 = RingDims<K, PF>::M0;
   constexpr int D = RingDims<K, PF>::D;
@@ -297,8 +304,6 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
   const int nx = st.nx, ny = st.ny, nz = st.nz;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const uint32_t chunks = (uint32_t)(nz + chunk_z - 1) / (uint32_t)chunk_z;
-  const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)tiles_x * (uint32_t)tiles_y * chunks);
   const int tile_x = (int)(b % (uint32_t)tiles_x);
   const uint32_t rest = b / (uint32_t)tiles_x;
   const int tile_y = (int)(rest % (uint32_t)tiles_y);
@@ -350,7 +355,7 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
   uint32_t vst[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) vst[r] = x_out && y_out[r] ? cx * PIO<P>::kBytes : 0x80000000u;
-  const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.jacobi_inv;
+  const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.inv;
   // wave-edge neighbours; wave 0's "down" and wave W-1's "up" are halo rows
   // or the zeroed top row, so any in-bounds slot will do
   const int wdn = wave > 0 ? wave - 1 : W - 1;
@@ -477,6 +482,31 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::W
 }
 
 template <int P, int K, int R, int W, int PF>
+__global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::Word* __restrict__ pin,
+                                                        typename PIO<P>::Word* __restrict__ pout,
+                                                        const float* __restrict__ div, RingGrid st, int chunk_z,
+                                                        int tiles_x, int tiles_y, int zero_start) {
+  const uint32_t chunks = (uint32_t)(st.nz + chunk_z - 1) / (uint32_t)chunk_z;
+  const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)tiles_x * (uint32_t)tiles_y * chunks);
+  ring_block<P, K, R, W, PF>(pin, pout, div, st, chunk_z, tiles_x, tiles_y, zero_start, b);
+}
+
+// Several plumes' ring launches as one launch: the blocks are numbered plume
+// after plume (RingBatchEntry::block_begin) and each block runs the tile of
+// its plume that the plume's own launch would give it, so every plume's
+// result is the same as alone.
+template <int P, int K, int R, int W, int PF>
+__global__ __launch_bounds__(64 * W) void k_jacobi_ring_batch(RingBatch batch) {
+  const uint32_t bg = xcd_remap(blockIdx.x, batch.total_blocks);
+  int i = 0;
+  while (i + 1 < batch.count && bg >= batch.entry[i + 1].block_begin) ++i;  // block-uniform
+  const RingBatchEntry& q = batch.entry[i];
+  const RingGrid g{q.nx, q.ny, q.nz, q.wx, q.wy, q.wz, q.inv};
+  ring_block<P, K, R, W, PF>((const typename PIO<P>::Word*)q.pin, (typename PIO<P>::Word*)q.pout, q.div, g, q.chunk_z,
+                             q.tiles_x, q.tiles_y, q.zero_start, bg - q.block_begin);
+}
+
+template <int P, int K, int R, int W, int PF>
 void launch_ring(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
                  int zero_start, hipStream_t s) {
   const int tiles_x = ring_tiles(st.nx, 64, K, 0);
@@ -484,7 +514,8 @@ void launch_ring(const JacobiPlan& plan, const void* pin, void* pout, const floa
   const int chunks = (st.nz + plan.chunk_z - 1) / plan.chunk_z;
   const uint32_t blocks = (uint32_t)tiles_x * (uint32_t)tiles_y * (uint32_t)chunks;
   using Word = typename PIO<P>::Word;
-  k_jacobi_ring<P, K, R, W, PF><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, st, plan.chunk_z, tiles_x,
+  const RingGrid g{st.nx, st.ny, st.nz, st.wx, st.wy, st.wz, st.jacobi_inv};
+  k_jacobi_ring<P, K, R, W, PF><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, g, plan.chunk_z, tiles_x,
                                                        tiles_y, zero_start);
 }
 
@@ -541,6 +572,33 @@ void launch_ring_k(int k, const JacobiPlan& plan, const void* pin, void* pout, c
     launch_ring_pf<P, 4>(k, plan, pin, pout, div, st, zero_start, s);
 }
 
+// Batched launches run the 16-wave shape only.
+template <int P, int K>
+void launch_ring_batch(RingBatch& b, hipStream_t s) {
+  constexpr int R = 2, W = kRingWideWaves;
+  uint32_t total = 0;
+  for (int i = 0; i < b.count; ++i) {
+    RingBatchEntry& e = b.entry[i];
+    e.tiles_x = ring_tiles(e.nx, 64, K, 0);
+    e.tiles_y = ring_tiles(e.ny, W * R, K, 1);
+    const int chunks = (e.nz + e.chunk_z - 1) / e.chunk_z;
+    e.block_begin = total;
+    total += (uint32_t)e.tiles_x * (uint32_t)e.tiles_y * (uint32_t)chunks;
+  }
+  b.total_blocks = total;
+  k_jacobi_ring_batch<P, K, R, W, 0><<<total, 64 * W, 0, s>>>(b);
+}
+
+template <int P>
+void launch_ring_batch_k(int k, RingBatch& b, hipStream_t s) {
+  switch (k) {
+    case 1: launch_ring_batch<P, 1>(b, s); break;
+    case 2: launch_ring_batch<P, 2>(b, s); break;
+    case 3: launch_ring_batch<P, 3>(b, s); break;
+    default: launch_ring_batch<P, 4>(b, s); break;
+  }
+}
+
 template <int P, int K>
 void launch_tb(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st, int zero_start,
                hipStream_t s) {
@@ -583,6 +641,18 @@ bool jacobi_plan_valid(const JacobiPlan& plan) {
       : plan.rows == 4 && plan.waves == 8;
   return plan.sweeps_per_launch >= 1 && plan.sweeps_per_launch <= max_k && shape && plan.chunk_z >= 1 &&
          (plan.kernel == JACOBI_KERNEL_SHIFT || plan.kernel == JACOBI_KERNEL_RING);
+}
+
+bool jacobi_plan_batchable(const JacobiPlan& plan) {
+  return jacobi_plan_valid(plan) && plan.kernel == JACOBI_KERNEL_RING && plan.waves == kRingWideWaves;
+}
+
+void launch_jacobi_ring_batch(const JacobiPlan& plan, int f16, int sweeps, RingBatch& batch, hipStream_t s) {
+  const int k = sweeps < plan.sweeps_per_launch ? sweeps : plan.sweeps_per_launch;
+  if (f16)
+    launch_ring_batch_k<1>(k, batch, s);
+  else
+    launch_ring_batch_k<0>(k, batch, s);
 }
 
 int jacobi_plan_blocks(const JacobiPlan& p, const int32_t dims[3]) {

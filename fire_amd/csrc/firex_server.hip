@@ -43,6 +43,7 @@ struct Request {
   firex_solver* solver = nullptr;
   uint64_t bytes = 0;
   uint32_t cells = 0;
+  bool batched = false;  // batch mode: a small plume on a batch stream
   std::deque<Inflight> inflight;
   std::vector<float> latency_ms;
 };
@@ -70,12 +71,13 @@ struct firex_server {
   std::vector<int64_t> active;
   uint64_t resident = 0;
   int64_t next_id = 1;
-  int next_stream = 0;
+  int next_stream = 0, next_batch_stream = 0;
   std::vector<hipEvent_t> pool;
   std::vector<float> latency_ms;
   int64_t steps_completed = 0;
   double cell_updates = 0;
   int32_t done = 0, failed = 0;
+  int64_t grouped_steps = 0, group_steps = 0;
 };
 
 namespace {
@@ -100,6 +102,20 @@ double event_time(firex_server* s, hipEvent_t e) {
   return s->epoch_seconds + (double)ms * 1e-3;
 }
 
+uint64_t batch_max_cells(const firex_server* s) {
+  return s->cfg.batch_max_cells > 0 ? (uint64_t)s->cfg.batch_max_cells : (uint64_t)4 << 20;
+}
+
+int batch_streams(const firex_server* s) {
+  const int n = (int)s->streams.size();
+  const int b = s->cfg.batch_streams > 0 ? s->cfg.batch_streams : n / 2;
+  return b < 1 ? 1 : b > n ? n : b;
+}
+
+bool batched_request(const firex_server* s, const firex_request& r) {
+  return s->cfg.batch_plumes && (uint64_t)r.dims[0] * (uint64_t)r.dims[1] * (uint64_t)r.dims[2] <= batch_max_cells(s);
+}
+
 firex_config plume_config(const firex_server* s, const firex_request& r) {
   firex_config c;
   firex_config_init(&c);
@@ -113,6 +129,13 @@ firex_config plume_config(const firex_server* s, const firex_request& r) {
   c.device = s->cfg.device;
   c.capture_graphs = s->cfg.capture_graphs;
   c.jacobi_autotune = s->cfg.jacobi_autotune;
+  if (batched_request(s, r)) {
+    // one block shape and depth for every plume, so that plumes can share
+    // launches: the 16-wave ring shape at depth 4 where the grid allows it
+    // (the autotuner, when on, picks the chunk height per grid)
+    c.jacobi_sweeps_per_launch = 4;
+    c.jacobi_ring_waves = 16;
+  }
   return c;
 }
 
@@ -166,7 +189,14 @@ int admit(firex_server* s) {
       continue;
     }
     firex_config c = plume_config(s, r.req);
-    hipStream_t stream = s->streams[(size_t)(s->next_stream++ % (int)s->streams.size())];
+    // batch mode: small plumes on the first batch_streams streams, the
+    // others on the rest (all of them when the batch streams are all)
+    r.batched = batched_request(s, r.req);
+    const int nb = s->cfg.batch_plumes ? batch_streams(s) : 0, ns = (int)s->streams.size();
+    const int slot = r.batched ? s->next_batch_stream++ % nb
+                   : nb < ns   ? nb + s->next_stream++ % (ns - nb)
+                               : s->next_stream++ % ns;
+    hipStream_t stream = s->streams[(size_t)slot];
     int rc = solver_create(&c, stream, &r.solver);
     if (rc == FIREX_OK && r.req.emitter_count > 0)
       rc = firex_set_emitters(r.solver, r.req.emitters, r.req.emitter_count, r.req.jitter, r.req.velocity_noise,
@@ -193,9 +223,75 @@ int admit(firex_server* s) {
   return admitted;
 }
 
+// Records the completion event of a step just issued for r.
+int note_issued(firex_server* s, Request& r, double t) {
+  hipEvent_t stop = take_event(s);
+  if (!stop) return set_error(FIREX_ERR_DEVICE, "hipEventCreate failed");
+  hipEventRecord(stop, solver_stream(r.solver));
+  r.inflight.push_back({stop, t});
+  r.steps_issued += 1;
+  return FIREX_OK;
+}
+
+// Batch mode: small plumes with one group key and one dt step in rounds, every
+// plume of the group with steps left joining each round (in groups of up to
+// kStepGroupMax), and a round is issued while the group's most loaded plume
+// is below the in-flight limit.  Issuing per plume instead would let the
+// plumes drift out of phase (each refilling as its own step completes) and
+// split the group into ever smaller launches.  Large plumes and those
+// without a key step alone.
+int issue_batched(firex_server* s, size_t limit) {
+  int issued = 0;
+  for (;;) {
+    std::vector<std::pair<uint64_t, Request*>> left;
+    for (int64_t id : s->active) {
+      Request& r = s->requests[id];
+      if (r.steps_issued < r.req.steps) left.push_back({r.batched ? solver_group_key(r.solver) : 0, &r});
+    }
+    // stable: plumes keep their admission order within a group
+    std::stable_sort(left.begin(), left.end(), [](const auto& a, const auto& b) {
+      return a.first != b.first ? a.first < b.first : a.second->req.dt < b.second->req.dt;
+    });
+    bool any = false;
+    for (size_t i = 0; i < left.size();) {
+      size_t j = i + 1;
+      if (left[i].first != 0)
+        while (j < left.size() && left[j].first == left[i].first && left[j].second->req.dt == left[i].second->req.dt)
+          ++j;
+      size_t busiest = 0;
+      for (size_t k = i; k < j; ++k) busiest = std::max(busiest, left[k].second->inflight.size());
+      if (busiest >= limit) {
+        i = j;
+        continue;
+      }
+      for (size_t c = i; c < j; c += (size_t)kStepGroupMax) {
+        const size_t e = std::min(j, c + (size_t)kStepGroupMax);
+        firex_solver* group[kStepGroupMax];
+        for (size_t k = c; k < e; ++k) group[k - c] = left[k].second->solver;
+        const double t = now(s);
+        const double dt = left[c].second->req.dt;
+        const int rc = e - c == 1 ? solver_step_async(group[0], dt) : solver_step_group(group, (int32_t)(e - c), dt);
+        if (rc != FIREX_OK) return -1;  // the solver's message stays in firex_last_error()
+        if (e - c > 1) {
+          s->group_steps += 1;
+          s->grouped_steps += (int64_t)(e - c);
+        }
+        for (size_t k = c; k < e; ++k) {
+          if (note_issued(s, *left[k].second, t) != FIREX_OK) return -1;
+          ++issued;
+        }
+      }
+      any = true;
+      i = j;
+    }
+    if (!any) return issued;
+  }
+}
+
 int issue(firex_server* s) {
   int issued = 0;
   const size_t limit = (size_t)(s->cfg.max_inflight_steps > 0 ? s->cfg.max_inflight_steps : 2);
+  if (s->cfg.batch_plumes) return issue_batched(s, limit);
   for (int64_t id : s->active) {
     Request& r = s->requests[id];
     while (r.inflight.size() < limit && r.steps_issued < r.req.steps) {
@@ -230,6 +326,8 @@ int firex_server_create(const firex_server_config* cfg, firex_server** out) {
   if (cfg->max_active < 1) return set_error(FIREX_ERR_INVALID, "max_active must be at least 1");
   if (cfg->streams < 1 || cfg->streams > 8) return set_error(FIREX_ERR_INVALID, "streams must be in [1, 8]");
   if (cfg->plan < 0 || cfg->plan > 1) return set_error(FIREX_ERR_INVALID, "unknown plan");
+  if (cfg->batch_max_cells < 0 || cfg->batch_streams < 0 || cfg->batch_streams > cfg->streams)
+    return set_error(FIREX_ERR_INVALID, "batch_max_cells must be non-negative and batch_streams in [0, streams]");
   if (firex_device_count() <= cfg->device || cfg->device < 0)
     return set_error(FIREX_ERR_DEVICE, "No HIP device is available for the Fire-X server.");
   if (hipSetDevice(cfg->device) != hipSuccess) return set_error(FIREX_ERR_DEVICE, "hipSetDevice failed");
@@ -372,6 +470,8 @@ int firex_server_get_stats(firex_server* s, firex_server_stats* out) {
   out->latency_ms_p50 = percentile(s->latency_ms, 0.50);
   out->latency_ms_p99 = percentile(s->latency_ms, 0.99);
   out->latency_ms_max = s->latency_ms.empty() ? 0.0 : *std::max_element(s->latency_ms.begin(), s->latency_ms.end());
+  out->grouped_steps = s->grouped_steps;
+  out->group_steps = s->group_steps;
   return FIREX_OK;
 }
 

@@ -14,6 +14,7 @@
 #include "firex_host.h"
 #include "firex_jacobi.h"
 #include "firex_kernels.h"
+#include "firex_solver_internal.h"
 
 using namespace fx;
 
@@ -765,6 +766,127 @@ int solver_step_async(firex_solver* s, double dt) {
   int rc = check_usable(s);
   if (rc) return rc;
   return step_with_dt(s, dt);
+}
+
+uint64_t solver_group_key(firex_solver* s) {
+  if (!s || s->lost || s->timing || s->heavy || s->cfg.plan != FIREX_PLAN_FUSED || !jacobi_plan_batchable(s->jacobi))
+    return 0;
+  const uint64_t v[] = {(uint64_t)(uintptr_t)s->stream,           (uint64_t)s->cfg.device,
+                        (uint64_t)s->f16p,                         (uint64_t)s->cfg.pressure_iterations,
+                        (uint64_t)s->cfg.correction_iterations,    (uint64_t)s->jacobi.sweeps_per_launch};
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (uint64_t x : v) h = (h ^ x) * 0x100000001b3ull;
+  return h | 1;
+}
+
+namespace {
+
+bool same_group(const firex_solver* a, const firex_solver* b) {
+  return a->stream == b->stream && a->cfg.device == b->cfg.device && a->f16p == b->f16p &&
+         a->cfg.pressure_iterations == b->cfg.pressure_iterations &&
+         a->cfg.correction_iterations == b->cfg.correction_iterations &&
+         a->jacobi.sweeps_per_launch == b->jacobi.sweeps_per_launch;
+}
+
+// `count` sweeps on every plume of a group, in batched ring launches of up
+// to K sweeps; corr picks the residual pair.  idx[i] is plume i's source
+// buffer on entry and its result buffer on return.
+void group_sweeps(firex_solver* const* g, int n, bool corr, const FxStep* st, int count, bool zero_start, int* idx) {
+  const JacobiPlan& plan = g[0]->jacobi;
+  bool zero = zero_start;
+  for (int left = count; left > 0;) {
+    const int k = left < plan.sweeps_per_launch ? left : plan.sweeps_per_launch;
+    RingBatch b;
+    b.count = n;
+    for (int i = 0; i < n; ++i) {
+      const firex_solver* s = g[i];
+      void* const* bufs = corr ? s->corr : s->pressure;
+      RingBatchEntry& e = b.entry[i];
+      e.pin = bufs[idx[i]];
+      e.pout = bufs[1 - idx[i]];
+      e.div = s->divergence;
+      e.nx = s->dims[0];
+      e.ny = s->dims[1];
+      e.nz = s->dims[2];
+      e.wx = st[i].wx;
+      e.wy = st[i].wy;
+      e.wz = st[i].wz;
+      e.inv = st[i].jacobi_inv;
+      e.chunk_z = s->jacobi.chunk_z;
+      e.zero_start = zero ? 1 : 0;
+      idx[i] = 1 - idx[i];
+    }
+    launch_jacobi_ring_batch(plan, g[0]->f16p ? 1 : 0, k, b, g[0]->stream);
+    zero = false;
+    left -= k;
+  }
+}
+
+}  // namespace
+
+// The fused branch of encode_rest over a group, with its Jacobi launches
+// batched (FireXEngine.ts:2158-2275 for each plume).
+int solver_step_group(firex_solver* const* g, int32_t n, double dt) {
+  static_assert(kStepGroupMax == kRingBatchMax, "a group's plumes share one ring launch");
+  if (!g || n < 1 || n > kStepGroupMax) return set_error(FIREX_ERR_INVALID, "a group holds 1 to 16 plumes");
+  if (!isfinite(dt) || dt < 0) return set_error(FIREX_ERR_INVALID, "step dt must be finite and non-negative");
+  for (int i = 0; i < n; ++i) {
+    int rc = check_usable(g[i]);
+    if (rc) return rc;
+    if (solver_group_key(g[i]) == 0 || !same_group(g[0], g[i]))
+      return set_error(FIREX_ERR_INVALID, "the plumes of a group must share a stream, pressure storage, iteration "
+                                          "counts and a batched ring Jacobi plan");
+    for (int j = 0; j < i; ++j)
+      if (g[j] == g[i]) return set_error(FIREX_ERR_INVALID, "a plume appears twice in the group");
+  }
+  hipSetDevice(g[0]->cfg.device);
+  if (n == 1) return step_with_dt(g[0], dt);
+  const hipStream_t q = g[0]->stream;
+  const int N = g[0]->cfg.pressure_iterations, M = g[0]->cfg.correction_iterations;
+  const int f16 = g[0]->f16p ? 1 : 0;
+  FxStep st[kRingBatchMax];
+  int idx[kRingBatchMax];
+  for (int i = 0; i < n; ++i) {
+    firex_solver* s = g[i];
+    float u[FXU_COUNT];
+    write_uniforms(s->params, s->dims, dt, s->simulation_time, (double)s->frame_index, s->cfg.time_step,
+                   s->cfg.cinematic != 0, u, s->cfg.particle_count);
+    int rc = droplet_passes(s, u);
+    if (rc) return rc;
+    st[i] = make_step(u, s->dims, s->emitters, s->frame_index, s->has_liquid);
+    encode_simulate(s, st[i]);
+    launch_divergence(flow1(s), s->divergence, st[i], q);
+    idx[i] = s->pressure_index;
+  }
+  group_sweeps(g, n, false, st, N, false, idx);
+  for (int i = 0; i < n; ++i) {
+    firex_solver* s = g[i];
+    s->pressure_index = idx[i];
+    launch_project(f16, flow1(s), s->pressure[s->pressure_index], flow0(s), st[i], q);
+    launch_divergence(flow0(s), s->divergence, st[i], q);
+    if (M == 0) {
+      const uint64_t pwords = ((uint64_t)s->cells * (s->f16p ? 2 : 4) + 3) / 4;
+      launch_fill_u32(s->corr[0], pwords, 0, q);
+      launch_fill_u32(s->corr[1], pwords, 0, q);
+    }
+    idx[i] = 0;
+  }
+  group_sweeps(g, n, true, st, M, true, idx);
+  for (int i = 0; i < n; ++i) {
+    firex_solver* s = g[i];
+    s->corr_index = idx[i];
+    launch_project(f16, flow0(s), s->corr[s->corr_index], flow1(s), st[i], q);
+    s->flow_a = 1 - s->flow_a;
+    s->divergence_is_rhs = true;
+    s->simulation_time += dt;
+    s->frame_index += 1;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    for (int i = 1; i < n; ++i) hip_fail(g[i], e, "group step");
+    return hip_fail(g[0], e, "group step");
+  }
+  return FIREX_OK;
 }
 
 hipStream_t solver_stream(firex_solver* s) { return s->stream; }

@@ -3,7 +3,8 @@
 `FireXServer` keeps many plumes resident on one MI355X, admits queued
 requests against an HBM byte budget and a slot limit, and advances every
 admitted plume concurrently (a pool of HIP streams, hipGraph-captured steps,
-at most a few steps in flight per plume).  Requests submitted while others
+at most a few steps in flight per plume; with batch_plumes, small plumes share
+streams and step together, each Jacobi launch covering all of them).  Requests submitted while others
 run join at the next pump.
 """
 from __future__ import annotations
@@ -30,7 +31,10 @@ class ServerConfig(ctypes.Structure):
         ("capture_graphs", ctypes.c_int32),
         ("max_inflight_steps", ctypes.c_int32),
         ("jacobi_autotune", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("batch_plumes", ctypes.c_int32),
+        ("batch_max_cells", ctypes.c_int32),
+        ("batch_streams", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -77,6 +81,8 @@ class ServerStats(ctypes.Structure):
         ("latency_ms_p50", ctypes.c_double),
         ("latency_ms_p99", ctypes.c_double),
         ("latency_ms_max", ctypes.c_double),
+        ("grouped_steps", ctypes.c_int64),
+        ("group_steps", ctypes.c_int64),
     ]
 
 
@@ -135,7 +141,8 @@ def make_request(dims: Sequence[int], steps: int, *, pressure_iterations: int = 
 class FireXServer:
     def __init__(self, memory_budget_bytes: int = 0, max_active: int = 16, streams: int = 4, plan: str = "fused",
                  capture_graphs: bool = True, max_inflight_steps: int = 2, device: int = 0,
-                 jacobi_autotune: bool = False):
+                 jacobi_autotune: bool = False, batch_plumes: bool = False, batch_max_cells: int = 0,
+                 batch_streams: int = 0):
         self._lib = _lib()
         cfg = ServerConfig()
         cfg.device = int(device)
@@ -146,6 +153,9 @@ class FireXServer:
         cfg.capture_graphs = int(bool(capture_graphs))
         cfg.max_inflight_steps = int(max_inflight_steps)
         cfg.jacobi_autotune = int(bool(jacobi_autotune))
+        cfg.batch_plumes = int(bool(batch_plumes))
+        cfg.batch_max_cells = int(batch_max_cells)
+        cfg.batch_streams = int(batch_streams)
         h = ctypes.c_void_p()
         _check(self._lib.firex_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -6,7 +6,9 @@
  * memory (an HBM byte budget) and slots free up, and advances every admitted
  * plume concurrently: each plume's steps go to one of a pool of HIP streams,
  * at most two steps in flight per plume, with the part of each step after
- * `simulate` replayed from a captured hipGraph.  Requests that arrive while
+ * `simulate` replayed from a captured hipGraph.  With batch_plumes, small
+ * plumes (whose launches alone fill a fraction of the device) share streams
+ * and step together, each Jacobi launch of their step covering all of them.  Requests that arrive while
  * others run join at the next pump (continuous batching).  Each plume's
  * kernels touch only its own buffers, so a plume's fields are bit-identical
  * whether it runs alone or in the mix.
@@ -36,7 +38,16 @@ typedef struct firex_server_config {
   int32_t capture_graphs;        /* 1: replay the post-simulate part of a step as a hipGraph */
   int32_t max_inflight_steps;    /* steps issued but not finished per plume (>= 1; 0 = 2)    */
   int32_t jacobi_autotune;       /* 1: autotune the Jacobi plan per grid shape (cached)      */
-  int32_t reserved[4];
+  int32_t batch_plumes;          /* 1: small plumes (at most batch_max_cells cells) go to the
+                                    first batch_streams streams, and those on one stream that
+                                    share iteration counts, pressure storage and dt step
+                                    together: each Jacobi launch of their step is one ring
+                                    launch over all their tiles (16-wave ring plan of depth
+                                    4; grids the ring kernel cannot run step alone).  Larger
+                                    plumes step alone on the other streams.                   */
+  int32_t batch_max_cells;       /* 0 = 4 Mi cells                                            */
+  int32_t batch_streams;         /* 0 = half of the streams (at least 1)                      */
+  int32_t reserved[1];
 } firex_server_config;
 
 typedef struct firex_request {
@@ -86,6 +97,9 @@ typedef struct firex_server_stats {
   double latency_ms_p50;         /* over every completed step                                 */
   double latency_ms_p99;
   double latency_ms_max;
+  int64_t grouped_steps;         /* batch_plumes: plume steps issued inside groups of two or
+                                    more (solver steps that shared their Jacobi launches)     */
+  int64_t group_steps;           /* batch_plumes: group steps issued                          */
 } firex_server_stats;
 
 int firex_server_create(const firex_server_config* config, firex_server** out);
