@@ -55,13 +55,14 @@ WORKLOADS = {
 }
 
 
-def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False, chunk: int = 0, heavy: str = "off"):
+def make_solver(wl: dict, plan: str, sweeps: int = 0, autotune: bool = False, chunk: int = 0, heavy: str = "off",
+                waves: int = 0):
     from fire_amd import FireXHipSolver, seeded_emitters
 
     s = FireXHipSolver(wl["dims"], tier="uhd", pressure_iterations=wl["iters"][0],
                        correction_iterations=wl["iters"][1], species_storage=wl["storage"], plan=plan,
                        heavy_mode=heavy, jacobi_sweeps_per_launch=sweeps, jacobi_chunk_z=chunk,
-                       jacobi_autotune=autotune)
+                       jacobi_autotune=autotune, jacobi_ring_waves=waves)
     em = seeded_emitters(wl["dims"], wl["emitters"], radius_range=wl["radius"], seed=wl["seed"])
     s.set_emitters(em, jitter=wl["jitter"], velocity_noise=wl["velocity_noise"], seed=wl["seed"])
     return s
@@ -225,9 +226,12 @@ def measured_traffic(workload: str, plan: dict):
         return None
     with open(path) as f:
         table = json.load(f)
+    def key(p):
+        return (p["kernel"], p["sweeps_per_launch"], p["chunk_z"], p["prefetch"], p.get("tile_height", 32),
+                p.get("waves", 8))
+
     for p in table["plans"]:
-        if (p["kernel"], p["sweeps_per_launch"], p["chunk_z"], p["prefetch"]) == (
-                plan["kernel"], plan["sweeps_per_launch"], plan["chunk_z"], plan["prefetch"]):
+        if key(p) == key(plan):
             return p["hbm_bytes_per_launch"]
     return None
 
@@ -350,6 +354,8 @@ def main() -> int:
     ap.add_argument("--plan", default="fused", choices=["fused", "reference"])
     ap.add_argument("--sweeps", type=int, default=0, help="Jacobi sweeps per launch (0 = autotuned)")
     ap.add_argument("--jacobi-chunk", type=int, default=0, help="Jacobi z planes per block (0 = autotuned)")
+    ap.add_argument("--jacobi-waves", type=int, default=0,
+                    help="waves per ring-kernel block: 8 (4 rows each) or 16 (2 rows each); 0 = autotuned or default")
     ap.add_argument("--no-autotune", action="store_true",
                     help="no Jacobi plan search (fixed --sweeps / --jacobi-chunk or the defaults): for profiles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -405,7 +411,7 @@ def main() -> int:
 
     t_setup = time.perf_counter()
     solver = make_solver(wl, args.plan, args.sweeps, autotune=not args.no_autotune and args.plan == "fused",
-                         chunk=args.jacobi_chunk)
+                         chunk=args.jacobi_chunk, waves=args.jacobi_waves)
     jacobi_plan = solver.info["jacobi"]
     print(f"bench: solver ready in {time.perf_counter() - t_setup:.2f} s, Jacobi plan {jacobi_plan}",
           file=sys.stderr, flush=True)

@@ -20,7 +20,8 @@ struct JacobiPlan {
 };
 // RING block shapes: 8 waves of 4, 6 or 8 rows (32-, 48- or 64-row tiles,
 // K <= 5), or 16 waves of 2 rows (32-row tiles, K <= 4, no prefetch: fewer
-// registers per thread, twice the waves per CU).
+// registers per thread, twice the waves per CU, per-lane row addressing and
+// with it the narrow wall tile of RingLayout).
 constexpr int kRingWideWaves = 16;
 constexpr int kRingWideMaxSweeps = 4;
 
@@ -53,6 +54,22 @@ void launch_jacobi_tb(const JacobiPlan& plan, int f16, const JacobiLaunch& l, vo
 // Host-side check of a plan against the kernel's assumptions.
 bool jacobi_plan_valid(const JacobiPlan& plan);
 
+// Tiles of a RING launch.  Full tiles are 64 columns wide and output
+// 64 - 2K columns each, placed from the left wall (the last at the right
+// wall).  With the 16-wave shape, where the columns left over at the right
+// wall number at most 32 - 2K, one 32-column tile covers them instead
+// (narrow = 1): a block of that tile runs two y tiles side by side, lanes
+// 0-31 on y tile 2q and lanes 32-63 on y tile 2q + 1 (pair q), so that no
+// half wave computes columns that feed no output.  At 192 columns and
+// K = 4: three full tiles and a narrow one, 224 columns per row computed
+// instead of 256.  Blocks: (tiles_x * tiles_y + pairs) per z chunk.
+struct RingLayout {
+  int32_t tiles_x;  // full 64-column tiles
+  int32_t tiles_y;
+  int32_t narrow;   // 1: a narrow tile at the right wall
+  int32_t pairs;    // y-tile pairs of the narrow tile ((tiles_y + 1) / 2)
+};
+
 // One launch of the RING kernel over several grids: each entry is one
 // grid's launch (its buffers, dims, stencil and chunk height), and the
 // launch's blocks are numbered entry after entry, so every grid gets exactly
@@ -67,8 +84,8 @@ struct RingBatchEntry {
   float wx, wy, wz, inv;
   int32_t chunk_z;
   int32_t zero_start;
-  int32_t tiles_x, tiles_y;  // set by launch_jacobi_ring_batch
-  uint32_t block_begin;      // set by launch_jacobi_ring_batch
+  RingLayout layout;     // set by launch_jacobi_ring_batch
+  uint32_t block_begin;  // set by launch_jacobi_ring_batch
 };
 struct RingBatch {
   int32_t count;

@@ -234,8 +234,10 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_tb(const typename PIO<P>::Wor
 //     y = ny - 1 is set to zero after every level; the last y tile ends
 //     there.
 // Tiles overlap by 2K columns/rows (K at the floor); an output cell is
-// only written by the tile whose valid range holds it.  Needs nx >= 64,
-// ny >= W * R and nz > K.
+// only written by the tile whose valid range holds it.  With 16-wave
+// blocks the columns left at the right wall may go to a 32-column tile
+// whose blocks carry two y tiles, one per half wave (RingLayout).  Needs
+// nx >= 64, ny >= W * R and nz > K.
 
 // lane i <- lane i - 1 (lane 0 <- 0)
 __device__ inline float shl_from_left(float v) {
@@ -265,6 +267,28 @@ __device__ inline void ring_tile(int i, int count, int n, int T, int K, int top,
   lo = i == 0 ? 0 : origin + K;
 }
 
+// Layout of a ring launch (RingLayout); lane_rows: the 16-wave shape, whose
+// addressing is per lane and so allows the narrow tile.
+__host__ __device__ inline RingLayout ring_layout(int nx, int ny, int K, int H, bool lane_rows) {
+  RingLayout l;
+  l.tiles_y = ring_tiles(ny, H, K, 1);
+  const int step = 64 - 2 * K;
+  const int full = nx / step, left = nx - full * step;
+  if (lane_rows && full >= 1 && left > 0 && left <= 32 - 2 * K) {
+    l.tiles_x = full;
+    l.narrow = 1;
+    l.pairs = (l.tiles_y + 1) / 2;
+  } else {
+    l.tiles_x = ring_tiles(nx, 64, K, 0);
+    l.narrow = 0;
+    l.pairs = 0;
+  }
+  return l;
+}
+__host__ __device__ inline uint32_t ring_columns(const RingLayout& l) {
+  return (uint32_t)l.tiles_x * (uint32_t)l.tiles_y + (uint32_t)l.pairs;
+}
+
 // Mirror of cell v across the walls of an axis of n cells (-K <= v < n + K, n > K).
 __device__ inline int mirror(int v, int n) { return v < 0 ? -1 - v : v >= n ? 2 * n - 1 - v : v; }
 
@@ -288,13 +312,19 @@ struct RingGrid {
   float wx, wy, wz, inv;
 };
 
-// One block of a ring launch: logical block b of tiles_x * tiles_y * chunks.
+// One block of a ring launch: logical block b of ring_columns(lay) * chunks
+// (chunk-major; within a chunk the full tiles x fastest, then the narrow
+// tile's y pairs).
 template <int P, int K, int R, int W, int PF>
 __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restrict__ pin,
                                            typename PIO<P>::Word* __restrict__ pout, const float* __restrict__ div,
-                                           const RingGrid& st, int chunk_z, int tiles_x, int tiles_y, int zero_start,
+                                           const RingGrid& st, int chunk_z, const RingLayout& lay, int zero_start,
                                            uint32_t b) {
   constexpr int H = W * R;
+  // 16-wave blocks address every row per lane (VGPR offsets), which the
+  // narrow tile's two half-wave y tiles need; 8-wave blocks keep the row
+  // offsets wave-uniform (SGPRs), which costs fewer registers for R >= 4.
+  constexpr bool kLaneRows = W == kRingWideWaves;
   constexpr int M0 = RingDims<K, PF>::M0;
   constexpr int D = RingDims<K, PF>::D;
   constexpr int U = RingDims<K, PF>::U;
@@ -304,15 +334,37 @@ __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restri
   const int nx = st.nx, ny = st.ny, nz = st.nz;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-  const int tile_x = (int)(b % (uint32_t)tiles_x);
-  const uint32_t rest = b / (uint32_t)tiles_x;
-  const int tile_y = (int)(rest % (uint32_t)tiles_y);
-  const int chunk = (int)(rest / (uint32_t)tiles_y);
-  int ox, xlo, xhi, oy, ylo, yhi;
-  ring_tile(tile_x, tiles_x, nx, 64, K, 0, ox, xlo, xhi);
+  const int tiles_x = lay.tiles_x, tiles_y = lay.tiles_y;
+  const uint32_t cols = ring_columns(lay);
+  const int chunk = (int)(b / cols);
+  const uint32_t c = b % cols;
+  const uint32_t n_full = (uint32_t)tiles_x * (uint32_t)tiles_y;
+  const bool narrow_block = kLaneRows && c >= n_full;  // block-uniform
+  int ox, xlo, xhi, gx, tile_y;
+  bool half_used = true;
+  if (narrow_block) {
+    // columns nx + K - 32 .. nx + K - 1 on both half waves; lanes 0-31 run
+    // y tile 2q, lanes 32-63 y tile 2q + 1 (absent for the last pair of an
+    // odd tile count: that half repeats the last tile and stores nothing).
+    // The wave-wide DPP shifts pass values between lanes 31 and 32, both
+    // outermost halo columns, which lose validity after the first level
+    // like the zero-filled lanes 0 and 63.
+    ox = nx + K - 32;
+    xlo = tiles_x * (64 - 2 * K);
+    xhi = nx;
+    gx = ox + (lane & 31);
+    const int ty = 2 * (int)(c - n_full) + (lane >> 5);
+    half_used = ty < tiles_y;
+    tile_y = half_used ? ty : tiles_y - 1;
+  } else {
+    const int tile_x = (int)(c % (uint32_t)tiles_x);
+    tile_y = (int)(c / (uint32_t)tiles_x);
+    ring_tile(tile_x, tiles_x + lay.narrow, nx, 64, K, 0, ox, xlo, xhi);
+    gx = ox + lane;
+  }
+  int oy, ylo, yhi;
   ring_tile(tile_y, tiles_y, ny, H, K, 1, oy, ylo, yhi);
   const bool fix_top = tile_y == tiles_y - 1 && wave == W - 1;  // row R - 1 is y = ny - 1
-  const int gx = ox + lane;
   const int gy0 = oy + wave * R;
   const int z0 = chunk * chunk_z;
   const int z1 = min(z0 + chunk_z, nz);
@@ -335,26 +387,32 @@ __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restri
       __builtin_amdgcn_make_buffer_rsrc((void*)pout, (short)0, (int)(cells * PIO<P>::kBytes), kBufferWord3);
   const __amdgpu_buffer_rsrc_t rdiv =
       __builtin_amdgcn_make_buffer_rsrc((void*)div, (short)0, (int)(cells * 4u), kBufferWord3);
-  // Addressing: the (clamped) column is the one per-lane offset; the
-  // (clamped) row offsets are wave-uniform and go with the plane offset.
+  // Addressing: byte offset = voff (VGPR) + plane offset + srow (SGPR).
+  // 8-wave blocks: voff is the (mirrored) column, srow the wave-uniform
+  // (mirrored) row; 16-wave blocks: voff holds column and row, srow is 0.
+  // Pressure and div offsets differ only with f16 pressure.
   const uint32_t cx = (uint32_t)mirror(gx, nx);
-  uint32_t rowoff[R];  // wave-uniform
-  bool y_out[R];       // wave-uniform
+  uint32_t voff[R], voffd[R], srow[R], srowd[R];
+  bool y_out[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int y = gy0 + r;
-    rowoff[r] = (uint32_t)nx * (uint32_t)mirror(y, ny);
-    y_out[r] = y >= ylo && y < yhi;
+    const uint32_t row = (uint32_t)nx * (uint32_t)mirror(y, ny);
+    voff[r] = (kLaneRows ? cx + row : cx) * PIO<P>::kBytes;
+    voffd[r] = (kLaneRows ? cx + row : cx) * 4u;
+    srow[r] = kLaneRows ? 0u : row * PIO<P>::kBytes;
+    srowd[r] = kLaneRows ? 0u : row * 4u;
+    y_out[r] = y >= ylo && y < yhi && half_used;
   }
   const bool x_out = gx >= xlo && gx < xhi;
-  // Store offsets: the lane's column where (x, y) is one of the tile's output
-  // cells, else 2 GiB, past the end of every field the ring kernel runs on
-  // (< 2 GiB), so the buffer store is discarded by the hardware range check
-  // (with or without the plane offset, which stays below 2 GiB, the sum
+  // Store offsets: voff where (x, y) is one of the tile's output cells, else
+  // 2 GiB, past the end of every field the ring kernel runs on (< 2 GiB), so
+  // the buffer store is discarded by the hardware range check (with or
+  // without the plane and row offsets, which stay below 2 GiB, the sum
   // neither wraps nor lands in range).  Stores then need no branches.
   uint32_t vst[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) vst[r] = x_out && y_out[r] ? cx * PIO<P>::kBytes : 0x80000000u;
+  for (int r = 0; r < R; ++r) vst[r] = x_out && y_out[r] ? voff[r] : 0x80000000u;
   const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.inv;
   // wave-edge neighbours; wave 0's "down" and wave W-1's "up" are halo rows
   // or the zeroed top row, so any in-bounds slot will do
@@ -402,9 +460,9 @@ __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restri
       const uint32_t pbase = (uint32_t)min(max(mirror(zr + zsign * t, nz), 0), nz - 1) * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        P0[ph % M0][r] = PIO<P>::bload(rin, cx * PIO<P>::kBytes, (pbase + rowoff[r]) * PIO<P>::kBytes);
+        P0[ph % M0][r] = PIO<P>::bload(rin, voff[r], pbase * PIO<P>::kBytes + srow[r]);
         Dv[ph % D][r] = __builtin_bit_cast(
-            float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)(cx * 4u), (int)((pbase + rowoff[r]) * 4u), 0));
+            float, __builtin_amdgcn_raw_buffer_load_b32(rdiv, (int)voffd[r], (int)(pbase * 4u + srowd[r]), 0));
       }
     }
 #pragma unroll
@@ -443,7 +501,7 @@ __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restri
         // z0 <= z < z1 here (see above)
         const uint32_t pbase = (uint32_t)(zr + zsign * z) * plane * PIO<P>::kBytes;
 #pragma unroll
-        for (int r = 0; r < R; ++r) PIO<P>::bstore(rout, vst[r], pbase + rowoff[r] * PIO<P>::kBytes, res[r]);
+        for (int r = 0; r < R; ++r) PIO<P>::bstore(rout, vst[r], pbase + srow[r], res[r]);
       }
     }
     // Publish rows 0 and R-1 of the plane each level's neighbours use next
@@ -485,10 +543,10 @@ template <int P, int K, int R, int W, int PF>
 __global__ __launch_bounds__(64 * W) void k_jacobi_ring(const typename PIO<P>::Word* __restrict__ pin,
                                                         typename PIO<P>::Word* __restrict__ pout,
                                                         const float* __restrict__ div, RingGrid st, int chunk_z,
-                                                        int tiles_x, int tiles_y, int zero_start) {
+                                                        RingLayout lay, int zero_start) {
   const uint32_t chunks = (uint32_t)(st.nz + chunk_z - 1) / (uint32_t)chunk_z;
-  const uint32_t b = xcd_remap(blockIdx.x, (uint32_t)tiles_x * (uint32_t)tiles_y * chunks);
-  ring_block<P, K, R, W, PF>(pin, pout, div, st, chunk_z, tiles_x, tiles_y, zero_start, b);
+  const uint32_t b = xcd_remap(blockIdx.x, ring_columns(lay) * chunks);
+  ring_block<P, K, R, W, PF>(pin, pout, div, st, chunk_z, lay, zero_start, b);
 }
 
 // Several plumes' ring launches as one launch: the blocks are numbered plume
@@ -503,20 +561,19 @@ __global__ __launch_bounds__(64 * W) void k_jacobi_ring_batch(RingBatch batch) {
   const RingBatchEntry& q = batch.entry[i];
   const RingGrid g{q.nx, q.ny, q.nz, q.wx, q.wy, q.wz, q.inv};
   ring_block<P, K, R, W, PF>((const typename PIO<P>::Word*)q.pin, (typename PIO<P>::Word*)q.pout, q.div, g, q.chunk_z,
-                             q.tiles_x, q.tiles_y, q.zero_start, bg - q.block_begin);
+                             q.layout, q.zero_start, bg - q.block_begin);
 }
 
 template <int P, int K, int R, int W, int PF>
 void launch_ring(const JacobiPlan& plan, const void* pin, void* pout, const float* div, const FxStep& st,
                  int zero_start, hipStream_t s) {
-  const int tiles_x = ring_tiles(st.nx, 64, K, 0);
-  const int tiles_y = ring_tiles(st.ny, W * R, K, 1);
+  const RingLayout lay = ring_layout(st.nx, st.ny, K, W * R, W == kRingWideWaves);
   const int chunks = (st.nz + plan.chunk_z - 1) / plan.chunk_z;
-  const uint32_t blocks = (uint32_t)tiles_x * (uint32_t)tiles_y * (uint32_t)chunks;
+  const uint32_t blocks = ring_columns(lay) * (uint32_t)chunks;
   using Word = typename PIO<P>::Word;
   const RingGrid g{st.nx, st.ny, st.nz, st.wx, st.wy, st.wz, st.jacobi_inv};
-  k_jacobi_ring<P, K, R, W, PF><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, g, plan.chunk_z, tiles_x,
-                                                       tiles_y, zero_start);
+  k_jacobi_ring<P, K, R, W, PF><<<blocks, 64 * W, 0, s>>>((const Word*)pin, (Word*)pout, div, g, plan.chunk_z, lay,
+                                                       zero_start);
 }
 
 template <int P, int R, int W, int PF>
@@ -579,11 +636,10 @@ void launch_ring_batch(RingBatch& b, hipStream_t s) {
   uint32_t total = 0;
   for (int i = 0; i < b.count; ++i) {
     RingBatchEntry& e = b.entry[i];
-    e.tiles_x = ring_tiles(e.nx, 64, K, 0);
-    e.tiles_y = ring_tiles(e.ny, W * R, K, 1);
+    e.layout = ring_layout(e.nx, e.ny, K, W * R, true);
     const int chunks = (e.nz + e.chunk_z - 1) / e.chunk_z;
     e.block_begin = total;
-    total += (uint32_t)e.tiles_x * (uint32_t)e.tiles_y * (uint32_t)chunks;
+    total += ring_columns(e.layout) * (uint32_t)chunks;
   }
   b.total_blocks = total;
   k_jacobi_ring_batch<P, K, R, W, 0><<<total, 64 * W, 0, s>>>(b);
@@ -658,7 +714,7 @@ void launch_jacobi_ring_batch(const JacobiPlan& plan, int f16, int sweeps, RingB
 int jacobi_plan_blocks(const JacobiPlan& p, const int32_t dims[3]) {
   const int K = p.sweeps_per_launch;
   const int tiles = p.kernel == JACOBI_KERNEL_RING
-      ? ring_tiles(dims[0], 64, K, 0) * ring_tiles(dims[1], p.waves * p.rows, K, 1)
+      ? (int)ring_columns(ring_layout(dims[0], dims[1], K, p.waves * p.rows, p.waves == kRingWideWaves))
       : ((dims[0] + 63 - 2 * K) / (64 - 2 * K)) * ((dims[1] + 31 - 2 * K) / (32 - 2 * K));
   return tiles * ((dims[2] + p.chunk_z - 1) / p.chunk_z);
 }

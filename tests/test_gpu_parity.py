@@ -226,10 +226,15 @@ def _ring_pair(dims, iters, sweeps, heavy="off", chunk=0, height=0, prefetch=Non
 
 @pytest.mark.parametrize("dims,sweeps,chunk,heavy", [
     ((80, 72, 30), 1, 7, "off"), ((80, 72, 30), 2, 5, "off"), ((80, 72, 30), 3, 0, "off"), ((80, 72, 30), 4, 7, "off"),
-    ((64, 32, 16), 4, 5, "off"), ((128, 96, 40), 4, 9, "off"), ((70, 100, 11), 3, 4, "off"), ((64, 48, 24), 4, 6, "f16")])
+    ((64, 32, 16), 4, 5, "off"), ((128, 96, 40), 4, 9, "off"), ((70, 100, 11), 3, 4, "off"), ((64, 48, 24), 4, 6, "f16"),
+    ((192, 72, 20), 4, 6, "off"), ((192, 40, 12), 3, 5, "f16"), ((160, 72, 20), 4, 6, "off"), ((150, 56, 13), 2, 4, "off")])
 def test_ring_kernel_sixteen_wave_blocks_bitwise(dims, sweeps, chunk, heavy):
     """The 32-row ring tile as 16 waves of 2 rows (K <= 4), f32 and f16
-    pressure, against the oracle bit for bit."""
+    pressure, against the oracle bit for bit.  Most of these grids leave at
+    most 32 - 2K columns at the right wall, which the narrow 32-column tile
+    covers with two y tiles per wave (an odd y-tile count leaves the last
+    pair's upper half unused); 160 columns at K = 4 and 150 at K = 2 take
+    full tiles only."""
     g, o = _ring_pair(dims, (11, 9), sweeps, heavy=heavy, chunk=chunk, waves=16)
     info = g.info["jacobi"]
     assert info["waves"] == 16 and info["tile_height"] == 32 and info["sweeps_per_launch"] == sweeps
