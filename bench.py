@@ -127,7 +127,7 @@ def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict, render: bool 
     (density and integration each test all N^2 ordered pairs; deposition
     scatters into the liquid field that simulate reads).  The Jacobi plan is
     the one the dry run used."""
-    s = make_solver(wl, "fused", plan["sweeps_per_launch"], chunk=plan["chunk_z"])
+    s = make_solver(wl, "fused", plan["sweeps_per_launch"], chunk=plan["chunk_z"], waves=plan.get("waves", 0))
     for k, v in (("waterFlow", 0.8), ("nozzleType", 1), ("sprayAngle", 24.0)):
         s.set_parameter(k, v)
     s.set_parameter("waterEnabled", True)
