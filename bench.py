@@ -281,10 +281,10 @@ SMALL_SERVE_GRIDS = [(64, 96, 64), (96, 64, 80), (64, 128, 64)]
 
 
 def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batch: bool = False,
-                   batch_streams: int = 0, grids=None) -> dict:
+                   batch_streams: int = 0, grids=None, inflight: int = 1, max_active: int = 16) -> dict:
     """Mixed-size plumes arriving as a seeded Poisson stream, served with
     continuous batching (admission by HBM budget and slots, captured steps,
-    up to two steps in flight per plume).  Wall-clock arrivals: a request is
+    one step in flight per plume).  Wall-clock arrivals: a request is
     submitted when its arrival time has passed.  batch: plumes of at most
     4 Mi cells go to batch_streams streams and those on one stream step
     together, each Jacobi launch one launch over all of them (16-wave ring
@@ -302,7 +302,8 @@ def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batc
         FireXHipSolver(g, tier="uhd", pressure_iterations=96, correction_iterations=96, species_storage="f16",
                        plan="fused", heavy_mode="off", jacobi_autotune=True,
                        **({"jacobi_sweeps_per_launch": 4, "jacobi_ring_waves": 16} if small else {})).close()
-    srv = FireXServer(max_active=16, streams=streams, plan="fused", capture_graphs=True, max_inflight_steps=2,
+    srv = FireXServer(max_active=max_active, streams=streams, plan="fused", capture_graphs=True,
+                      max_inflight_steps=inflight,
                       jacobi_autotune=True, batch_plumes=batch, batch_streams=batch_streams)
     ids = []
     t0 = time.perf_counter()
@@ -338,7 +339,7 @@ def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batc
                             "max": round(stats["latency_ms_max"], 3)},
         "request_turnaround_s": {"p50": round(pct(turnaround, 0.5), 3), "p99": round(pct(turnaround, 0.99), 3)},
         "admission_wait_s": {"p50": round(pct(waits, 0.5), 3), "p99": round(pct(waits, 0.99), 3)},
-        "server": {"max_active": 16, "streams": streams, "capture_graphs": True, "max_inflight_steps": 2,
+        "server": {"max_active": max_active, "streams": streams, "capture_graphs": True, "max_inflight_steps": inflight,
                    "batch_plumes": batch, "batch_streams": batch_streams if batch else 0,
                    "memory_budget_bytes": int(stats["memory_budget_bytes"])},
         "group_steps": int(stats["group_steps"]), "grouped_plume_steps": int(stats["grouped_steps"]),

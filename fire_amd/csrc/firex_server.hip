@@ -290,7 +290,10 @@ int issue_batched(firex_server* s, size_t limit) {
 
 int issue(firex_server* s) {
   int issued = 0;
-  const size_t limit = (size_t)(s->cfg.max_inflight_steps > 0 ? s->cfg.max_inflight_steps : 2);
+  // One step in flight per plume by default: on the mixed serving stream a
+  // second one doubled the step latency (each step also queued behind its
+  // plume's previous one) without raising throughput.
+  const size_t limit = (size_t)(s->cfg.max_inflight_steps > 0 ? s->cfg.max_inflight_steps : 1);
   if (s->cfg.batch_plumes) return issue_batched(s, limit);
   for (int64_t id : s->active) {
     Request& r = s->requests[id];

@@ -140,7 +140,7 @@ def make_request(dims: Sequence[int], steps: int, *, pressure_iterations: int = 
 
 class FireXServer:
     def __init__(self, memory_budget_bytes: int = 0, max_active: int = 16, streams: int = 4, plan: str = "fused",
-                 capture_graphs: bool = True, max_inflight_steps: int = 2, device: int = 0,
+                 capture_graphs: bool = True, max_inflight_steps: int = 1, device: int = 0,
                  jacobi_autotune: bool = False, batch_plumes: bool = False, batch_max_cells: int = 0,
                  batch_streams: int = 0):
         self._lib = _lib()

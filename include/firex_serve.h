@@ -5,7 +5,7 @@
  * plumes of different grid sizes resident in HBM, admits queued requests as
  * memory (an HBM byte budget) and slots free up, and advances every admitted
  * plume concurrently: each plume's steps go to one of a pool of HIP streams,
- * at most two steps in flight per plume, with the part of each step after
+ * one step in flight per plume by default, with the part of each step after
  * `simulate` replayed from a captured hipGraph.  With batch_plumes, small
  * plumes (whose launches alone fill a fraction of the device) share streams
  * and step together, each Jacobi launch of their step covering all of them.  Requests that arrive while
@@ -36,7 +36,7 @@ typedef struct firex_server_config {
   int32_t streams;               /* HIP streams the plumes are spread over (1..8)            */
   int32_t plan;                  /* FIREX_PLAN_* for every plume                              */
   int32_t capture_graphs;        /* 1: replay the post-simulate part of a step as a hipGraph */
-  int32_t max_inflight_steps;    /* steps issued but not finished per plume (>= 1; 0 = 2)    */
+  int32_t max_inflight_steps;    /* steps issued but not finished per plume (>= 1; 0 = 1)    */
   int32_t jacobi_autotune;       /* 1: autotune the Jacobi plan per grid shape (cached)      */
   int32_t batch_plumes;          /* 1: small plumes (at most batch_max_cells cells) go to the
                                     first batch_streams streams, and those on one stream that
