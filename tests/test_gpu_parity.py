@@ -245,6 +245,34 @@ def test_ring_kernel_sixteen_wave_blocks_bitwise(dims, sweeps, chunk, heavy):
     g.close()
 
 
+def _layout_cases():
+    """Grids drawn (seeded) across the widths where the 16-wave layout does
+    and does not take the 32-column wall tile, with odd and even y-tile
+    counts, short z chunks and every depth."""
+    rng = np.random.default_rng(20261204)
+    cases = []
+    for _ in range(12):
+        k = int(rng.integers(1, 5))
+        nx = int(rng.integers(64, 260))
+        ny = int(rng.integers(32, 120))
+        nz = int(rng.integers(k + 2, 24))
+        chunk = int(rng.integers(2, 9))
+        cases.append(((nx, ny, nz), k, chunk))
+    return cases
+
+
+@pytest.mark.parametrize("dims,sweeps,chunk", _layout_cases())
+def test_ring_kernel_sixteen_wave_layouts_bitwise(dims, sweeps, chunk):
+    g, o = _ring_pair(dims, (7, 5), sweeps, chunk=chunk, waves=16)
+    o.step(DT, 1)
+    g.step(DT, 1)
+    assert_fields_equal(g, o, f"ring 16x2 layout {dims} K={sweeps} chunk {chunk}",
+                        names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    np.testing.assert_array_equal(_bits(g.read_field("correction")), _bits(o.read(orc.FIELD_CORRECTION)))
+    g.close()
+
+
 @pytest.mark.parametrize("height,dims,sweeps,chunk,prefetch", [
     (48, (80, 100, 20), 3, 7, False), (48, (80, 100, 20), 2, 5, True), (48, (64, 48, 12), 5, 0, False),
     (64, (64, 130, 17), 3, 6, False), (64, (64, 130, 17), 3, 6, True), (64, (128, 96, 40), 4, 9, False),
