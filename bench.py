@@ -353,6 +353,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="uhd-mixed", choices=sorted(WORKLOADS))
     ap.add_argument("--plan", default="fused", choices=["fused", "reference"])
+    ap.add_argument("--heavy", default=None, choices=["auto", "off", "f16", "f32"],
+                    help="heavy path (>= 64 Mi cells): default auto for plume-508, off otherwise")
     ap.add_argument("--sweeps", type=int, default=0, help="Jacobi sweeps per launch (0 = autotuned)")
     ap.add_argument("--jacobi-chunk", type=int, default=0, help="Jacobi z planes per block (0 = autotuned)")
     ap.add_argument("--jacobi-waves", type=int, default=0,
@@ -411,16 +413,18 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     t_setup = time.perf_counter()
+    heavy = args.heavy or ("auto" if args.workload == "plume-508" else "off")
     solver = make_solver(wl, args.plan, args.sweeps, autotune=not args.no_autotune and args.plan == "fused",
-                         chunk=args.jacobi_chunk, waves=args.jacobi_waves)
+                         chunk=args.jacobi_chunk, waves=args.jacobi_waves, heavy=heavy)
     jacobi_plan = solver.info["jacobi"]
+    f16_pressure = bool(solver.info["f16_pressure"])
     print(f"bench: solver ready in {time.perf_counter() - t_setup:.2f} s, Jacobi plan {jacobi_plan}",
           file=sys.stderr, flush=True)
     elapsed = time_steps(solver, args.steps, args.warmup)
     ms_per_step = elapsed / args.steps * 1e3
     value = cells * args.steps / elapsed
     print(f"bench: timed {args.steps} steps in {elapsed:.3f} s ({ms_per_step:.3f} ms/step)", file=sys.stderr, flush=True)
-    passes = pass_profile(solver, args.profile_steps, cells, wl["storage"])
+    passes = pass_profile(solver, args.profile_steps, cells, wl["storage"], f16_pressure=f16_pressure)
     passes["pack_fields"] = pack_profile(solver, cells, wl["storage"])
     audit = audit_profile(solver)
     render = render_profile(solver) if args.render else None
@@ -431,7 +435,7 @@ def main() -> int:
     raw = passes.pop("_raw")
     sweeps_per_launch = raw["jacobi_sweeps"] / max(1, raw["jacobi"]["launches"])
     avg_launch_s = jac.get("avg_launch_us", 0.0) * 1e-6
-    bytes_per_launch = BYTES_PER_CELL["jacobi_launch"] * cells
+    bytes_per_launch = (8 if f16_pressure else BYTES_PER_CELL["jacobi_launch"]) * cells  # f16 p: 2 + 4 + 2 B
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(args.workload, jacobi_plan),
@@ -556,7 +560,7 @@ def main() -> int:
         "config": {"workload": args.workload, "grid": list(wl["dims"]), "cells": cells,
                    "jacobi": list(wl["iters"]), "species_storage": wl["storage"], "dt": DT,
                    "emitters": wl["emitters"], "seed": wl["seed"], "plan": args.plan, "jacobi_plan": jacobi_plan,
-                   "parallelism": "single GPU"},
+                   "pressure_storage": "f16" if f16_pressure else "f32", "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "passes": passes,
