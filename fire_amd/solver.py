@@ -27,7 +27,9 @@ from typing import Iterable, Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(HERE, "libfirex_hip.so")
+# FIREX_LIBRARY selects another build of the library (to compare builds);
+# by default the in-tree one.
+LIBRARY_PATH = os.environ.get("FIREX_LIBRARY") or os.path.join(HERE, "libfirex_hip.so")
 
 # status codes (include/firex.h)
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_BUDGET, ERR_OUT_OF_MEMORY, ERR_DEVICE, ERR_ABORTED, ERR_DEVICE_LOST = range(8)
