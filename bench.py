@@ -363,6 +363,9 @@ def main() -> int:
                     help="no Jacobi plan search (fixed --sweeps / --jacobi-chunk or the defaults): for profiles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES three times (counter calibration)")
+    ap.add_argument("--bandwidth-probe", action="store_true",
+                    help="only time plain device reads (sum) and copies from 64 MiB to 4 GiB: the achievable rates "
+                         "with and without the 256 MB last-level cache")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=60, help="steps timed on the plain f32 path (0 = skip)")
     ap.add_argument("--cpu-sample-steps", type=int, default=5)
@@ -395,6 +398,36 @@ def main() -> int:
         ms = a.elapsed_time(b) / 3
         print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 4, "copy_ms": ms,
                           "copy_GBps_read_plus_write": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
+
+    if args.bandwidth_probe:
+        # What a plain stream reaches on this device: a sum reads the buffer
+        # once, a copy reads and writes it; 20 back-to-back launches each.
+        import torch
+
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def launch_us(fn, reps=20):
+            fn()
+            torch.cuda.synchronize()
+            a.record()
+            for _ in range(reps):
+                fn()
+            b.record()
+            torch.cuda.synchronize()
+            return a.elapsed_time(b) / reps * 1e3
+
+        rows = []
+        for mib in (64, 128, 256, 1024, 4096):
+            n = mib * 2**20 // 4
+            x = torch.rand(n, device="cuda")
+            y = torch.empty_like(x)
+            read_us, copy_us = launch_us(lambda: x.sum()), launch_us(lambda: y.copy_(x))
+            rows.append({"MiB": mib, "read_TBps": round(n * 4 / read_us / 1e6, 2),
+                         "copy_TBps_read_plus_write": round(2 * n * 4 / copy_us / 1e6, 2)})
+            print(json.dumps(rows[-1]), flush=True)
+            del x, y
+        print(json.dumps({"bandwidth_probe": rows}), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

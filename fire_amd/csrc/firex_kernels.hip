@@ -608,15 +608,67 @@ __global__ __launch_bounds__(256) void k_pack_fields(const float4* __restrict__ 
   out[2u * cells + id] = pack_half4(react, soot, liq, display);
 }
 
-// computeDivergence, shaders.ts:344-364.
+// Column kernels (divergence, projection): a 64 x 4 block owns a 64 x 4
+// column of cells over kColPlanes consecutive planes, and each thread issues
+// every load of its column before it computes, so a wave has kColPlanes
+// planes of reads in flight instead of one (these passes stream at the
+// latency of their loads otherwise).  The z neighbour of a plane is the
+// next or previous plane's value, already loaded.  Lanes past the x wall
+// and planes past the last one work on the clamped cell and store exactly
+// the value that cell's own lane or plane stores, to the same address, so
+// no store is conditional (a conditional store lets the compiler sink the
+// loads it consumes behind the branch, one plane at a time).
+constexpr int kColPlanes = 4;
+
+struct ColumnBlock {
+  int x, y, z0;
+  uint32_t col;  // x (clamped to the grid) + nx * y
+};
+__device__ inline bool column_block(const FxStep& st, ColumnBlock& c) {
+  const uint32_t bx = (uint32_t)(st.nx + kBX - 1) / kBX;
+  const uint32_t by = (uint32_t)(st.ny + kBY - 1) / kBY;
+  const uint32_t bz = (uint32_t)(st.nz + kColPlanes - 1) / kColPlanes;
+  const uint32_t b = xcd_remap(blockIdx.x, bx * by * bz);
+  const uint32_t ix = b % bx, rest = b / bx;
+  const uint32_t iy = rest % by, iz = rest / by;
+  c.x = (int)(ix * kBX + threadIdx.x);
+  c.y = (int)(iy * kBY + threadIdx.y);
+  c.z0 = (int)iz * kColPlanes;
+  c.col = (uint32_t)min(c.x, st.nx - 1) + (uint32_t)st.nx * (uint32_t)c.y;
+  return c.y < st.ny;  // uniform over a wave (one x row)
+}
+static inline dim3 column_grid(const FxStep& st) {
+  const uint32_t bx = (uint32_t)(st.nx + kBX - 1) / kBX, by = (uint32_t)(st.ny + kBY - 1) / kBY;
+  return dim3(bx * by * ((uint32_t)(st.nz + kColPlanes - 1) / kColPlanes));
+}
+
+// computeDivergence, shaders.ts:344-364.  The reads at x - 1, y - 1, z - 1
+// clamp to the cell itself at the walls, as cidx does.
 __global__ __launch_bounds__(256) void k_divergence(const float4* __restrict__ flow, float* __restrict__ div, FxStep st) {
-  int x, y, z;
-  if (!cell_coords(st, x, y, z)) return;
-  const float4 c = flow[cidx(st, x, y, z)];
-  float left = flow[cidx(st, x - 1, y, z)].x;
-  float bottom = flow[cidx(st, x, y - 1, z)].y;
-  float back = flow[cidx(st, x, y, z - 1)].z;
-  div[cidx(st, x, y, z)] = (c.x - left) * st.mx + (c.y - bottom) * st.my + (c.z - back) * st.mz;
+  ColumnBlock cb;
+  if (!column_block(st, cb)) return;
+  const uint32_t plane = (uint32_t)st.nx * (uint32_t)st.ny;
+  const uint32_t dx = min(cb.x, st.nx - 1) > 0 ? 1u : 0u, dy = cb.y > 0 ? (uint32_t)st.nx : 0u;
+  uint32_t id[kColPlanes];
+  float4 c[kColPlanes];
+  float left[kColPlanes], bottom[kColPlanes];
+#pragma unroll
+  for (int k = 0; k < kColPlanes; ++k) {
+    id[k] = cb.col + plane * (uint32_t)min(cb.z0 + k, st.nz - 1);
+    c[k] = flow[id[k]];
+    left[k] = flow[id[k] - dx].x;
+    bottom[k] = flow[id[k] - dy].y;
+  }
+  float back = flow[cb.col + plane * (uint32_t)max(cb.z0 - 1, 0)].z;
+  float d = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kColPlanes; ++k) {
+    // past the last plane: the last plane's value again (k = 0 never is)
+    const float dk = (c[k].x - left[k]) * st.mx + (c[k].y - bottom[k]) * st.my + (c[k].z - back) * st.mz;
+    d = k == 0 || cb.z0 + k < st.nz ? dk : d;
+    div[id[k]] = d;
+    back = c[k].z;
+  }
 }
 
 // loadPressure, shaders.ts:366-373: y >= ny is the open-top Dirichlet zero.
@@ -657,11 +709,10 @@ __device__ inline void gradient(const typename PressureIO<P>::Word* p, const FxS
   g2 = (load_p<P>(p, st, x, y, z + 1) - c) * st.mz;
 }
 
+// One cell of the projection, every read through load_p / cidx.
 template <int P>
-__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ fin, const typename PressureIO<P>::Word* __restrict__ p,
-                                                 float4* __restrict__ fout, FxStep st) {
-  int x, y, z;
-  if (!cell_coords(st, x, y, z)) return;
+__device__ inline float4 project_cell(const float4* fin, const typename PressureIO<P>::Word* p, const FxStep& st,
+                                      int x, int y, int z) {
   const uint32_t id = cidx(st, x, y, z);
   float g0, g1, g2;
   gradient<P>(p, st, x, y, z, g0, g1, g2);
@@ -676,7 +727,54 @@ __global__ __launch_bounds__(256) void k_project(const float4* __restrict__ fin,
   if (z == 0 || z == st.nz - 1) f.z = 0.0f;
   if (y == 0) f.y = 0.0f;
   if (y == st.ny - 1) f.y = fxm_max(0.0f, f.y);
-  fout[id] = f;
+  return f;
+}
+
+// Column form of project_cell for the rows below the top row: p at x + 1
+// clamps to the cell itself at the x wall, p at y + 1 is in the grid, and p
+// at z + 1 is the next plane's centre value (the last plane's clamps to
+// itself).  The top row, which reads the row below, goes through
+// project_cell.
+template <int P>
+__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ fin, const typename PressureIO<P>::Word* __restrict__ p,
+                                                 float4* __restrict__ fout, FxStep st) {
+  using PIO = PressureIO<P>;
+  ColumnBlock cb;
+  if (!column_block(st, cb)) return;
+  const int xc = min(cb.x, st.nx - 1);
+  if (cb.y == st.ny - 1) {
+    for (int k = 0; k < kColPlanes && cb.z0 + k < st.nz; ++k)
+      fout[cidx(st, xc, cb.y, cb.z0 + k)] = project_cell<P>(fin, p, st, xc, cb.y, cb.z0 + k);
+    return;
+  }
+  const uint32_t plane = (uint32_t)st.nx * (uint32_t)st.ny;
+  const uint32_t dx = xc < st.nx - 1 ? 1u : 0u, dy = (uint32_t)st.nx;
+  uint32_t id[kColPlanes];
+  float4 f[kColPlanes];
+  float pc[kColPlanes + 1], px[kColPlanes], py[kColPlanes];
+#pragma unroll
+  for (int k = 0; k < kColPlanes; ++k) {
+    id[k] = cb.col + plane * (uint32_t)min(cb.z0 + k, st.nz - 1);
+    f[k] = fin[id[k]];
+    pc[k] = PIO::load(p, id[k]);
+    px[k] = PIO::load(p, id[k] + dx);
+    py[k] = PIO::load(p, id[k] + dy);
+  }
+  pc[kColPlanes] = PIO::load(p, cb.col + plane * (uint32_t)min(cb.z0 + kColPlanes, st.nz - 1));
+  const bool wall_x = xc == 0 || xc == st.nx - 1;
+#pragma unroll
+  for (int k = 0; k < kColPlanes; ++k) {
+    const int z = min(cb.z0 + k, st.nz - 1);
+    const float g0 = (px[k] - pc[k]) * st.mx;
+    const float g1 = (py[k] - pc[k]) * st.my;
+    const float g2 = (pc[k + 1] - pc[k]) * st.mz;
+    float4 o = f[k];
+    o.x = o.x - g0; o.y = o.y - g1; o.z = o.z - g2;
+    if (wall_x) o.x = 0.0f;
+    if (z == 0 || z == st.nz - 1) o.z = 0.0f;
+    if (cb.y == 0) o.y = 0.0f;
+    fout[id[k]] = o;
+  }
 }
 
 // ---------------------------------------------------------------- reductions
@@ -868,7 +966,7 @@ void launch_pack_fields(int storage, const float4* flow, const void* species, co
 }
 
 void launch_divergence(const float4* flow, float* div, const FxStep& st, hipStream_t s) {
-  k_divergence<<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(flow, div, st);
+  k_divergence<<<column_grid(st), dim3(kBX, kBY), 0, s>>>(flow, div, st);
 }
 
 void launch_jacobi(int f16, const void* pin, void* pout, const float* div, const FxStep& st, hipStream_t s) {
@@ -880,9 +978,9 @@ void launch_jacobi(int f16, const void* pin, void* pout, const float* div, const
 
 void launch_project(int f16, const float4* fin, const void* p, float4* fout, const FxStep& st, hipStream_t s) {
   if (f16)
-    k_project<1><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const _Float16*)p, fout, st);
+    k_project<1><<<column_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const _Float16*)p, fout, st);
   else
-    k_project<0><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const float*)p, fout, st);
+    k_project<0><<<column_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const float*)p, fout, st);
 }
 
 static uint32_t reduce_blocks(uint32_t cells, uint32_t& per_block) {
