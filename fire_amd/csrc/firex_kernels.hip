@@ -608,16 +608,17 @@ __global__ __launch_bounds__(256) void k_pack_fields(const float4* __restrict__ 
   out[2u * cells + id] = pack_half4(react, soot, liq, display);
 }
 
-// Column kernels (divergence, projection): a 64 x 4 block owns a 64 x 4
-// column of cells over kColPlanes consecutive planes, and each thread issues
-// every load of its column before it computes, so a wave has kColPlanes
-// planes of reads in flight instead of one (these passes stream at the
-// latency of their loads otherwise).  The z neighbour of a plane is the
-// next or previous plane's value, already loaded.  Lanes past the x wall
-// and planes past the last one work on the clamped cell and store exactly
-// the value that cell's own lane or plane stores, to the same address, so
-// no store is conditional (a conditional store lets the compiler sink the
-// loads it consumes behind the branch, one plane at a time).
+// Column form (the divergence): a 64 x 4 block owns a 64 x 4 column of
+// cells over kColPlanes consecutive planes, and each thread issues every
+// load of its column before it computes, so a wave has kColPlanes planes of
+// reads in flight instead of one.  The z neighbour of a plane is the
+// previous plane's value, already loaded.  Lanes past the x wall and planes
+// past the last one work on the clamped cell and store exactly the value
+// that cell's own lane or plane stores, to the same address, so no store is
+// conditional (a conditional store lets the compiler sink the loads it
+// consumes behind the branch, one plane at a time).  The projection keeps
+// one cell per thread: its five independent loads per cell already run it
+// at the rate of a plain copy, and the column form measured no faster.
 constexpr int kColPlanes = 4;
 
 struct ColumnBlock {
@@ -709,10 +710,11 @@ __device__ inline void gradient(const typename PressureIO<P>::Word* p, const FxS
   g2 = (load_p<P>(p, st, x, y, z + 1) - c) * st.mz;
 }
 
-// One cell of the projection, every read through load_p / cidx.
 template <int P>
-__device__ inline float4 project_cell(const float4* fin, const typename PressureIO<P>::Word* p, const FxStep& st,
-                                      int x, int y, int z) {
+__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ fin, const typename PressureIO<P>::Word* __restrict__ p,
+                                                 float4* __restrict__ fout, FxStep st) {
+  int x, y, z;
+  if (!cell_coords(st, x, y, z)) return;
   const uint32_t id = cidx(st, x, y, z);
   float g0, g1, g2;
   gradient<P>(p, st, x, y, z, g0, g1, g2);
@@ -727,54 +729,7 @@ __device__ inline float4 project_cell(const float4* fin, const typename Pressure
   if (z == 0 || z == st.nz - 1) f.z = 0.0f;
   if (y == 0) f.y = 0.0f;
   if (y == st.ny - 1) f.y = fxm_max(0.0f, f.y);
-  return f;
-}
-
-// Column form of project_cell for the rows below the top row: p at x + 1
-// clamps to the cell itself at the x wall, p at y + 1 is in the grid, and p
-// at z + 1 is the next plane's centre value (the last plane's clamps to
-// itself).  The top row, which reads the row below, goes through
-// project_cell.
-template <int P>
-__global__ __launch_bounds__(256) void k_project(const float4* __restrict__ fin, const typename PressureIO<P>::Word* __restrict__ p,
-                                                 float4* __restrict__ fout, FxStep st) {
-  using PIO = PressureIO<P>;
-  ColumnBlock cb;
-  if (!column_block(st, cb)) return;
-  const int xc = min(cb.x, st.nx - 1);
-  if (cb.y == st.ny - 1) {
-    for (int k = 0; k < kColPlanes && cb.z0 + k < st.nz; ++k)
-      fout[cidx(st, xc, cb.y, cb.z0 + k)] = project_cell<P>(fin, p, st, xc, cb.y, cb.z0 + k);
-    return;
-  }
-  const uint32_t plane = (uint32_t)st.nx * (uint32_t)st.ny;
-  const uint32_t dx = xc < st.nx - 1 ? 1u : 0u, dy = (uint32_t)st.nx;
-  uint32_t id[kColPlanes];
-  float4 f[kColPlanes];
-  float pc[kColPlanes + 1], px[kColPlanes], py[kColPlanes];
-#pragma unroll
-  for (int k = 0; k < kColPlanes; ++k) {
-    id[k] = cb.col + plane * (uint32_t)min(cb.z0 + k, st.nz - 1);
-    f[k] = fin[id[k]];
-    pc[k] = PIO::load(p, id[k]);
-    px[k] = PIO::load(p, id[k] + dx);
-    py[k] = PIO::load(p, id[k] + dy);
-  }
-  pc[kColPlanes] = PIO::load(p, cb.col + plane * (uint32_t)min(cb.z0 + kColPlanes, st.nz - 1));
-  const bool wall_x = xc == 0 || xc == st.nx - 1;
-#pragma unroll
-  for (int k = 0; k < kColPlanes; ++k) {
-    const int z = min(cb.z0 + k, st.nz - 1);
-    const float g0 = (px[k] - pc[k]) * st.mx;
-    const float g1 = (py[k] - pc[k]) * st.my;
-    const float g2 = (pc[k + 1] - pc[k]) * st.mz;
-    float4 o = f[k];
-    o.x = o.x - g0; o.y = o.y - g1; o.z = o.z - g2;
-    if (wall_x) o.x = 0.0f;
-    if (z == 0 || z == st.nz - 1) o.z = 0.0f;
-    if (cb.y == 0) o.y = 0.0f;
-    fout[id[k]] = o;
-  }
+  fout[id] = f;
 }
 
 // ---------------------------------------------------------------- reductions
@@ -978,9 +933,9 @@ void launch_jacobi(int f16, const void* pin, void* pout, const float* div, const
 
 void launch_project(int f16, const float4* fin, const void* p, float4* fout, const FxStep& st, hipStream_t s) {
   if (f16)
-    k_project<1><<<column_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const _Float16*)p, fout, st);
+    k_project<1><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const _Float16*)p, fout, st);
   else
-    k_project<0><<<column_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const float*)p, fout, st);
+    k_project<0><<<cell_grid(st), dim3(kBX, kBY), 0, s>>>(fin, (const float*)p, fout, st);
 }
 
 static uint32_t reduce_blocks(uint32_t cells, uint32_t& per_block) {
