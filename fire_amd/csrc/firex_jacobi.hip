@@ -414,6 +414,13 @@ __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restri
 #pragma unroll
   for (int r = 0; r < R; ++r) vst[r] = x_out && y_out[r] ? voff[r] : 0x80000000u;
   const float wx = st.wx, wy = st.wy, wz = st.wz, inv = st.inv;
+  // Row R - 1 takes its last step as fma(nb - div, top_mul, top_add): on the
+  // top row y = ny - 1 that is (nb - div) * 0 + (+0) = +0 whatever the sign
+  // of nb - div, the forced zero of the open top; on every other row it is
+  // (nb - div) * inv + (-0), which equals the product bit for bit (x + (-0)
+  // = x for every x, zeros included).  So no per-level select is needed.
+  const float top_mul = fix_top ? 0.0f : inv;
+  const float top_add = fix_top ? 0.0f : -0.0f;
   // wave-edge neighbours; wave 0's "down" and wave W-1's "up" are halo rows
   // or the zeroed top row, so any in-bounds slot will do
   const int wdn = wave > 0 ? wave - 1 : W - 1;
@@ -489,9 +496,8 @@ __device__ __forceinline__ void ring_block(const typename PIO<P>::Word* __restri
         const float down = r > 0 ? centre[r - 1] : edge[par][s - 1][wdn][1][lane];
         const float up = r < R - 1 ? centre[r + 1] : edge[par][s - 1][wup][0][lane];
         const float nb = jacobi_sum(wx, left + right, wy, down + up, wz, below[r] + above[r]);
-        res[r] = PIO<P>::round((nb - Dv[sd][r]) * inv);
+        res[r] = PIO<P>::round(r == R - 1 ? fxm_fma(nb - Dv[sd][r], top_mul, top_add) : (nb - Dv[sd][r]) * inv);
       }
-      if (fix_top) res[R - 1] = 0.0f;
       if (s < K) {
         // level s keeps planes in slots (plane - t0) % 3
         const int qc = ((ph - s - PF) % 3 + 3) % 3;
