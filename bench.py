@@ -161,7 +161,7 @@ def droplet_profile(wl: dict, steps: int, warmup: int, plan: dict, render: bool 
 
 
 RENDER_4K = (3840, 2160, 384)  # the reference's 4K offline output and the uhd tier's ray budget
-REFERENCE_508_FRAMES_PER_S = 0.21  # README.md:64-67: one 508^3 step + 4K presentation per frame
+REFERENCE_508_FRAMES_PER_S = 0.21  # the reference's README.md:64-67: one 508^3 step + 4K presentation per frame
 
 
 def render_profile(solver, frames: int = 3) -> dict:

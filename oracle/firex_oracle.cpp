@@ -29,7 +29,7 @@
 //     particle state:                            shaders.ts:514-825,
 //                                                FireXEngine.ts:1976-2034, 2158-2171
 //
-// Two precisions are instantiated.  State<double> is the FP64 judge: every
+// Two precisions are instantiated.  State<double> is the FP64 yardstick: every
 // operation in double, libm transcendentals, true division.  State<float>
 // is the FP32 "reference step": f32 arithmetic in WGSL operation order with
 // the f32 built-in layer of fx_math.h standing in for the WGSL built-ins

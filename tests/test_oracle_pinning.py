@@ -486,7 +486,7 @@ def test_primary_solve_is_warm_started():
     np.testing.assert_array_equal(run_heavy(False), run_heavy(True))
 
 
-def test_fp32_step_tracks_fp64_judge():
+def test_fp32_step_tracks_fp64():
     dims = (32, 48, 32)
     a = orc.Oracle(dims, precision=32, pressure_iterations=16, correction_iterations=16)
     b = orc.Oracle(dims, precision=64, pressure_iterations=16, correction_iterations=16)
