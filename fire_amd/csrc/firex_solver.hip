@@ -328,9 +328,15 @@ void encode_rest(firex_solver* s, const FxStep& st) {
 
   if (s->heavy && s->compact) {
     // compact f32 fallback, FireXEngine.ts:2214-2246: shared fields, cleared
-    // before each solve (no warm start).
-    fill(s, s->pressure[0], pwords);
-    fill(s, s->pressure[1], pwords);
+    // before each solve (no warm start).  In the fused plan the first launch
+    // of a solve reads the cleared field as zeros without reading it, and
+    // every launch writes its whole output field, so the clears are only
+    // needed where a solve has no sweep; copyFlow becomes a buffer swap as in
+    // the other fused paths.
+    if (!fused || N == 0) {
+      fill(s, s->pressure[0], pwords);
+      fill(s, s->pressure[1], pwords);
+    }
     s->pressure_index = sweeps(s, 0, s->pressure, 0, fused, N, st, fused);
     timed(s, FIREX_PASS_PROJECT, [&] { launch_project(0, flow1(s), s->pressure[s->pressure_index], flow0(s), st, q); });
     timed(s, FIREX_PASS_DIVERGENCE, [&] { launch_divergence(flow0(s), s->divergence, st, q); });
@@ -340,7 +346,10 @@ void encode_rest(firex_solver* s, const FxStep& st) {
     }
     s->corr_index = sweeps(s, 0, s->pressure, 0, fused, M, st, fused);
     timed(s, FIREX_PASS_PROJECT, [&] { launch_project(0, flow0(s), s->pressure[s->corr_index], flow1(s), st, q); });
-    timed(s, FIREX_PASS_FILL_COPY, [&] { launch_copy_u128(flow0(s), flow1(s), s->cells, q); });
+    if (fused)
+      s->flow_a = 1 - s->flow_a;
+    else
+      timed(s, FIREX_PASS_FILL_COPY, [&] { launch_copy_u128(flow0(s), flow1(s), s->cells, q); });
   } else if (fused) {
     // The same passes, merged: temporally blocked sweeps, the residual
     // solve started from its known zero field, and copyFlow replaced by
@@ -545,8 +554,7 @@ int field_view(firex_solver* s, int32_t field, FieldView* v) {
     case FIREX_FIELD_FLOW_PREDICTED:
       // After a step the reference's flow[1] equals flow[0] (copyFlow); the
       // fused plan swaps instead of copying, so both read the same buffer.
-      v->ptr = s->cfg.plan == FIREX_PLAN_FUSED && !(s->heavy && s->compact) && s->frame_index > 0 ? flow0(s)
-                                                                                                  : flow1(s);
+      v->ptr = s->cfg.plan == FIREX_PLAN_FUSED && s->frame_index > 0 ? flow0(s) : flow1(s);
       v->comps = 4;
       break;
     case FIREX_FIELD_SPECIES:
