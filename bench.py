@@ -281,7 +281,8 @@ SMALL_SERVE_GRIDS = [(64, 96, 64), (96, 64, 80), (64, 128, 64)]
 
 
 def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batch: bool = False,
-                   batch_streams: int = 0, grids=None, inflight: int = 1, max_active: int = 16) -> dict:
+                   batch_streams: int = 0, grids=None, inflight: int = 1, max_active: int = 16,
+                   graphs: bool = True) -> dict:
     """Mixed-size plumes arriving as a seeded Poisson stream, served with
     continuous batching (admission by HBM budget and slots, captured steps,
     one step in flight per plume).  Wall-clock arrivals: a request is
@@ -302,7 +303,7 @@ def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batc
         FireXHipSolver(g, tier="uhd", pressure_iterations=96, correction_iterations=96, species_storage="f16",
                        plan="fused", heavy_mode="off", jacobi_autotune=True,
                        **({"jacobi_sweeps_per_launch": 4, "jacobi_ring_waves": 16} if small else {})).close()
-    srv = FireXServer(max_active=max_active, streams=streams, plan="fused", capture_graphs=True,
+    srv = FireXServer(max_active=max_active, streams=streams, plan="fused", capture_graphs=graphs,
                       max_inflight_steps=inflight,
                       jacobi_autotune=True, batch_plumes=batch, batch_streams=batch_streams)
     ids = []
@@ -339,7 +340,7 @@ def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batc
                             "max": round(stats["latency_ms_max"], 3)},
         "request_turnaround_s": {"p50": round(pct(turnaround, 0.5), 3), "p99": round(pct(turnaround, 0.99), 3)},
         "admission_wait_s": {"p50": round(pct(waits, 0.5), 3), "p99": round(pct(waits, 0.99), 3)},
-        "server": {"max_active": max_active, "streams": streams, "capture_graphs": True, "max_inflight_steps": inflight,
+        "server": {"max_active": max_active, "streams": streams, "capture_graphs": graphs, "max_inflight_steps": inflight,
                    "batch_plumes": batch, "batch_streams": batch_streams if batch else 0,
                    "memory_budget_bytes": int(stats["memory_budget_bytes"])},
         "group_steps": int(stats["group_steps"]), "grouped_plume_steps": int(stats["grouped_steps"]),
@@ -366,6 +367,10 @@ def main() -> int:
     ap.add_argument("--bandwidth-probe", action="store_true",
                     help="only time plain device reads (sum) and copies from 64 MiB to 4 GiB: the achievable rates "
                          "with and without the 256 MB last-level cache")
+    ap.add_argument("--serve-only", choices=["mixed", "small"], default=None,
+                    help="only run one serving stream (the mixed grids or the small grids), per plume and batched")
+    ap.add_argument("--no-serve-graphs", dest="serve_graphs", action="store_false",
+                    help="serve without captured step graphs (every launch eager)")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=60, help="steps timed on the plain f32 path (0 = skip)")
     ap.add_argument("--cpu-sample-steps", type=int, default=5)
@@ -428,6 +433,16 @@ def main() -> int:
             print(json.dumps(rows[-1]), flush=True)
             del x, y
         print(json.dumps({"bandwidth_probe": rows}), flush=True)
+        return 0
+
+    if args.serve_only:
+        n, rate, kw = args.serve_requests, args.serve_rate, {}
+        if args.serve_only == "small":
+            n, rate, kw = 2 * n, 10 * rate, {"grids": SMALL_SERVE_GRIDS}
+        out = {"per_plume": serving_stream(n, rate, seed=2026, graphs=args.serve_graphs, **kw),
+               "batched": serving_stream(n, rate, seed=2026, streams=3 if kw else 4, batch=True,
+                                         batch_streams=3 if kw else 2, graphs=args.serve_graphs, **kw)}
+        print(json.dumps({"serving_stream": args.serve_only, **out}), flush=True)
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
