@@ -813,14 +813,19 @@ struct TuneEntry {
 std::mutex g_tune_mutex;
 std::map<TuneKey, TuneEntry> g_tune_cache;
 
-// z chunk sizes worth timing: about 10% apart, giving 200..3000 blocks.
+// z chunk sizes worth timing: about 10% apart, giving 200..3000 blocks.  A
+// grid too small for 200 blocks at chunks of 4 planes or more is searched
+// from 16 blocks, down to 1-plane chunks: there more blocks pay for the halo
+// planes a short chunk adds (64 x 96 x 64 at depth 4: 2.1 us per sweep on
+// 2-plane chunks against 3.3 us on the 6-plane chunks of the 200-block rule).
 std::vector<int> chunk_candidates(int nz, long tiles) {
   std::vector<int> out;
+  const long lo = tiles * ((nz + 3) / 4) < 200 ? 16 : 200;
   for (int n = 1; n <= nz; ++n) {
     const int c = (nz + n - 1) / n;
-    if (c < 4) break;
+    if (c < (lo < 200 ? 1 : 4)) break;
     const long blocks = tiles * ((nz + c - 1) / c);
-    if (blocks < 200 || blocks > 3000) continue;
+    if (blocks < lo || blocks > 3000) continue;
     if (!out.empty() && c * 10 > out.back() * 9) continue;
     out.push_back(c);
   }
@@ -928,8 +933,14 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
   std::vector<std::pair<float, int>> score;
   float us = 0;
   e = time_plan(cands[0], 12, &us);
+  // Repetitions scaled so that a measurement spans at least about 100 us
+  // (quick pass) and 400 us (final pass): on small grids a launch takes a
+  // few microseconds and three of them do not rank the candidates.
+  const float launch_us = us * (float)cands[0].sweeps_per_launch;
+  const int quick = std::max(3, std::min(32, (int)(100.0f / std::max(launch_us, 1.0f)) + 1));
+  const int final_reps = std::max(8, std::min(128, (int)(400.0f / std::max(launch_us, 1.0f)) + 1));
   for (size_t i = 0; i < cands.size() && e == hipSuccess; ++i) {
-    e = time_plan(cands[i], 3, &us);
+    e = time_plan(cands[i], quick, &us);
     score.emplace_back(us, (int)i);
   }
   if (e == hipSuccess && chunk_z <= 0) {
@@ -950,7 +961,7 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
       }
     }
     for (size_t i = first; i < cands.size() && e == hipSuccess; ++i) {
-      e = time_plan(cands[i], 3, &us);
+      e = time_plan(cands[i], quick, &us);
       score.emplace_back(us, (int)i);
     }
   }
@@ -959,8 +970,8 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
     const size_t top = std::min<size_t>(score.size(), 6);
     for (size_t j = 0; j < top && e == hipSuccess; ++j) {
       float a = 0, b = 0;
-      e = time_plan(cands[score[j].second], 8, &a);
-      if (e == hipSuccess) e = time_plan(cands[score[j].second], 8, &b);
+      e = time_plan(cands[score[j].second], final_reps, &a);
+      if (e == hipSuccess) e = time_plan(cands[score[j].second], final_reps, &b);
       score[j].first = std::min(a, b);
     }
     std::sort(score.begin(), score.begin() + top);

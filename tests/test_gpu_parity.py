@@ -273,6 +273,24 @@ def test_ring_kernel_sixteen_wave_layouts_bitwise(dims, sweeps, chunk):
     g.close()
 
 
+@pytest.mark.parametrize("dims,sweeps,chunk,waves,heavy", [
+    ((64, 96, 64), 4, 1, 16, "off"), ((64, 96, 64), 4, 2, 16, "off"), ((64, 96, 64), 4, 3, 16, "f16"),
+    ((96, 64, 80), 5, 2, 8, "off"), ((96, 64, 80), 5, 1, 8, "off"), ((64, 128, 64), 3, 2, 8, "f16"),
+    ((70, 40, 23), 2, 1, 16, "off")])
+def test_ring_kernel_short_chunks_bitwise(dims, sweeps, chunk, waves, heavy):
+    """Chunks of 1-3 planes, which the tuner picks on small grids (each
+    block streams its chunk plus K halo planes on either side)."""
+    g, o = _ring_pair(dims, (9, 7), sweeps, heavy=heavy, chunk=chunk, waves=waves)
+    info = g.info["jacobi"]
+    assert info["chunk_z"] == chunk and info["sweeps_per_launch"] == sweeps and info["waves"] == waves
+    o.step(DT, 1)
+    g.step(DT, 1)
+    assert_fields_equal(g, o, f"ring short chunk {dims} K={sweeps} chunk {chunk} waves {waves} {heavy}",
+                        names=["flow", "species", "divergence", "reaction"])
+    np.testing.assert_array_equal(_bits(g.read_field("pressure")), _bits(o.read(orc.FIELD_PRESSURE)))
+    g.close()
+
+
 @pytest.mark.parametrize("height,dims,sweeps,chunk,prefetch", [
     (48, (80, 100, 20), 3, 7, False), (48, (80, 100, 20), 2, 5, True), (48, (64, 48, 12), 5, 0, False),
     (64, (64, 130, 17), 3, 6, False), (64, (64, 130, 17), 3, 6, True), (64, (128, 96, 40), 4, 9, False),
