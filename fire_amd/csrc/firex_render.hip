@@ -170,7 +170,7 @@ __device__ inline float hash21(float vx, float vy) {
 
 // blackbody, qualityShaders.ts:301-309
 __device__ V3 blackbody(float temperature) {
-  const float t = fxm_clamp((temperature - 720.0f) / 1900.0f, 0.0f, 1.0f);
+  const float t = fxm_clamp((temperature - 720.0f) * (1.0f / 1900.0f), 0.0f, 1.0f);
   const V3 red = v3(1.0f, 0.035f, 0.002f), orange = v3(1.0f, 0.24f, 0.008f);
   const V3 gold = v3(1.0f, 0.68f, 0.12f), white = v3(1.0f, 0.94f, 0.72f);
   return mix3(mix3(red, orange, fxm_smoothstep(0.0f, 0.32f, t)), mix3(gold, white, fxm_smoothstep(0.58f, 1.0f, t)),
@@ -327,6 +327,10 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
         VIEW == kRuntimeView ? (uint32_t)fxm_clamp(fxm_rint(r.view_mode), 0.0f, 7.0f) : VIEW;
     const bool beauty = view_mode == 0u;
     const float front_def = fxm_clamp(r.front_definition, 0.5f, 2.5f);
+    // Divisions by the extents and by a sample's extinction are multiplies
+    // by the f32 reciprocal (one of the results WGSL's 2.5 ULP f32 division
+    // admits), as in simulate; the FP32 oracle makes the same ones.
+    const V3 inv_extent = v3(1.0f / extent.x, 1.0f / extent.y, 1.0f / extent.z);
     for (uint32_t step = 0; step < 384u; ++step) {
       if (step >= step_count || distance >= far_t || transmittance < 0.006f) break;
       const V3 position = add(eye, mul(dir, distance));
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
         const float detail_b = fxm_sin(dot3(detail_position, v3(-21.0f, 29.0f, 17.0f)) - r.time * 4.7f);
         float optical_detail = 0.90f + 0.20f * detail_a * detail_b;
         if (cinematic > 0.5f && reaction_front > 0.002f) {
-          const V3 normalized = divv(detail_position, extent);
+          const V3 normalized = mulv(detail_position, inv_extent);
           const float requested = r.detail_target;
           const float explicit_detail = fxm_clamp(requested, 64.0f, 1024.0f);
           const V3 detail_cells = requested >= 1.0f ? v3(explicit_detail, explicit_detail, explicit_detail)
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
         const float standard_emission =
             flame_density * (1.20f + 1.80f * reaction_front) * (0.90f + 0.30f * r.chem_z) + hot_soot * 0.25f;
         const V3 emissive = mul(flame_color, fxm_mix(standard_emission, cinematic_emission, cinematic));
-        const float base_zone = 1.0f - fxm_smoothstep(0.16f, 0.48f, position.y / r.ey);
+        const float base_zone = 1.0f - fxm_smoothstep(0.16f, 0.48f, position.y * inv_extent.y);
         const V3 blue = mul(mul(mul(v3(0.018f, 0.12f, 0.74f), flame_density), 1.0f - richness),
                             fxm_mix(0.48f, base_zone * 0.34f, cinematic));
         const V3 to_fire = sub(v3(r.ex * 0.5f, 0.08f, r.ez * 0.5f), position);
@@ -439,7 +443,8 @@ __global__ __launch_bounds__(256) void k_render_volume(const Half4* __restrict__
         const V3 scattered = mix3(standard_scatter, cinematic_scatter, cinematic);
         const float inv_ext = fxm_max(extinction, 0.0001f);
         const V3 source = add(add(emissive, blue), scattered);
-        const V3 source_radiance = v3(source.x / inv_ext, source.y / inv_ext, source.z / inv_ext);
+        const float rcp_ext = 1.0f / inv_ext;
+        const V3 source_radiance = v3(source.x * rcp_ext, source.y * rcp_ext, source.z * rcp_ext);
         radiance = add(radiance, mul(source_radiance, transmittance * alpha));
         transmittance *= 1.0f - alpha;
       } else {

@@ -1055,7 +1055,7 @@ template <typename T> struct State {
     }
 
     static F3 blackbody(float temp) {
-      float t = fxm_clamp((temp - 720.0f) / 1900.0f, 0.0f, 1.0f);
+      float t = fxm_clamp((temp - 720.0f) * (1.0f / 1900.0f), 0.0f, 1.0f);
       F3 red = f3(1.0f, 0.035f, 0.002f), orange = f3(1.0f, 0.24f, 0.008f);
       F3 gold = f3(1.0f, 0.68f, 0.12f), white = f3(1.0f, 0.94f, 0.72f);
       return lerp3(lerp3(red, orange, fxm_smoothstep(0.0f, 0.32f, t)), lerp3(gold, white, fxm_smoothstep(0.58f, 1.0f, t)),
@@ -1134,6 +1134,9 @@ template <typename T> struct State {
       const float uvx = ((float)px + 0.5f) / (float)width, uvy = 1.0f - ((float)py + 0.5f) / (float)height;
       const float sx = uvx * 2.0f - 1.0f, sy = uvy * 2.0f - 1.0f;
       const F3 E = f3(ext(0), ext(1), ext(2));
+      // divisions by the extents as multiplies by their f32 reciprocals, as
+      // the device kernel makes them
+      const F3 iE = f3(1.0f / E.x, 1.0f / E.y, 1.0f / E.z);
       const F3 target = f3(E.x * 0.5f, E.y * 0.40f, E.z * 0.5f);
       const float yaw = u[16], pitch = u[17];
       const F3 orbit = f3(fxm_sin(yaw) * fxm_cos(pitch), fxm_sin(pitch), fxm_cos(yaw) * fxm_cos(pitch));
@@ -1199,7 +1202,7 @@ template <typename T> struct State {
           const float db = fxm_sin(dot(dp, f3(-21.0f, 29.0f, 17.0f)) - u[5] * 4.7f);
           float optical = 0.90f + 0.20f * da * db;
           if (cin > 0.5f && front > 0.002f) {
-            const F3 nrm = f3(dp.x / E.x, dp.y / E.y, dp.z / E.z);
+            const F3 nrm = f3(dp.x * iE.x, dp.y * iE.y, dp.z * iE.z);
             const float req = u[43];
             const float expl = fxm_clamp(req, 64.0f, 1024.0f);
             const F3 cells = req >= 1.0f ? f3(expl, expl, expl) : f3(u[0], u[1], u[2]);
@@ -1249,7 +1252,7 @@ template <typename T> struct State {
           const float eCin = flame * (1.35f + 3.65f * front) * (0.92f + 0.32f * u[10]) + hot * 0.08f;
           const float eStd = flame * (1.20f + 1.80f * front) * (0.90f + 0.30f * u[10]) + hot * 0.25f;
           const F3 emissive = scale(color, fxm_mix(eStd, eCin, cin));
-          const float baseZone = 1.0f - fxm_smoothstep(0.16f, 0.48f, pos.y / E.y);
+          const float baseZone = 1.0f - fxm_smoothstep(0.16f, 0.48f, pos.y * iE.y);
           const F3 blue = scale(scale(scale(f3(0.018f, 0.12f, 0.74f), flame), 1.0f - rich),
                                 fxm_mix(0.48f, baseZone * 0.34f, cin));
           const F3 toFire = minus(f3(E.x * 0.5f, 0.08f, E.z * 0.5f), pos);
@@ -1261,8 +1264,9 @@ template <typename T> struct State {
                                scale(plus(scale(amb, 0.55f), scale(fire, 0.05f)), sv));
           const F3 scattered = lerp3(sStd, sCin, cin);
           const float m = fxm_max(extinction, 0.0001f);
+          const float im = 1.0f / m;
           const F3 src = plus(plus(emissive, blue), scattered);
-          rad = plus(rad, scale(f3(src.x / m, src.y / m, src.z / m), trans * alpha));
+          rad = plus(rad, scale(f3(src.x * im, src.y * im, src.z * im), trans * alpha));
           trans *= 1.0f - alpha;
         } else {
           const V4<float> fv = diagnostic(view, fl, ch, me);
