@@ -289,8 +289,8 @@ def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batc
     submitted when its arrival time has passed.  batch: plumes of at most
     4 Mi cells go to batch_streams streams and those on one stream step
     together, each Jacobi launch one launch over all of them (16-wave ring
-    plan of depth 4, chunk height autotuned per grid); larger plumes step
-    alone on the other streams."""
+    plan of depth 4 on 8-plane z chunks); larger plumes step alone on the
+    other streams."""
     from fire_amd import FireXHipSolver, seeded_emitters
     from fire_amd.serve import FireXServer, make_request, poisson_stream
 
@@ -302,7 +302,8 @@ def serving_stream(requests: int, rate: float, seed: int, streams: int = 4, batc
         small = batch and g[0] * g[1] * g[2] <= 4 << 20
         FireXHipSolver(g, tier="uhd", pressure_iterations=96, correction_iterations=96, species_storage="f16",
                        plan="fused", heavy_mode="off", jacobi_autotune=True,
-                       **({"jacobi_sweeps_per_launch": 4, "jacobi_ring_waves": 16} if small else {})).close()
+                       **({"jacobi_sweeps_per_launch": 4, "jacobi_ring_waves": 16, "jacobi_chunk_z": 8}
+                          if small else {})).close()
     srv = FireXServer(max_active=max_active, streams=streams, plan="fused", capture_graphs=graphs,
                       max_inflight_steps=inflight,
                       jacobi_autotune=True, batch_plumes=batch, batch_streams=batch_streams)

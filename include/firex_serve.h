@@ -43,8 +43,9 @@ typedef struct firex_server_config {
                                     share iteration counts, pressure storage and dt step
                                     together: each Jacobi launch of their step is one ring
                                     launch over all their tiles (16-wave ring plan of depth
-                                    4; grids the ring kernel cannot run step alone).  Larger
-                                    plumes step alone on the other streams.                   */
+                                    4 on 8-plane z chunks; grids the ring kernel cannot run
+                                    step alone).  Larger plumes step alone on the other
+                                    streams.                                                  */
   int32_t batch_max_cells;       /* 0 = 4 Mi cells                                            */
   int32_t batch_streams;         /* 0 = half of the streams (at least 1)                      */
   int32_t reserved[1];
