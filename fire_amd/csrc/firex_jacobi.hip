@@ -911,7 +911,7 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
   hipEvent_t ev[2];
   if ((e = hipEventCreate(&ev[0])) != hipSuccess) return e;
   if ((e = hipEventCreate(&ev[1])) != hipSuccess) {
-    hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[0]);
     return e;
   }
   auto time_plan = [&](const JacobiPlan& p, int reps, float* us) -> hipError_t {
@@ -976,8 +976,8 @@ int tune_jacobi_plan(const int32_t dims[3], int f16, int32_t sweeps_per_launch, 
     }
     std::sort(score.begin(), score.begin() + top);
   }
-  hipEventDestroy(ev[0]);
-  hipEventDestroy(ev[1]);
+  (void)hipEventDestroy(ev[0]);  // teardown: a failure here changes nothing
+  (void)hipEventDestroy(ev[1]);
   if (e != hipSuccess) return e;
   const TuneEntry best{cands[score[0].second], score[0].first};
   {

@@ -232,7 +232,10 @@ int admit(firex_server* s) {
 int note_issued(firex_server* s, Request& r, double t) {
   hipEvent_t stop = take_event(s);
   if (!stop) return set_error(FIREX_ERR_DEVICE, "hipEventCreate failed");
-  hipEventRecord(stop, solver_stream(r.solver));
+  if (hipEventRecord(stop, solver_stream(r.solver)) != hipSuccess) {
+    s->pool.push_back(stop);
+    return set_error(FIREX_ERR_DEVICE, "hipEventRecord failed for a step's completion event");
+  }
   r.inflight.push_back({stop, t});
   r.steps_issued += 1;
   return FIREX_OK;
@@ -314,7 +317,11 @@ int issue(firex_server* s) {
         s->pool.push_back(stop);
         return -1;  // the solver's message stays in firex_last_error()
       }
-      hipEventRecord(stop, solver_stream(r.solver));
+      if (hipEventRecord(stop, solver_stream(r.solver)) != hipSuccess) {
+        s->pool.push_back(stop);
+        set_error(FIREX_ERR_DEVICE, "hipEventRecord failed for a step's completion event");
+        return -1;
+      }
       r.inflight.push_back({stop, t});
       r.steps_issued += 1;
       ++issued;
