@@ -87,6 +87,7 @@ struct firex_solver {
   struct TimedLaunch {
     int pass;
     hipEvent_t start, stop;
+    int launches;
   };
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
@@ -231,7 +232,11 @@ int collect_timing(firex_solver* s) {
   FX_HIP(s, hipStreamSynchronize(s->stream), "timing synchronize");
   for (const auto& t : s->pending) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) s->totals.milliseconds[t.pass] += ms;
+    // a pair whose time cannot be read counts neither its time nor its launches
+    if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+      s->totals.milliseconds[t.pass] += ms;
+      s->totals.launches[t.pass] += t.launches;
+    }
     s->event_pool.push_back(t.start);
     s->event_pool.push_back(t.stop);
   }
@@ -251,14 +256,19 @@ void timed(firex_solver* s, int pass, F&& launch, int launches = 1) {
   }
   hipEvent_t a = take_event(s), b = take_event(s);
   if (!a || !b) {
+    if (a) s->event_pool.push_back(a);
+    if (b) s->event_pool.push_back(b);
     launch();
     return;
   }
-  hipEventRecord(a, s->stream);
+  const bool started = hipEventRecord(a, s->stream) == hipSuccess;
   launch();
-  hipEventRecord(b, s->stream);
-  s->pending.push_back({pass, a, b});
-  s->totals.launches[pass] += launches;
+  if (!started || hipEventRecord(b, s->stream) != hipSuccess) {
+    s->event_pool.push_back(a);  // untimed: the launches still ran
+    s->event_pool.push_back(b);
+    return;
+  }
+  s->pending.push_back({pass, a, b, launches});
 }
 
 float4* flow0(firex_solver* s) { return s->flow[s->flow_a]; }
