@@ -376,7 +376,7 @@ int firex_server_create(const firex_server_config* cfg, firex_server** out) {
 
 void firex_server_destroy(firex_server* s) {
   if (!s) return;
-  hipSetDevice(s->cfg.device);
+  (void)hipSetDevice(s->cfg.device);  // teardown: best effort
   for (hipStream_t st : s->streams) hipStreamSynchronize(st);
   for (auto& kv : s->requests) {
     for (auto& f : kv.second.inflight) hipEventDestroy(f.stop);
@@ -413,7 +413,7 @@ int firex_server_submit(firex_server* s, const firex_request* req, int64_t* id) 
 
 int firex_server_pump(firex_server* s, int32_t* busy) {
   if (!s) return set_error(FIREX_ERR_INVALID, "server is null");
-  hipSetDevice(s->cfg.device);
+  if (hipSetDevice(s->cfg.device) != hipSuccess) return set_error(FIREX_ERR_DEVICE, "hipSetDevice failed");
   harvest(s);
   admit(s);
   if (issue(s) < 0) return FIREX_ERR_DEVICE;
@@ -425,7 +425,7 @@ int firex_server_pump(firex_server* s, int32_t* busy) {
 
 int firex_server_drain(firex_server* s) {
   if (!s) return set_error(FIREX_ERR_INVALID, "server is null");
-  hipSetDevice(s->cfg.device);
+  if (hipSetDevice(s->cfg.device) != hipSuccess) return set_error(FIREX_ERR_DEVICE, "hipSetDevice failed");
   for (;;) {
     int h = harvest(s);
     int a = admit(s);

@@ -857,7 +857,7 @@ int solver_step_group(firex_solver* const* g, int32_t n, double dt) {
     for (int j = 0; j < i; ++j)
       if (g[j] == g[i]) return set_error(FIREX_ERR_INVALID, "a plume appears twice in the group");
   }
-  hipSetDevice(g[0]->cfg.device);
+  FX_HIP(g[0], hipSetDevice(g[0]->cfg.device), "hipSetDevice");
   if (n == 1) return step_with_dt(g[0], dt);
   const hipStream_t q = g[0]->stream;
   const int N = g[0]->cfg.pressure_iterations, M = g[0]->cfg.correction_iterations;
@@ -919,7 +919,7 @@ int firex_create(const firex_config* cfg, firex_solver** out) { return fx::solve
 
 void firex_destroy(firex_solver* s) {
   if (!s) return;
-  hipSetDevice(s->cfg.device);
+  (void)hipSetDevice(s->cfg.device);  // teardown: best effort
   if (!s->lost) hipStreamSynchronize(s->stream);
   for (const auto& t : s->pending) {
     hipEventDestroy(t.start);
@@ -995,7 +995,7 @@ int firex_reconfigure(firex_solver* s, const firex_config* config, uint64_t memo
   uint64_t need = 0;
   rc = solver_plan(&next, &heavy, &f16p, &compact, &need);
   if (rc) return rc;
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   FX_HIP(s, hipStreamSynchronize(s->stream), "reconfigure");
   rc = collect_timing(s);
   if (rc) return rc;
@@ -1059,7 +1059,7 @@ int firex_reconfigure(firex_solver* s, const firex_config* config, uint64_t memo
 int firex_reset(firex_solver* s) {
   int rc = check_usable(s);
   if (rc) return rc;
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   s->clock.reset();
   s->simulation_time = 0;
   s->frame_index = 0;
@@ -1114,7 +1114,7 @@ int firex_step(firex_solver* s, double dt, int32_t count) {
   if (rc) return rc;
   if (!isfinite(dt) || dt < 0) return set_error(FIREX_ERR_INVALID, "step dt must be finite and non-negative");
   if (count < 0) return set_error(FIREX_ERR_INVALID, "count must be non-negative");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   for (int32_t i = 0; i < count; ++i) {
     rc = step_with_dt(s, dt);
     if (rc) return rc;
@@ -1127,7 +1127,7 @@ int firex_render_offline_frame(firex_solver* s, double delta, const volatile int
   if (rc) return rc;
   if (!isfinite(delta) || delta < 0)
     return set_error(FIREX_ERR_INVALID, "Offline frame delta must be finite and non-negative.");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   s->clock.reset();
   int32_t count = 0;
   rc = firex_split_offline_delta(delta, s->cfg.time_step, nullptr, 0, &count);
@@ -1144,7 +1144,7 @@ int firex_render_offline_frame(firex_solver* s, double delta, const volatile int
 int firex_frame(firex_solver* s, double delta, int32_t* steps_taken) {
   int rc = check_usable(s);
   if (rc) return rc;
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   int32_t n = s->clock.advance(delta);
   if (steps_taken) *steps_taken = n;
   if (n == 0) return FIREX_OK;
@@ -1164,7 +1164,7 @@ int firex_frame(firex_solver* s, double delta, int32_t* steps_taken) {
 int firex_sync(firex_solver* s) {
   int rc = check_usable(s);
   if (rc) return rc;
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   FX_HIP(s, hipStreamSynchronize(s->stream), "synchronize");
   return FIREX_OK;
 }
@@ -1173,7 +1173,7 @@ int firex_read_field(firex_solver* s, int32_t field, void* dst, size_t bytes, in
   int rc = check_usable(s);
   if (rc) return rc;
   if (!dst) return set_error(FIREX_ERR_INVALID, "dst is null");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   FieldView v;
   rc = field_view(s, field, &v);
   if (rc) return rc;
@@ -1245,7 +1245,7 @@ int firex_field_checksum(firex_solver* s, int32_t field, uint64_t* out) {
   int rc = check_usable(s);
   if (rc) return rc;
   if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   FieldView v;
   rc = field_view(s, field, &v);
   if (rc) return rc;
@@ -1264,7 +1264,7 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
   int rc = check_usable(s);
   if (rc) return rc;
   if (!srcp) return set_error(FIREX_ERR_INVALID, "src is null");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   uint64_t c = s->cells;
   void* dst = nullptr;
   int comps = 1, kind = 0;
@@ -1322,7 +1322,7 @@ int firex_write_field(firex_solver* s, int32_t field, const void* srcp, size_t b
 int firex_pack_fields(firex_solver* s) {
   int rc = check_usable(s);
   if (rc) return rc;
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   if (!s->packed) {
     const uint64_t bytes = (uint64_t)s->cells * 24;
     if (s->cfg.memory_budget_bytes && s->bytes + bytes > s->cfg.memory_budget_bytes)
@@ -1514,7 +1514,7 @@ int firex_reduce(firex_solver* s, int32_t which, double* out) {
   int rc = check_usable(s);
   if (rc) return rc;
   if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   double* result = s->scratch + kReducePartials;
   rc = enqueue_reduction(s, which, s->scratch, result);
   if (rc) return rc;
@@ -1566,7 +1566,7 @@ int firex_reduce_noalloc(firex_solver* s, int32_t which, double* device_out) {
   if (reduce_block_count(s->cells) > kReducePartials)
     return set_error(FIREX_ERR_INVALID, "grid of " + std::to_string(s->cells) + " cells needs more than " +
                                             std::to_string(kReducePartials) + " reduction partials");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   double* partial = nullptr;
   rc = noalloc_scratch(s->cfg.device, &partial);
   if (rc) return rc;
@@ -1580,7 +1580,7 @@ int firex_reduce_noalloc(firex_solver* s, int32_t which, double* device_out) {
 int firex_set_pass_timing(firex_solver* s, int32_t enabled) {
   int rc = check_usable(s);
   if (rc) return rc;
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   rc = collect_timing(s);
   if (rc) return rc;
   memset(&s->totals, 0, sizeof s->totals);
@@ -1592,7 +1592,7 @@ int firex_get_pass_timing(firex_solver* s, firex_pass_timing* out) {
   int rc = check_usable(s);
   if (rc) return rc;
   if (!out) return set_error(FIREX_ERR_INVALID, "out is null");
-  hipSetDevice(s->cfg.device);
+  FX_HIP(s, hipSetDevice(s->cfg.device), "hipSetDevice");
   rc = collect_timing(s);
   if (rc) return rc;
   *out = s->totals;
