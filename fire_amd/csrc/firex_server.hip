@@ -356,7 +356,7 @@ int firex_server_create(const firex_server_config* cfg, firex_server** out) {
   for (int i = 0; i < cfg->streams; ++i) {
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-      for (hipStream_t x : s->streams) hipStreamDestroy(x);
+      for (hipStream_t x : s->streams) (void)hipStreamDestroy(x);
       delete s;
       return set_error(FIREX_ERR_DEVICE, "hipStreamCreate failed");
     }
@@ -365,7 +365,8 @@ int firex_server_create(const firex_server_config* cfg, firex_server** out) {
   s->t0 = Clock::now();
   if (hipEventCreate(&s->epoch) != hipSuccess || hipEventRecord(s->epoch, s->streams[0]) != hipSuccess ||
       hipEventSynchronize(s->epoch) != hipSuccess) {
-    for (hipStream_t x : s->streams) hipStreamDestroy(x);
+    if (s->epoch) (void)hipEventDestroy(s->epoch);
+    for (hipStream_t x : s->streams) (void)hipStreamDestroy(x);
     delete s;
     return set_error(FIREX_ERR_DEVICE, "could not anchor the device clock");
   }
@@ -377,14 +378,14 @@ int firex_server_create(const firex_server_config* cfg, firex_server** out) {
 void firex_server_destroy(firex_server* s) {
   if (!s) return;
   (void)hipSetDevice(s->cfg.device);  // teardown: best effort
-  for (hipStream_t st : s->streams) hipStreamSynchronize(st);
+  for (hipStream_t st : s->streams) (void)hipStreamSynchronize(st);  // teardown: best effort
   for (auto& kv : s->requests) {
-    for (auto& f : kv.second.inflight) hipEventDestroy(f.stop);
+    for (auto& f : kv.second.inflight) (void)hipEventDestroy(f.stop);
     if (kv.second.solver) firex_destroy(kv.second.solver);
   }
-  for (hipEvent_t e : s->pool) hipEventDestroy(e);
-  if (s->epoch) hipEventDestroy(s->epoch);
-  for (hipStream_t st : s->streams) hipStreamDestroy(st);
+  for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
+  if (s->epoch) (void)hipEventDestroy(s->epoch);
+  for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
   delete s;
 }
 
@@ -441,7 +442,8 @@ int firex_server_drain(firex_server* s) {
       for (int64_t id : s->active) {
         Request& r = s->requests[id];
         if (!r.inflight.empty()) {
-          hipEventSynchronize(r.inflight.front().stop);
+          if (hipEventSynchronize(r.inflight.front().stop) != hipSuccess)
+            return set_error(FIREX_ERR_DEVICE, "waiting for a step in flight failed");
           break;
         }
       }

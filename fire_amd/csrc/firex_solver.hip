@@ -135,11 +135,11 @@ int check_launch(firex_solver* s, const char* what) {
 }
 
 void free_all(firex_solver* s) {
-  for (void* p : s->allocations) hipFree(p);
+  for (void* p : s->allocations) (void)hipFree(p);  // teardown: best effort
   s->allocations.clear();
-  if (s->particle_host) hipHostFree(s->particle_host);
+  if (s->particle_host) (void)hipHostFree(s->particle_host);
   s->particle_host = nullptr;
-  if (s->particle_upload) hipEventDestroy(s->particle_upload);
+  if (s->particle_upload) (void)hipEventDestroy(s->particle_upload);
   s->particle_upload = nullptr;
 }
 
@@ -397,7 +397,7 @@ uint64_t graph_key(const FxStep& st) {
 }
 
 void drop_graphs(firex_solver* s) {
-  for (auto& g : s->graphs) hipGraphExecDestroy(g.exec);
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.exec);  // best effort
   s->graphs.clear();
 }
 
@@ -433,7 +433,7 @@ bool launch_rest_graph(firex_solver* s, const FxStep& st) {
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-  if (graph) hipGraphDestroy(graph);
+  if (graph) (void)hipGraphDestroy(graph);  // the executable graph, if any, is independent of it
   if (e != hipSuccess) {
     (void)hipGetLastError();
     s->flow_a = g.flow_a;  // nothing ran: restore the state for the eager path
@@ -769,9 +769,9 @@ int solver_create(const firex_config* cfg, hipStream_t stream, firex_solver** ou
   }
   if (rc) {
     std::string msg = firex_last_error();
-    if (!s->own_stream) hipStreamSynchronize(s->stream);
+    if (!s->own_stream) (void)hipStreamSynchronize(s->stream);  // cleanup after a failure: best effort
     free_all(s);
-    if (s->own_stream) hipStreamDestroy(s->stream);
+    if (s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return set_error(rc, msg);
   }
@@ -920,15 +920,15 @@ int firex_create(const firex_config* cfg, firex_solver** out) { return fx::solve
 void firex_destroy(firex_solver* s) {
   if (!s) return;
   (void)hipSetDevice(s->cfg.device);  // teardown: best effort
-  if (!s->lost) hipStreamSynchronize(s->stream);
+  if (!s->lost) (void)hipStreamSynchronize(s->stream);
   for (const auto& t : s->pending) {
-    hipEventDestroy(t.start);
-    hipEventDestroy(t.stop);
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
   }
-  for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
+  for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
   drop_graphs(s);
   free_all(s);
-  if (s->own_stream) hipStreamDestroy(s->stream);
+  if (s->own_stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -1403,7 +1403,7 @@ int firex_render_volume(firex_solver* s, const firex_render_config* rc) {
     if (s->scene) {
       FX_HIP(s, hipStreamSynchronize(s->stream), "scene image resize");
       s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), s->scene));
-      hipFree(s->scene);
+      (void)hipFree(s->scene);  // no longer referenced: a failed free only leaks it
     }
     s->bytes += image_bytes - s->scene_bytes;
     s->scene = p;
@@ -1450,7 +1450,7 @@ int firex_render_frame(firex_solver* s, const firex_render_config* rc) {
     if (s->post) {
       FX_HIP(s, hipStreamSynchronize(s->stream), "presentation resize");
       s->allocations.erase(std::find(s->allocations.begin(), s->allocations.end(), s->post));
-      hipFree(s->post);
+      (void)hipFree(s->post);  // no longer referenced: a failed free only leaks it
     }
     s->bytes += need - s->post_bytes;
     s->post = p;
